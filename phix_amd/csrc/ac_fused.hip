@@ -1,0 +1,244 @@
+// ac_fused.hip -- fused multi-order-parameter Allen-Cahn RK4 stage kernel.
+//
+// The reference runs a VectorEquation as one Equation per component, each a
+// memset plus one launch per Term and a device sync (src/equation/
+// VectorEquation.cu:33-40), with pw() limited to three fields
+// (include/equation/TermPW.inl:212-268), so an N-field coupling costs ~N^2
+// launches per stage. Here one kernel evaluates every component's RHS for a
+// tile and applies the RK4 stage epilogue to all of them:
+//   for each field: halo tile -> LDS (double buffered), lap and centre value
+//                   kept in registers;
+//   then per field: rhs_i = L*kappa*lap_i + L*(e_i - e_i^3)
+//                           + sum_{j!=i} (-2*L*gamma)*(e_i*e_j*e_j)
+//                   in that order (the order of the VectorEquation terms),
+//                   and the stage update.
+// BCs are index maps of the halo load (stage states are BC-filled in the
+// reference, src/solver/VectorSolver.cu:149-179).
+#pragma clang fp contract(off)
+
+#include "common.hpp"
+
+namespace phx {
+
+constexpr int AC_MAXF = 8;
+
+template <typename T>
+struct ACArgs {
+    const T* u[AC_MAXF];
+    const T* c[AC_MAXF];
+    T* acc[AC_MAXF];
+    T* out[AC_MAXF];
+    Layout L;
+    AxisBC<T> bx, by;
+    T idx2, idy2;
+    T cl, cp, cg;   // L*kappa, L, -2*L*gamma
+    T a, dt6, dt3;
+    int stage;
+};
+
+template <typename T, int DIM>
+__device__ __forceinline__ T ac_load(const T* __restrict__ u, const ACArgs<T>& A, int px, int py) {
+    int qy = 0;
+    if (DIM >= 2) {
+        qy = map_in(py, A.by);
+        if (qy < 0) return fixed_value(py, A.by);
+    }
+    int qx = map_in(px, A.bx);
+    if (qx < 0) return fixed_value(px, A.bx);
+    return u[A.L.at(qx, qy, 0)];
+}
+
+template <typename T, int DIM, int NF, int TX, int TY, bool EXACT>
+__global__ __launch_bounds__(256) void k_ac_stage_2d(ACArgs<T> A) {
+    constexpr int UX = TX + 2, UY = TY + 2;
+    constexpr int R = (TX * TY) / 256;   // outputs per thread
+    __shared__ T U[2][UY][UX];
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    const bool interior = (x0 >= 1) && (x0 + TX < A.L.n0) && (DIM < 2 || ((y0 >= 1) && (y0 + TY < A.L.n1)));
+
+    T e[NF][R], lp[NF][R];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        const int b = f & 1;
+        if (interior) {
+            const T* base = A.u[f] + A.L.at(x0 - 1, y0 - 1, 0);
+            for (int t = tid; t < UX * UY; t += 256) {
+                int ly = t / UX, lx = t - ly * UX;
+                U[b][ly][lx] = base[(long long)ly * A.L.sx + lx];
+            }
+        } else {
+            for (int t = tid; t < UX * UY; t += 256) {
+                int ly = t / UX, lx = t - ly * UX;
+                U[b][ly][lx] = ac_load<T, DIM>(A.u[f], A, x0 - 1 + lx, DIM >= 2 ? y0 - 1 + ly : 0);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            int t = tid + r * 256;
+            int oy = t / TX, ox = t - oy * TX;
+            T cc = U[b][oy + 1][ox + 1];
+            T val = (U[b][oy + 1][ox + 2] - (T)2 * cc + U[b][oy + 1][ox]) * A.idx2;
+            if (DIM >= 2) val += (U[b][oy + 2][ox + 1] - (T)2 * cc + U[b][oy][ox + 1]) * A.idy2;
+            e[f][r] = cc;
+            lp[f][r] = val;
+        }
+        // buffer b is rewritten by field f+2, after the barrier of field f+1
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        int t = tid + r * 256;
+        int oy = t / TX, ox = t - oy * TX;
+        int x = x0 + ox, y = y0 + oy;
+        if (x >= A.L.n0 || y >= A.L.n1) continue;
+        const long long gi = A.L.at(x, y, 0);
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            T ef = e[f][r];
+            T k = A.cl * lp[f][r];
+            k = k + A.cp * (ef - ef * ef * ef);
+#pragma unroll
+            for (int h = 0; h < NF; ++h) {
+                if (h == f) continue;
+                k = k + A.cg * (ef * e[h][r] * e[h][r]);
+            }
+            const int s = A.stage;
+            if (EXACT) {
+                if (s == 1) {
+                    A.acc[f][gi] = k;
+                    A.out[f][gi] = ef + A.a * k;
+                } else if (s < 4) {
+                    A.acc[f][gi] = A.acc[f][gi] + (T)2 * k;
+                    A.out[f][gi] = A.c[f][gi] + A.a * k;
+                } else {
+                    A.out[f][gi] = A.c[f][gi] + A.dt6 * (A.acc[f][gi] + k);
+                }
+            } else {
+                if (s == 1) {
+                    A.acc[f][gi] = ef + A.dt6 * k;
+                    A.out[f][gi] = ef + A.a * k;
+                } else if (s < 4) {
+                    A.acc[f][gi] = A.acc[f][gi] + A.dt3 * k;
+                    A.out[f][gi] = A.c[f][gi] + A.a * k;
+                } else {
+                    A.out[f][gi] = A.acc[f][gi] + A.dt6 * k;
+                }
+            }
+        }
+    }
+}
+
+int bc_fill_impl(const phx_grid* G, int dtype, void* data, hipStream_t st);
+
+template <typename T>
+static AxisBC<T> ac_axis(const phx_grid* G, int a) {
+    AxisBC<T> b;
+    b.n = G->n[a];
+    b.klo = a < G->dim ? G->bc[a][0] : PHX_BC_PERIODIC;
+    b.khi = a < G->dim ? G->bc[a][1] : PHX_BC_PERIODIC;
+    b.vlo = (T)G->bc_value[a][0];
+    b.vhi = (T)G->bc_value[a][1];
+    return b;
+}
+
+constexpr int ACX = 64, ACY = 8;
+
+template <typename T, int NF, bool EXACT>
+static void ac_launch(const phx_grid* G, const ACArgs<T>& A, dim3 grid, hipStream_t st) {
+    if (G->dim == 2) hipLaunchKernelGGL((k_ac_stage_2d<T, 2, NF, ACX, ACY, EXACT>), grid, dim3(256), 0, st, A);
+    else hipLaunchKernelGGL((k_ac_stage_2d<T, 1, NF, ACX, ACY, EXACT>), grid, dim3(256), 0, st, A);
+}
+
+template <typename T, bool EXACT>
+static void ac_dispatch(int nf, const phx_grid* G, const ACArgs<T>& A, dim3 grid, hipStream_t st) {
+    switch (nf) {
+        case 1: ac_launch<T, 1, EXACT>(G, A, grid, st); break;
+        case 2: ac_launch<T, 2, EXACT>(G, A, grid, st); break;
+        case 3: ac_launch<T, 3, EXACT>(G, A, grid, st); break;
+        case 4: ac_launch<T, 4, EXACT>(G, A, grid, st); break;
+        case 5: ac_launch<T, 5, EXACT>(G, A, grid, st); break;
+        case 6: ac_launch<T, 6, EXACT>(G, A, grid, st); break;
+        case 7: ac_launch<T, 7, EXACT>(G, A, grid, st); break;
+        default: ac_launch<T, 8, EXACT>(G, A, grid, st); break;
+    }
+}
+
+template <typename T, bool EXACT>
+static int ac_steps_t(const phx_grid* G, int nf, T* const* eta, T* work, const phx_ac_params* p, int nsteps,
+                      hipStream_t st) {
+    ACArgs<T> A;
+    A.L = make_layout(G);
+    const long long N = A.L.total();
+    T* w0[AC_MAXF];
+    T* w1[AC_MAXF];
+    for (int f = 0; f < nf; ++f) {
+        w0[f] = work + (long long)(3 * f + 0) * N;
+        w1[f] = work + (long long)(3 * f + 1) * N;
+        A.acc[f] = work + (long long)(3 * f + 2) * N;
+        A.c[f] = eta[f];
+    }
+    A.bx = ac_axis<T>(G, 0);
+    A.by = ac_axis<T>(G, 1);
+    A.idx2 = (T)(1.0 / (G->d[0] * G->d[0]));
+    A.idy2 = G->dim >= 2 ? (T)(1.0 / (G->d[1] * G->d[1])) : (T)0;
+    A.cl = (T)(p->L * p->kappa);
+    A.cp = (T)p->L;
+    A.cg = (T)(-2.0 * p->L * p->gamma);
+    const double dt = p->dt, dt6 = dt / 6.0;
+    A.dt6 = (T)dt6;
+    A.dt3 = (T)(2.0 * dt6);
+    const T shift[4] = {(T)(dt * 0.5), (T)(dt * 0.5), (T)dt, (T)0};
+    dim3 grid(cdiv(A.L.n0, ACX), cdiv(A.L.n1, ACY), 1);
+    for (int s = 0; s < nsteps; ++s) {
+        if (s == nsteps - 1)
+            for (int f = 0; f < nf; ++f) {
+                int rc = bc_fill_impl(G, sizeof(T) == 8 ? PHX_F64 : PHX_F32, eta[f], st);
+                if (rc) return rc;
+            }
+        for (int stg = 0; stg < 4; ++stg) {
+            A.stage = stg + 1;
+            A.a = shift[stg];
+            for (int f = 0; f < nf; ++f) {
+                A.u[f] = stg == 0 ? eta[f] : (stg == 2 ? w1[f] : w0[f]);
+                A.out[f] = stg == 3 ? eta[f] : (stg == 1 ? w1[f] : w0[f]);
+            }
+            ac_dispatch<T, EXACT>(nf, G, A, grid, st);
+        }
+    }
+    PHX_LAUNCH_CHECK("phx_ac_rk4_steps");
+    return PHX_OK;
+}
+
+}  // namespace phx
+
+using namespace phx;
+
+extern "C" {
+
+size_t phx_ac_rk4_workspace_bytes(const phx_grid* G, int dtype, int nfields) {
+    if (check_grid(G, "phx_ac_rk4_workspace_bytes") || check_dtype(dtype, "phx_ac_rk4_workspace_bytes")) return 0;
+    if (nfields < 1 || nfields > AC_MAXF) return 0;
+    return (size_t)3 * nfields * make_layout(G).total() * (dtype == PHX_F64 ? 8 : 4);
+}
+
+int phx_ac_rk4_steps(const phx_grid* G, int dtype, int nfields, void* const* eta, void* work,
+                     const phx_ac_params* p, int nsteps, phx_stream s) {
+    int rc = check_grid_fused(G, "phx_ac_rk4_steps");
+    if (rc) return rc;
+    if ((rc = check_dtype(dtype, "phx_ac_rk4_steps"))) return rc;
+    if (G->dim > 2) return fail(PHX_ERR_UNSUPPORTED, "phx_ac_rk4_steps: dim must be 1 or 2");
+    if (nfields < 1 || nfields > AC_MAXF)
+        return fail(PHX_ERR_UNSUPPORTED, "phx_ac_rk4_steps: nfields must be in [1, 8]");
+    if (!eta || !work || !p) return fail(PHX_ERR_INVALID, "phx_ac_rk4_steps: NULL argument");
+    for (int f = 0; f < nfields; ++f)
+        if (!eta[f]) return fail(PHX_ERR_INVALID, "phx_ac_rk4_steps: NULL field pointer");
+    if (!(p->dt > 0.0)) return fail(PHX_ERR_INVALID, "phx_ac_rk4_steps: dt must be > 0");
+    if (nsteps < 0) return fail(PHX_ERR_INVALID, "phx_ac_rk4_steps: nsteps < 0");
+    if (nsteps == 0) return PHX_OK;
+    hipStream_t st = as_stream(s);
+    return dtype == PHX_F64 ? ac_steps_t<double, true>(G, nfields, (double* const*)eta, (double*)work, p, nsteps, st)
+                            : ac_steps_t<float, false>(G, nfields, (float* const*)eta, (float*)work, p, nsteps, st);
+}
+
+}  // extern "C"

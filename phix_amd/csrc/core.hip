@@ -1,0 +1,335 @@
+// core.hip -- error plumbing, grid checks, boundary kernels and the FP64
+// per-term kernels of the generic (unfused) DSL path.
+//
+// The per-term kernels keep the operation order of the reference launchers
+// (src/equation/Equation.cu:40-124, 386-404; include/equation/TermPW.inl:18-37)
+// so the generic path agrees with the FP64 oracle bit for bit; FMA contraction is
+// off in this file for that reason.
+#pragma clang fp contract(off)
+
+#include "common.hpp"
+
+#include <cstring>
+
+namespace phx {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int check_grid(const phx_grid* G, const char* who) {
+    if (!G) return fail(PHX_ERR_INVALID, std::string(who) + ": grid is NULL");
+    if (G->dim < 1 || G->dim > 3) return fail(PHX_ERR_INVALID, std::string(who) + ": dim must be 1, 2 or 3");
+    if (G->ghost < 0) return fail(PHX_ERR_INVALID, std::string(who) + ": ghost must be >= 0");
+    for (int a = 0; a < 3; ++a) {
+        if (a < G->dim) {
+            if (G->n[a] <= 0) return fail(PHX_ERR_INVALID, std::string(who) + ": n[" + std::to_string(a) + "] must be > 0");
+            if (!(G->d[a] > 0.0)) return fail(PHX_ERR_INVALID, std::string(who) + ": d[" + std::to_string(a) + "] must be > 0");
+        } else if (G->n[a] != 1) {
+            return fail(PHX_ERR_INVALID, std::string(who) + ": n[" + std::to_string(a) + "] must be 1 for unused axes");
+        }
+    }
+    return PHX_OK;
+}
+
+int check_grid_fused(const phx_grid* G, const char* who) {
+    int rc = check_grid(G, who);
+    if (rc) return rc;
+    if (G->ghost < 1) return fail(PHX_ERR_UNSUPPORTED, std::string(who) + ": ghost must be >= 1");
+    for (int a = 0; a < G->dim; ++a) {
+        int lo = G->bc[a][0], hi = G->bc[a][1];
+        if (lo < PHX_BC_PERIODIC || lo > PHX_BC_FIXED || hi < PHX_BC_PERIODIC || hi > PHX_BC_FIXED)
+            return fail(PHX_ERR_UNSUPPORTED, std::string(who) + ": every face of axis " + std::to_string(a) +
+                                                 " needs a PERIODIC, NOFLUX or FIXED condition");
+        if ((lo == PHX_BC_PERIODIC) != (hi == PHX_BC_PERIODIC))
+            return fail(PHX_ERR_INVALID, std::string(who) + ": periodic condition on one side only of axis " +
+                                             std::to_string(a));
+        if (G->n[a] < 2) return fail(PHX_ERR_UNSUPPORTED, std::string(who) + ": active axes need n >= 2");
+    }
+    return PHX_OK;
+}
+
+int check_dtype(int dtype, const char* who) {
+    if (dtype != PHX_F64 && dtype != PHX_F32)
+        return fail(PHX_ERR_INVALID, std::string(who) + ": dtype must be PHX_F64 or PHX_F32");
+    return PHX_OK;
+}
+
+// ------------------------------------------------------------------ BCs ----
+// One thread per (face0, face1) position, looping over ghost layers, as the
+// reference face kernels do (Boundary.cu:95-195). Consecutive threads walk the
+// faster stored axis of the face, so the Y and Z faces are coalesced.
+template <typename T>
+__global__ void k_bc_face(T* __restrict__ data, int nf0, int nf1, long long astride, long long fs0,
+                          long long fs1, int na, int g, int kind, int do_lo, int do_hi, T value) {
+    int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    int t1 = blockIdx.y * blockDim.y + threadIdx.y;
+    if (t0 >= nf0 || t1 >= nf1) return;
+    long long off = t0 * fs0 + t1 * fs1;
+    if (kind == PHX_BC_PERIODIC) {
+        for (int l = 1; l <= g; ++l) {
+            data[(g - l) * astride + off] = data[(g + na - l) * astride + off];
+            data[(g + na + l - 1) * astride + off] = data[(g + l - 1) * astride + off];
+        }
+    } else if (kind == PHX_BC_NOFLUX) {
+        if (do_lo) {
+            T v = data[g * astride + off];
+            for (int l = 1; l <= g; ++l) data[(g - l) * astride + off] = v;
+        }
+        if (do_hi) {
+            T v = data[(g + na - 1) * astride + off];
+            for (int l = 1; l <= g; ++l) data[(g + na + l - 1) * astride + off] = v;
+        }
+    } else {
+        if (do_lo)
+            for (int l = 1; l <= g; ++l) data[(g - l) * astride + off] = value;
+        if (do_hi)
+            for (int l = 1; l <= g; ++l) data[(g + na + l - 1) * astride + off] = value;
+    }
+}
+
+template <typename T>
+static int bc_apply_t(const phx_grid* G, T* data, int axis, int kind, int side, double value, hipStream_t st) {
+    Layout L = make_layout(G);
+    long long astride, fs0, fs1;
+    int nf0, nf1, na;
+    if (axis == 0) { astride = 1; nf0 = L.sy; nf1 = L.sz; fs0 = L.sx; fs1 = L.sxy; na = L.n0; }
+    else if (axis == 1) { astride = L.sx; nf0 = L.sx; nf1 = L.sz; fs0 = 1; fs1 = L.sxy; na = L.n1; }
+    else { astride = L.sxy; nf0 = L.sx; nf1 = L.sy; fs0 = 1; fs1 = L.sx; na = L.n2; }
+    int lo = (side == PHX_SIDE_LOW || side == PHX_SIDE_BOTH);
+    int hi = (side == PHX_SIDE_HIGH || side == PHX_SIDE_BOTH);
+    dim3 blk(64, 4);
+    dim3 grd(cdiv(nf0, 64), cdiv(nf1, 4));
+    if (grd.y > 65535) return fail(PHX_ERR_UNSUPPORTED, "phx_bc_apply: face too large");
+    hipLaunchKernelGGL(k_bc_face<T>, grd, blk, 0, st, data, nf0, nf1, astride, fs0, fs1, na, L.g, kind, lo, hi,
+                       (T)value);
+    PHX_LAUNCH_CHECK("phx_bc_apply");
+    return PHX_OK;
+}
+
+int bc_fill_impl(const phx_grid* G, int dtype, void* data, hipStream_t st) {
+    for (int a = 0; a < G->dim; ++a) {
+        int klo = G->bc[a][0], khi = G->bc[a][1];
+        int rc = 0;
+        if (klo == PHX_BC_PERIODIC && khi == PHX_BC_PERIODIC) {
+            rc = dtype == PHX_F64 ? bc_apply_t(G, (double*)data, a, PHX_BC_PERIODIC, PHX_SIDE_BOTH, 0.0, st)
+                                  : bc_apply_t(G, (float*)data, a, PHX_BC_PERIODIC, PHX_SIDE_BOTH, 0.0, st);
+            if (rc) return rc;
+            continue;
+        }
+        if (klo == PHX_BC_PERIODIC || khi == PHX_BC_PERIODIC)
+            return fail(PHX_ERR_INVALID, "phx_bc_fill: periodic condition on one side only of axis " + std::to_string(a));
+        for (int s = 0; s < 2; ++s) {
+            int k = G->bc[a][s];
+            if (k == PHX_BC_NONE) continue;
+            int side = s == 0 ? PHX_SIDE_LOW : PHX_SIDE_HIGH;
+            rc = dtype == PHX_F64 ? bc_apply_t(G, (double*)data, a, k, side, G->bc_value[a][s], st)
+                                  : bc_apply_t(G, (float*)data, a, k, side, G->bc_value[a][s], st);
+            if (rc) return rc;
+        }
+    }
+    return PHX_OK;
+}
+
+// ---------------------------------------------------------- term kernels ----
+struct TermGeom {
+    Layout L;
+    double idx2, idy2, idz2;
+};
+
+static inline TermGeom term_geom(const phx_grid* G) {
+    TermGeom t;
+    t.L = make_layout(G);
+    t.idx2 = 1.0 / (G->d[0] * G->d[0]);
+    t.idy2 = G->dim >= 2 ? 1.0 / (G->d[1] * G->d[1]) : 0.0;
+    t.idz2 = G->dim >= 3 ? 1.0 / (G->d[2] * G->d[2]) : 0.0;
+    return t;
+}
+
+static inline dim3 phys_grid(const Layout& L) { return dim3(cdiv(L.n0, 64), L.n1, L.n2); }
+
+__global__ void k_lap_acc(double* __restrict__ rhs, const double* __restrict__ src, double coeff, TermGeom t) {
+    int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= t.L.n0) return;
+    long long c = t.L.at(i, j, k);
+    double val = (src[c + 1] - 2.0 * src[c] + src[c - 1]) * t.idx2;
+    if (t.L.dim >= 2) val += (src[c + t.L.sx] - 2.0 * src[c] + src[c - t.L.sx]) * t.idy2;
+    if (t.L.dim >= 3) val += (src[c + t.L.sxy] - 2.0 * src[c] + src[c - t.L.sxy]) * t.idz2;
+    rhs[c] += coeff * val;
+}
+
+__global__ void k_grad_acc(double* __restrict__ rhs, const double* __restrict__ src, double coeff, long long st,
+                           double inv_2d, Layout L) {
+    int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= L.n0) return;
+    long long c = L.at(i, j, k);
+    rhs[c] += coeff * (src[c + st] - src[c - st]) * inv_2d;
+}
+
+__global__ void k_mul_acc(double* __restrict__ rhs, const double* __restrict__ s1, const double* __restrict__ s2,
+                          double coeff, Layout L) {
+    int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= L.n0) return;
+    long long c = L.at(i, j, k);
+    rhs[c] += coeff * s1[c] * s2[c];
+}
+
+__global__ void k_pw_dw_acc(double* __restrict__ rhs, const double* __restrict__ src, double rho, double ca,
+                            double cb, double coeff, Layout L) {
+    int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= L.n0) return;
+    long long c = L.at(i, j, k);
+    double v = src[c];
+    rhs[c] += coeff * (2.0 * rho * (v - ca) * (v - cb) * (2.0 * v - ca - cb));
+}
+
+// Whole-array kernels: grid-stride over 64-bit n.
+__global__ void k_fill(double* __restrict__ d, double v, size_t n) {
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) d[q] = v;
+}
+__global__ void k_copy(double* __restrict__ d, const double* __restrict__ s, size_t n) {
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) d[q] = s[q];
+}
+__global__ void k_axpy(double* __restrict__ d, const double* __restrict__ s, double a, size_t n) {
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        d[q] += a * s[q];
+}
+__global__ void k_rk4_tmp(double* __restrict__ t, const double* __restrict__ p, const double* __restrict__ kk,
+                          double a, size_t n) {
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        t[q] = p[q] + a * kk[q];
+}
+__global__ void k_rk4_update(double* __restrict__ p, const double* __restrict__ k1, const double* __restrict__ k2,
+                             const double* __restrict__ k3, const double* __restrict__ k4, double dt6, size_t n) {
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        p[q] += dt6 * (k1[q] + 2.0 * k2[q] + 2.0 * k3[q] + k4[q]);
+}
+
+static inline int flat_blocks(size_t n) {
+    size_t b = (n + 255) / 256;
+    return (int)(b > 8192 ? 8192 : (b == 0 ? 1 : b));
+}
+
+}  // namespace phx
+
+using namespace phx;
+
+extern "C" {
+
+int phx_abi_version(void) { return PHX_ABI_VERSION; }
+
+const char* phx_last_error(void) { return g_last_error.c_str(); }
+
+size_t phx_stored_size(const phx_grid* G) {
+    if (check_grid(G, "phx_stored_size")) return 0;
+    return (size_t)make_layout(G).total();
+}
+
+int phx_stream_sync(phx_stream s) {
+    PHX_HIP(hipStreamSynchronize(as_stream(s)));
+    return PHX_OK;
+}
+
+int phx_bc_apply(const phx_grid* G, int dtype, void* data, int axis, int kind, int side, double value,
+                 phx_stream s) {
+    int rc = check_grid(G, "phx_bc_apply");
+    if (rc) return rc;
+    if ((rc = check_dtype(dtype, "phx_bc_apply"))) return rc;
+    if (!data) return fail(PHX_ERR_INVALID, "phx_bc_apply: data is NULL");
+    if (axis < 0 || axis > 2) return fail(PHX_ERR_INVALID, "phx_bc_apply: axis must be 0, 1 or 2");
+    if (kind < PHX_BC_PERIODIC || kind > PHX_BC_FIXED) return fail(PHX_ERR_INVALID, "phx_bc_apply: bad kind");
+    if (side < PHX_SIDE_LOW || side > PHX_SIDE_BOTH) return fail(PHX_ERR_INVALID, "phx_bc_apply: bad side");
+    if (G->ghost == 0) return PHX_OK;
+    return dtype == PHX_F64 ? bc_apply_t(G, (double*)data, axis, kind, side, value, as_stream(s))
+                            : bc_apply_t(G, (float*)data, axis, kind, side, value, as_stream(s));
+}
+
+int phx_bc_fill(const phx_grid* G, int dtype, void* data, phx_stream s) {
+    int rc = check_grid(G, "phx_bc_fill");
+    if (rc) return rc;
+    if ((rc = check_dtype(dtype, "phx_bc_fill"))) return rc;
+    if (!data) return fail(PHX_ERR_INVALID, "phx_bc_fill: data is NULL");
+    if (G->ghost == 0) return PHX_OK;
+    return bc_fill_impl(G, dtype, data, as_stream(s));
+}
+
+int phx_lap_accumulate(const phx_grid* G, double* rhs, const double* src, double coeff, phx_stream s) {
+    int rc = check_grid(G, "phx_lap_accumulate");
+    if (rc) return rc;
+    if (G->ghost < 1) return fail(PHX_ERR_INVALID, "phx_lap_accumulate: needs ghost >= 1");
+    if (!rhs || !src) return fail(PHX_ERR_INVALID, "phx_lap_accumulate: NULL buffer");
+    TermGeom t = term_geom(G);
+    hipLaunchKernelGGL(k_lap_acc, phys_grid(t.L), dim3(64), 0, as_stream(s), rhs, src, coeff, t);
+    PHX_LAUNCH_CHECK("phx_lap_accumulate");
+    return PHX_OK;
+}
+
+int phx_grad_accumulate(const phx_grid* G, double* rhs, const double* src, int axis, double coeff, phx_stream s) {
+    int rc = check_grid(G, "phx_grad_accumulate");
+    if (rc) return rc;
+    if (axis < 0 || axis >= G->dim)
+        return fail(PHX_ERR_INVALID, "grad: axis out of range for this mesh dimension");
+    if (G->ghost < 1) return fail(PHX_ERR_INVALID, "phx_grad_accumulate: needs ghost >= 1");
+    if (!rhs || !src) return fail(PHX_ERR_INVALID, "phx_grad_accumulate: NULL buffer");
+    Layout L = make_layout(G);
+    long long st = axis == 0 ? 1 : (axis == 1 ? L.sx : L.sxy);
+    hipLaunchKernelGGL(k_grad_acc, phys_grid(L), dim3(64), 0, as_stream(s), rhs, src, coeff, st, 0.5 / G->d[axis], L);
+    PHX_LAUNCH_CHECK("phx_grad_accumulate");
+    return PHX_OK;
+}
+
+int phx_mul_accumulate(const phx_grid* G, double* rhs, const double* s1, const double* s2, double coeff,
+                       phx_stream s) {
+    int rc = check_grid(G, "phx_mul_accumulate");
+    if (rc) return rc;
+    if (!rhs || !s1 || !s2) return fail(PHX_ERR_INVALID, "phx_mul_accumulate: NULL buffer");
+    Layout L = make_layout(G);
+    hipLaunchKernelGGL(k_mul_acc, phys_grid(L), dim3(64), 0, as_stream(s), rhs, s1, s2, coeff, L);
+    PHX_LAUNCH_CHECK("phx_mul_accumulate");
+    return PHX_OK;
+}
+
+int phx_pw_doublewell_accumulate(const phx_grid* G, double* rhs, const double* src, double rho, double ca,
+                                 double cb, double coeff, phx_stream s) {
+    int rc = check_grid(G, "phx_pw_doublewell_accumulate");
+    if (rc) return rc;
+    if (!rhs || !src) return fail(PHX_ERR_INVALID, "phx_pw_doublewell_accumulate: NULL buffer");
+    Layout L = make_layout(G);
+    hipLaunchKernelGGL(k_pw_dw_acc, phys_grid(L), dim3(64), 0, as_stream(s), rhs, src, rho, ca, cb, coeff, L);
+    PHX_LAUNCH_CHECK("phx_pw_doublewell_accumulate");
+    return PHX_OK;
+}
+
+#define PHX_FLAT_ENTRY(name, kernel, check, ...)                                            \
+    do {                                                                                    \
+        if (n == 0) return PHX_OK;                                                          \
+        if (check) return fail(PHX_ERR_INVALID, std::string(name) + ": NULL buffer");       \
+        hipLaunchKernelGGL(kernel, dim3(flat_blocks(n)), dim3(256), 0, as_stream(s), __VA_ARGS__); \
+        PHX_LAUNCH_CHECK(name);                                                             \
+        return PHX_OK;                                                                      \
+    } while (0)
+
+int phx_fill_f64(double* dst, double value, size_t n, phx_stream s) {
+    PHX_FLAT_ENTRY("phx_fill_f64", k_fill, !dst, dst, value, n);
+}
+int phx_copy_f64(double* dst, const double* src, size_t n, phx_stream s) {
+    PHX_FLAT_ENTRY("phx_copy_f64", k_copy, !dst || !src, dst, src, n);
+}
+int phx_axpy_f64(double* dst, const double* src, double coeff, size_t n, phx_stream s) {
+    PHX_FLAT_ENTRY("phx_axpy_f64", k_axpy, !dst || !src, dst, src, coeff, n);
+}
+int phx_rk4_tmp_f64(double* tmp, const double* phi, const double* k, double coeff, size_t n, phx_stream s) {
+    PHX_FLAT_ENTRY("phx_rk4_tmp_f64", k_rk4_tmp, !tmp || !phi || !k, tmp, phi, k, coeff, n);
+}
+int phx_rk4_update_f64(double* phi, const double* k1, const double* k2, const double* k3, const double* k4,
+                       double dt6, size_t n, phx_stream s) {
+    PHX_FLAT_ENTRY("phx_rk4_update_f64", k_rk4_update, !phi || !k1 || !k2 || !k3 || !k4, phi, k1, k2, k3, k4, dt6,
+                   n);
+}
+
+}  // extern "C"
