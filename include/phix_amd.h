@@ -139,6 +139,10 @@ int phx_rk4_tmp_f64(double* tmp, const double* phi, const double* k, double coef
                     size_t n, phx_stream stream);
 int phx_rk4_update_f64(double* phi, const double* k1, const double* k2, const double* k3,
                        const double* k4, double dt6, size_t n, phx_stream stream);
+/* Precision conversion of n stored values (round to nearest / exact widening).
+ * Used by the C++ Solver's FP32-bulk option; no reference counterpart. */
+int phx_convert_f64_to_f32(float* dst, const double* src, size_t n, phx_stream stream);
+int phx_convert_f32_to_f64(double* dst, const float* src, size_t n, phx_stream stream);
 
 /* ------------------------------------------- fused Cahn-Hilliard ---- */
 /* nsteps of the reference CH multi-step Euler solver
@@ -182,6 +186,40 @@ size_t phx_ac_rk4_workspace_bytes(const phx_grid* grid, int dtype, int nfields);
  * 1 <= nfields <= 8, dim 1 or 2. work is phx_ac_rk4_workspace_bytes (3*nfields fields). */
 int phx_ac_rk4_steps(const phx_grid* grid, int dtype, int nfields, void* const* eta,
                      void* work, const phx_ac_params* p, int nsteps, phx_stream stream);
+
+/* ------------------------------------ batched uneven-size ensemble ---- */
+/* No reference counterpart (PhiX runs one problem per process). An ensemble
+ * is nruns independent 2D periodic Cahn-Hilliard runs (f = c^2(1-c)^2,
+ * per-run kappa and mobility M), side lengths sides[r], advanced together by
+ * one persistent-kernel launch per RK4 stage. FP32 state: the runs' physical
+ * n_r x n_r fields back to back in one arena of phx_ensemble_cells() floats,
+ * run r at element phx_ensemble_offset(r), x fastest, no ghost cells.
+ * work: 3 * phx_ensemble_cells() floats. create allocates the device tables;
+ * the other calls allocate nothing (graph instantiation aside). */
+typedef struct phx_ensemble phx_ensemble;
+int       phx_ensemble_create(int nruns, const int* sides, const double* kappa,
+                              const double* mobility /* NULL: all 1 */, phx_ensemble** out);
+int       phx_ensemble_destroy(phx_ensemble* ens);
+long long phx_ensemble_cells(const phx_ensemble* ens);
+long long phx_ensemble_offset(const phx_ensemble* ens, int run);
+/* nsteps RK4 steps (4 launches each) on `stream`. */
+int phx_ensemble_rk4_steps(phx_ensemble* ens, float* c, float* work, double dt, int nsteps,
+                           phx_stream stream);
+/* Captures steps_per_graph RK4 steps into a hipGraph (once per (c, work,
+ * steps, tiling, dt)) and replays it nreplays times on `stream`. */
+int phx_ensemble_graph_steps(phx_ensemble* ens, float* c, float* work, double dt,
+                             int steps_per_graph, int nreplays, phx_stream stream);
+/* out_dev[r] (device, nruns doubles) <- FP64 compensated sum of run r. */
+int phx_ensemble_mass(const phx_ensemble* ens, const float* c, double* out_dev, phx_stream stream);
+/* Tile shape index (0..4: 64x16, 32x16, 64x8, 128x8, 32x32) and persistent
+ * grid size; grid <= 0 picks 8 workgroups per CU. */
+int phx_ensemble_set_tiling(phx_ensemble* ens, int shape, int grid);
+int phx_ensemble_get_tiling(const phx_ensemble* ens, int* shape, int* grid, int* tx, int* ty);
+/* Times every tile shape x {2,4,8,16} workgroups per CU on `steps` steps of
+ * the given state (advancing it: pass a scratch copy), keeps the fastest and
+ * returns its ms per step. Blocks the host. */
+int phx_ensemble_autotune(phx_ensemble* ens, float* c, float* work, double dt, int steps,
+                          double* best_ms_per_step, phx_stream stream);
 
 #ifdef __cplusplus
 }

@@ -62,6 +62,29 @@ def build_lib(jobs: int = 8, verbose: bool = True) -> Path:
     return LIB
 
 
+def build_cpp_tests(jobs: int = 8, verbose: bool = True) -> list[Path]:
+    """Compiles tests/cpp/*.cpp (programs written against the drop-in C++ API)
+    as HIP and links them to libphix_amd.so (rpath to the package dir)."""
+    srcs = sorted((ROOT / "tests" / "cpp").glob("*.cpp"))
+    out_dir = OBJ / "bin"
+    out_dir.mkdir(parents=True, exist_ok=True)
+
+    def one(src: Path) -> Path:
+        exe = out_dir / src.stem
+        cmd = [HIPCC, *HIPFLAGS, "-x", "hip", str(src), "-o", str(exe), f"-L{PKG}", "-lphix_amd",
+               f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/../.."]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")
+        return exe
+
+    with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        exes = list(ex.map(one, srcs))
+    if verbose:
+        print(f"built {len(exes)} C++ API test programs in {out_dir}", flush=True)
+    return exes
+
+
 def build_oracle(verbose: bool = True) -> Path:
     r = subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], capture_output=True, text=True)
     if r.returncode != 0:
@@ -78,6 +101,7 @@ def main(argv=None) -> int:
     a = ap.parse_args(argv)
     build_oracle()
     build_lib(a.jobs)
+    build_cpp_tests(a.jobs)
     return 0
 
 

@@ -115,6 +115,8 @@ SIG = {
     "phx_axpy_f64": (C.c_int, [VP, VP, C.c_double, C.c_size_t, VP]),
     "phx_rk4_tmp_f64": (C.c_int, [VP, VP, VP, C.c_double, C.c_size_t, VP]),
     "phx_rk4_update_f64": (C.c_int, [VP, VP, VP, VP, VP, C.c_double, C.c_size_t, VP]),
+    "phx_convert_f64_to_f32": (C.c_int, [VP, VP, C.c_size_t, VP]),
+    "phx_convert_f32_to_f64": (C.c_int, [VP, VP, C.c_size_t, VP]),
     "phx_ch_euler_steps": (C.c_int, [GP, C.c_int, VP, VP, C.POINTER(CHParams), C.c_int, VP]),
     "phx_ch_rk4_workspace_bytes": (C.c_size_t, [GP, C.c_int]),
     "phx_ch_rk4_steps": (C.c_int, [GP, C.c_int, VP, VP, C.POINTER(CHParams), C.c_int, VP]),
@@ -122,6 +124,18 @@ SIG = {
     "phx_mass_energy": (C.c_int, [GP, C.c_int, VP, C.POINTER(CHParams), VP, VP, VP]),
     "phx_ac_rk4_workspace_bytes": (C.c_size_t, [GP, C.c_int, C.c_int]),
     "phx_ac_rk4_steps": (C.c_int, [GP, C.c_int, C.c_int, C.POINTER(VP), VP, C.POINTER(ACParams), C.c_int, VP]),
+    "phx_ensemble_create": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                      C.POINTER(VP)]),
+    "phx_ensemble_destroy": (C.c_int, [VP]),
+    "phx_ensemble_cells": (C.c_longlong, [VP]),
+    "phx_ensemble_offset": (C.c_longlong, [VP, C.c_int]),
+    "phx_ensemble_rk4_steps": (C.c_int, [VP, VP, VP, C.c_double, C.c_int, VP]),
+    "phx_ensemble_graph_steps": (C.c_int, [VP, VP, VP, C.c_double, C.c_int, C.c_int, VP]),
+    "phx_ensemble_mass": (C.c_int, [VP, VP, VP, VP]),
+    "phx_ensemble_set_tiling": (C.c_int, [VP, C.c_int, C.c_int]),
+    "phx_ensemble_get_tiling": (C.c_int, [VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                          C.POINTER(C.c_int)]),
+    "phx_ensemble_autotune": (C.c_int, [VP, VP, VP, C.c_double, C.c_int, C.POINTER(C.c_double), VP]),
 }
 
 

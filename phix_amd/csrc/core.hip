@@ -210,6 +210,15 @@ __global__ void k_rk4_update(double* __restrict__ p, const double* __restrict__ 
         p[q] += dt6 * (k1[q] + 2.0 * k2[q] + 2.0 * k3[q] + k4[q]);
 }
 
+__global__ void k_f64_to_f32(float* __restrict__ d, const double* __restrict__ s, size_t n) {
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        d[q] = (float)s[q];
+}
+__global__ void k_f32_to_f64(double* __restrict__ d, const float* __restrict__ s, size_t n) {
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        d[q] = (double)s[q];
+}
+
 static inline int flat_blocks(size_t n) {
     size_t b = (n + 255) / 256;
     return (int)(b > 8192 ? 8192 : (b == 0 ? 1 : b));
@@ -330,6 +339,13 @@ int phx_rk4_update_f64(double* phi, const double* k1, const double* k2, const do
                        double dt6, size_t n, phx_stream s) {
     PHX_FLAT_ENTRY("phx_rk4_update_f64", k_rk4_update, !phi || !k1 || !k2 || !k3 || !k4, phi, k1, k2, k3, k4, dt6,
                    n);
+}
+
+int phx_convert_f64_to_f32(float* dst, const double* src, size_t n, phx_stream s) {
+    PHX_FLAT_ENTRY("phx_convert_f64_to_f32", k_f64_to_f32, !dst || !src, dst, src, n);
+}
+int phx_convert_f32_to_f64(double* dst, const float* src, size_t n, phx_stream s) {
+    PHX_FLAT_ENTRY("phx_convert_f32_to_f64", k_f32_to_f64, !dst || !src, dst, src, n);
 }
 
 }  // extern "C"

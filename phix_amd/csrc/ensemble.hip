@@ -1,0 +1,390 @@
+// ensemble.hip -- batched ensemble of independent 2D periodic Cahn-Hilliard
+// runs of uneven sizes, all advanced by one launch per RK4 stage.
+//
+// No reference counterpart: PhiX runs one problem per process. Layout: the
+// runs' physical N_r x N_r FP32 fields packed back to back in one arena (no
+// ghost cells -- periodic halos are index maps), offsets in a device table.
+// A stage is one launch of a persistent kernel: a fixed grid of workgroups
+// walks a work list of (run, tile) entries ordered largest run first, so the
+// uneven sizes load-balance across the 256 CUs. The step loop (4 stage
+// launches per step) is captured once into a hipGraph and replayed.
+// Per-run mass is reduced in FP64 with Neumaier-compensated sums, one
+// workgroup per run.
+// Tile shape and grid size come from phx_ensemble_autotune (timed candidates).
+#include "common.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <vector>
+
+namespace phx {
+
+struct EnsRun {
+    long long off;   // element offset of the run in the arena
+    int n;           // side length
+    float kappa;     // per-run kappa (f = c^2(1-c)^2, M = 1 unless set)
+    float M;
+};
+
+struct EnsTile {
+    int run;
+    int x0, y0;
+};
+
+struct EnsArgs {
+    const float* u;
+    const float* c;
+    float* acc;
+    float* out;
+    const EnsRun* runs;
+    const EnsTile* tiles;
+    int ntiles;
+    float a, dt6, dt3;
+    int stage;
+};
+
+// Periodic wrap for |p - n| < n.
+__device__ __forceinline__ int wrapp(int p, int n) { return p < 0 ? p + n : (p >= n ? p - n : p); }
+
+template <int TX, int TY>
+__global__ __launch_bounds__(256) void k_ens_stage(EnsArgs A) {
+    constexpr int UX = TX + 4, UY = TY + 4, MX = TX + 2, MY = TY + 2;
+    __shared__ float U[UY][UX];
+    __shared__ float MU[MY][MX];
+    const int tid = threadIdx.x;
+    for (int w = blockIdx.x; w < A.ntiles; w += gridDim.x) {
+        const EnsTile T = A.tiles[w];
+        const EnsRun R = A.runs[T.run];
+        const int n = R.n;
+        const long long off = R.off;
+        const bool interior = T.x0 >= 2 && T.x0 + TX + 1 < n && T.y0 >= 2 && T.y0 + TY + 1 < n;
+        const float* u = A.u + off;
+        if (interior) {
+            const float* base = u + (long long)(T.y0 - 2) * n + (T.x0 - 2);
+            for (int t = tid; t < UX * UY; t += 256) {
+                int ly = t / UX, lx = t - ly * UX;
+                U[ly][lx] = base[(long long)ly * n + lx];
+            }
+        } else {
+            for (int t = tid; t < UX * UY; t += 256) {
+                int ly = t / UX, lx = t - ly * UX;
+                int px = T.x0 - 2 + lx, py = T.y0 - 2 + ly;
+                px = min(max(px, -2), n + 1);
+                py = min(max(py, -2), n + 1);
+                U[ly][lx] = u[(long long)wrapp(py, n) * n + wrapp(px, n)];
+            }
+        }
+        __syncthreads();
+        const float nk = -R.kappa;
+        for (int t = tid; t < MX * MY; t += 256) {
+            int my = t / MX, mx = t - my * MX;
+            float cc = U[my + 1][mx + 1];
+            float lap = (U[my + 1][mx + 2] - 2.f * cc + U[my + 1][mx]) + (U[my + 2][mx + 1] - 2.f * cc + U[my][mx + 1]);
+            // f'(c) = 2c(c-1)(2c-1) for f = c^2(1-c)^2
+            float fp = 2.f * cc * (cc - 1.f) * (2.f * cc - 1.f);
+            MU[my][mx] = fp + nk * lap;
+        }
+        __syncthreads();
+        for (int t = tid; t < TX * TY; t += 256) {
+            int oy = t / TX, ox = t - oy * TX;
+            int x = T.x0 + ox, y = T.y0 + oy;
+            if (x < n && y < n) {
+                float m = MU[oy + 1][ox + 1];
+                float lm = (MU[oy + 1][ox + 2] - 2.f * m + MU[oy + 1][ox]) + (MU[oy + 2][ox + 1] - 2.f * m + MU[oy][ox + 1]);
+                float k = R.M * lm;
+                long long gi = off + (long long)y * n + x;
+                float uc = U[oy + 2][ox + 2];
+                if (A.stage == 1) {
+                    A.acc[gi] = uc + A.dt6 * k;
+                    A.out[gi] = uc + A.a * k;
+                } else if (A.stage < 4) {
+                    A.acc[gi] = A.acc[gi] + A.dt3 * k;
+                    A.out[gi] = A.c[gi] + A.a * k;
+                } else {
+                    A.out[gi] = A.acc[gi] + A.dt6 * k;
+                }
+            }
+        }
+        __syncthreads();   // LDS is reused by the next tile of this workgroup
+    }
+}
+
+// FP64 compensated per-run sum: one workgroup per run.
+__global__ __launch_bounds__(256) void k_ens_mass(const float* __restrict__ c, const EnsRun* __restrict__ runs,
+                                                  double* __restrict__ out) {
+    const EnsRun R = runs[blockIdx.x];
+    const long long cnt = (long long)R.n * R.n;
+    double s = 0.0, comp = 0.0;
+    for (long long q = threadIdx.x; q < cnt; q += 256) {
+        double x = (double)c[R.off + q];
+        double t = s + x;
+        comp += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+        s = t;
+    }
+    __shared__ double S[256], Cc[256];
+    S[threadIdx.x] = s;
+    Cc[threadIdx.x] = comp;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            double a = S[threadIdx.x], b = S[threadIdx.x + h];
+            double t = a + b;
+            double e = fabs(a) >= fabs(b) ? (a - t) + b : (b - t) + a;
+            S[threadIdx.x] = t;
+            Cc[threadIdx.x] += Cc[threadIdx.x + h] + e;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = S[0] + Cc[0];
+}
+
+// ------------------------------------------------------------------ host --
+struct TileShape {
+    int tx, ty;
+};
+static const TileShape kShapes[] = {{64, 16}, {32, 16}, {64, 8}, {128, 8}, {32, 32}};
+constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+
+template <int TX, int TY>
+static void launch_stage(int grid, const EnsArgs& A, hipStream_t st) {
+    hipLaunchKernelGGL((k_ens_stage<TX, TY>), dim3(grid), dim3(256), 0, st, A);
+}
+
+static void launch_shape(int shape, int grid, const EnsArgs& A, hipStream_t st) {
+    switch (shape) {
+        case 0: launch_stage<64, 16>(grid, A, st); break;
+        case 1: launch_stage<32, 16>(grid, A, st); break;
+        case 2: launch_stage<64, 8>(grid, A, st); break;
+        case 3: launch_stage<128, 8>(grid, A, st); break;
+        default: launch_stage<32, 32>(grid, A, st); break;
+    }
+}
+
+}  // namespace phx
+
+using namespace phx;
+
+// Opaque handle: device tables for one ensemble and one tile shape, plus the
+// captured graph. Allocation happens in phx_ensemble_create only.
+struct phx_ensemble {
+    int nruns = 0;
+    long long total = 0;
+    std::vector<EnsRun> runs;
+    EnsRun* d_runs = nullptr;
+    EnsTile* d_tiles[kNumShapes] = {};
+    int ntiles[kNumShapes] = {};
+    int shape = 0;
+    int grid = 0;
+    int cus = 256;
+    hipGraphExec_t graph = nullptr;
+    int graph_steps = 0;
+    const void* graph_key[4] = {};
+    double dt = 0.0;
+};
+
+extern "C" {
+
+int phx_ensemble_create(int nruns, const int* sides, const double* kappa, const double* mobility,
+                        phx_ensemble** out) {
+    if (!out || nruns <= 0 || !sides || !kappa) return fail(PHX_ERR_INVALID, "phx_ensemble_create: bad arguments");
+    *out = nullptr;
+    phx_ensemble* E = new phx_ensemble();
+    E->nruns = nruns;
+    E->runs.resize(nruns);
+    long long off = 0;
+    for (int r = 0; r < nruns; ++r) {
+        if (sides[r] < 2 || sides[r] > 46340) {
+            delete E;
+            return fail(PHX_ERR_INVALID, "phx_ensemble_create: side lengths must be in [2, 46340]");
+        }
+        E->runs[r] = EnsRun{off, sides[r], (float)kappa[r], (float)(mobility ? mobility[r] : 1.0)};
+        off += (long long)sides[r] * sides[r];
+    }
+    E->total = off;
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+        E->cus = prop.multiProcessorCount;
+    // work lists per tile shape, largest runs first
+    std::vector<int> order(nruns);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sides[a] > sides[b]; });
+    if (hipMalloc(&E->d_runs, sizeof(EnsRun) * nruns) != hipSuccess ||
+        hipMemcpy(E->d_runs, E->runs.data(), sizeof(EnsRun) * nruns, hipMemcpyHostToDevice) != hipSuccess) {
+        phx_ensemble_destroy(E);
+        return fail(PHX_ERR_NOMEM, "phx_ensemble_create: device tables");
+    }
+    for (int s = 0; s < kNumShapes; ++s) {
+        std::vector<EnsTile> tl;
+        for (int r : order) {
+            int n = sides[r];
+            for (int y = 0; y < n; y += kShapes[s].ty)
+                for (int x = 0; x < n; x += kShapes[s].tx) tl.push_back(EnsTile{r, x, y});
+        }
+        E->ntiles[s] = (int)tl.size();
+        if (hipMalloc(&E->d_tiles[s], sizeof(EnsTile) * tl.size()) != hipSuccess ||
+            hipMemcpy(E->d_tiles[s], tl.data(), sizeof(EnsTile) * tl.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            phx_ensemble_destroy(E);
+            return fail(PHX_ERR_NOMEM, "phx_ensemble_create: device tables");
+        }
+    }
+    E->shape = 0;
+    E->grid = std::min(E->ntiles[0], 8 * E->cus);
+    *out = E;
+    return PHX_OK;
+}
+
+int phx_ensemble_destroy(phx_ensemble* E) {
+    if (!E) return PHX_OK;
+    if (E->graph) (void)hipGraphExecDestroy(E->graph);
+    if (E->d_runs) (void)hipFree(E->d_runs);
+    for (int s = 0; s < kNumShapes; ++s)
+        if (E->d_tiles[s]) (void)hipFree(E->d_tiles[s]);
+    delete E;
+    return PHX_OK;
+}
+
+long long phx_ensemble_cells(const phx_ensemble* E) { return E ? E->total : -1; }
+
+long long phx_ensemble_offset(const phx_ensemble* E, int run) {
+    if (!E || run < 0 || run >= E->nruns) return -1;
+    return E->runs[run].off;
+}
+
+int phx_ensemble_set_tiling(phx_ensemble* E, int shape, int grid) {
+    if (!E || shape < 0 || shape >= kNumShapes) return fail(PHX_ERR_INVALID, "phx_ensemble_set_tiling: bad shape");
+    E->shape = shape;
+    E->grid = grid > 0 ? std::min(grid, E->ntiles[shape]) : std::min(E->ntiles[shape], 8 * E->cus);
+    if (E->graph) {
+        (void)hipGraphExecDestroy(E->graph);
+        E->graph = nullptr;
+    }
+    return PHX_OK;
+}
+
+int phx_ensemble_get_tiling(const phx_ensemble* E, int* shape, int* grid, int* tx, int* ty) {
+    if (!E) return fail(PHX_ERR_INVALID, "phx_ensemble_get_tiling: NULL");
+    if (shape) *shape = E->shape;
+    if (grid) *grid = E->grid;
+    if (tx) *tx = kShapes[E->shape].tx;
+    if (ty) *ty = kShapes[E->shape].ty;
+    return PHX_OK;
+}
+
+static void enqueue_steps(phx_ensemble* E, float* c, float* work, double dt, int nsteps, hipStream_t st) {
+    const long long N = E->total;
+    float* w0 = work;
+    float* w1 = work + N;
+    float* acc = work + 2 * N;
+    EnsArgs A;
+    A.c = c;
+    A.acc = acc;
+    A.runs = E->d_runs;
+    A.tiles = E->d_tiles[E->shape];
+    A.ntiles = E->ntiles[E->shape];
+    const double dt6 = dt / 6.0;
+    A.dt6 = (float)dt6;
+    A.dt3 = (float)(2.0 * dt6);
+    const float shift[4] = {(float)(dt * 0.5), (float)(dt * 0.5), (float)dt, 0.f};
+    const float* uin[4] = {c, w0, w1, w0};
+    float* uout[4] = {w0, w1, w0, c};
+    for (int s = 0; s < nsteps; ++s)
+        for (int k = 0; k < 4; ++k) {
+            A.stage = k + 1;
+            A.a = shift[k];
+            A.u = uin[k];
+            A.out = uout[k];
+            launch_shape(E->shape, E->grid, A, st);
+        }
+}
+
+int phx_ensemble_rk4_steps(phx_ensemble* E, float* c, float* work, double dt, int nsteps, phx_stream s) {
+    if (!E || !c || !work) return fail(PHX_ERR_INVALID, "phx_ensemble_rk4_steps: NULL argument");
+    if (nsteps < 0 || !(dt > 0.0)) return fail(PHX_ERR_INVALID, "phx_ensemble_rk4_steps: bad nsteps or dt");
+    enqueue_steps(E, c, work, dt, nsteps, as_stream(s));
+    PHX_LAUNCH_CHECK("phx_ensemble_rk4_steps");
+    return PHX_OK;
+}
+
+int phx_ensemble_graph_steps(phx_ensemble* E, float* c, float* work, double dt, int steps_per_graph, int nreplays,
+                             phx_stream s) {
+    if (!E || !c || !work || steps_per_graph <= 0 || nreplays < 0 || !(dt > 0.0))
+        return fail(PHX_ERR_INVALID, "phx_ensemble_graph_steps: bad arguments");
+    hipStream_t st = as_stream(s);
+    const void* key[4] = {c, work, (const void*)(intptr_t)steps_per_graph, (const void*)(intptr_t)E->shape};
+    if (!E->graph || std::memcmp(key, E->graph_key, sizeof(key)) != 0 || E->dt != dt) {
+        if (E->graph) {
+            (void)hipGraphExecDestroy(E->graph);
+            E->graph = nullptr;
+        }
+        hipStream_t cap;
+        PHX_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        hipGraph_t g;
+        PHX_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+        enqueue_steps(E, c, work, dt, steps_per_graph, cap);
+        PHX_HIP(hipStreamEndCapture(cap, &g));
+        PHX_HIP(hipGraphInstantiate(&E->graph, g, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(g);
+        (void)hipStreamDestroy(cap);
+        std::memcpy(E->graph_key, key, sizeof(key));
+        E->dt = dt;
+        E->graph_steps = steps_per_graph;
+    }
+    for (int r = 0; r < nreplays; ++r) PHX_HIP(hipGraphLaunch(E->graph, st));
+    return PHX_OK;
+}
+
+int phx_ensemble_mass(const phx_ensemble* E, const float* c, double* out_dev, phx_stream s) {
+    if (!E || !c || !out_dev) return fail(PHX_ERR_INVALID, "phx_ensemble_mass: NULL argument");
+    hipLaunchKernelGGL(k_ens_mass, dim3(E->nruns), dim3(256), 0, as_stream(s), c, E->d_runs, out_dev);
+    PHX_LAUNCH_CHECK("phx_ensemble_mass");
+    return PHX_OK;
+}
+
+// Times every (shape, grid) candidate on `steps` RK4 steps of the caller's
+// state (which it advances: pass a scratch copy) and keeps the fastest.
+int phx_ensemble_autotune(phx_ensemble* E, float* c, float* work, double dt, int steps, double* best_ms,
+                          phx_stream s) {
+    if (!E || !c || !work || steps <= 0) return fail(PHX_ERR_INVALID, "phx_ensemble_autotune: bad arguments");
+    hipStream_t st = as_stream(s);
+    hipEvent_t e0, e1;
+    PHX_HIP(hipEventCreate(&e0));
+    PHX_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    int bs = 0, bg = 0;
+    const int mult[] = {2, 4, 8, 16};
+    for (int sh = 0; sh < kNumShapes; ++sh)
+        for (int m : mult) {
+            int grid = std::min(E->ntiles[sh], m * E->cus);
+            E->shape = sh;
+            E->grid = grid;
+            enqueue_steps(E, c, work, dt, 1, st);   // warm
+            PHX_HIP(hipEventRecord(e0, st));
+            enqueue_steps(E, c, work, dt, steps, st);
+            PHX_HIP(hipEventRecord(e1, st));
+            PHX_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            PHX_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) {
+                best = ms;
+                bs = sh;
+                bg = grid;
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (E->graph) {
+        (void)hipGraphExecDestroy(E->graph);
+        E->graph = nullptr;
+    }
+    E->shape = bs;
+    E->grid = bg;
+    if (best_ms) *best_ms = best / steps;
+    PHX_LAUNCH_CHECK("phx_ensemble_autotune");
+    return PHX_OK;
+}
+
+}  // extern "C"

@@ -42,3 +42,38 @@ def to_np(t):
 
 def stream():
     return capi.stream_handle()
+
+
+# ---------------------------------------------------------------- .field IO
+def write_field(path, phys, name="c", ghost=1):
+    """Reference BINARY .field: text header then physical FP64, x fastest.
+    phys has numpy shape (nz, ny, nx)."""
+    nz, ny, nx = phys.shape
+    with open(path, "wb") as f:
+        f.write(f"# PhiX ScalarField\nname    {name}\nnx {nx}  ny {ny}  nz {nz}\nghost   {ghost}\n---\n".encode())
+        f.write(np.ascontiguousarray(phys, dtype="<f8").tobytes())
+
+
+def write_vfield(path, phys, name="eta", ghost=1):
+    """Reference BINARY .vfield; phys has shape (ncomp, nz, ny, nx)."""
+    nc, nz, ny, nx = phys.shape
+    with open(path, "wb") as f:
+        f.write((f"# PhiX VectorField\nname         {name}\nnComponents  {nc}\nnx {nx}  ny {ny}  nz {nz}\n"
+                 f"ghost        {ghost}\n---\n").encode())
+        f.write(np.ascontiguousarray(phys, dtype="<f8").tobytes())
+
+
+def read_field(path):
+    """Returns (header dict, data) of a .field/.vfield file; data is
+    (ncomp, nz, ny, nx)."""
+    raw = open(path, "rb").read()
+    end = raw.index(b"\n---\n") + 5
+    meta = {}
+    for line in raw[:end].decode().splitlines():
+        toks = line.split()
+        for i, t in enumerate(toks[:-1]):
+            if t in ("nx", "ny", "nz", "ghost", "nComponents"):
+                meta[t] = int(toks[i + 1])
+    nc = meta.get("nComponents", 1)
+    data = np.frombuffer(raw[end:], dtype="<f8").reshape(nc, meta["nz"], meta["ny"], meta["nx"])
+    return meta, data
