@@ -109,6 +109,13 @@ def main(argv=None) -> int:
     def run(n):
         capi.call("phx_ch_rk4_steps", C.byref(g), dtype, capi.ptr(c), capi.ptr(work), C.byref(p), n, sh)
 
+    # launch shape from the autotuner, timed on a scratch copy of the state
+    zc = C.c_int(0)
+    tune_ms = C.c_double(0)
+    scratch = c.clone()
+    capi.call("phx_ch_rk4_autotune", C.byref(g), dtype, capi.ptr(scratch), capi.ptr(work), C.byref(p), 10,
+              C.byref(zc), C.byref(tune_ms), sh)
+    del scratch
     run(a.warmup)
     torch.cuda.synchronize()
     ev0 = torch.cuda.Event(enable_timing=True)
@@ -155,7 +162,7 @@ def main(argv=None) -> int:
         "data": "synthetic (c0 = 0.4 + U(-0.02, 0.02), SplitMix64 seed 2)",
         "config": {"workload": "ch3d_rk4_256cube_noflux", "grid": [NX, NX, NX], "bc": "noflux", "scheme": "rk4",
                    "dt": 0.005, "kappa": 1.0, "M": 1.0, "f": "c^2(1-c)^2", "bulk": a.dtype,
-                   "parallelism": "single-gpu"},
+                   "parallelism": "single-gpu", "zchunk": zc.value},
         "roofline": roof,
         "cpu_baseline": cpu,
         "host": {"nproc": os.cpu_count(), "cpu": cpu_model()},

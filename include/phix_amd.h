@@ -164,6 +164,15 @@ size_t phx_ch_rk4_workspace_bytes(const phx_grid* grid, int dtype);
 int phx_ch_rk4_steps(const phx_grid* grid, int dtype, void* c, void* work,
                      const phx_ch_params* p, int nsteps, phx_stream stream);
 
+/* Launch shape of the 3D stage kernel: z planes streamed per workgroup
+ * (default 16), per (n, dtype). The autotuner times {4,8,16,32,64} on `steps`
+ * steps of the given state (advancing it: pass a scratch copy), keeps the
+ * fastest and reports it. Both are host-side settings; no device allocation. */
+int phx_ch_rk4_set_zchunk(const phx_grid* grid, int dtype, int zc);
+int phx_ch_rk4_autotune(const phx_grid* grid, int dtype, void* c, void* work,
+                        const phx_ch_params* p, int steps, int* best_zc,
+                        double* best_ms_per_step, phx_stream stream);
+
 /* ------------------------------------------ FP64 diagnostics ---- */
 /* Workspace in bytes for phx_mass_energy. */
 size_t phx_reduce_workspace_bytes(const phx_grid* grid);
@@ -220,6 +229,43 @@ int phx_ensemble_get_tiling(const phx_ensemble* ens, int* shape, int* grid, int*
  * returns its ms per step. Blocks the host. */
 int phx_ensemble_autotune(phx_ensemble* ens, float* c, float* work, double dt, int steps,
                           double* best_ms_per_step, phx_stream stream);
+
+/* ------------------------------------------------ learned surrogate ---- */
+/* No reference counterpart. A 6-layer 3x3 circular-padding conv net mapping
+ * c(t) -> c(t + 100 dt) on B periodic H x W FP32 fields (H, W multiples of 16):
+ *   relu(conv 1->32), 4 x relu(conv 32->32), conv 32->1 = d,
+ *   c' = c + d - mean(d)  (exact per-sample mean projection).
+ * BF16 weights and activations, FP32 accumulation; the 32->32 layers run on
+ * v_mfma_f32_16x16x32_bf16. params: phx_sur_param_count() FP32 master weights,
+ * [out][in][kh][kw] per layer followed by its bias, layers in order.
+ * ws: phx_sur_workspace_bytes(B, H, W) bytes; forward keeps the activations
+ * there for the following backward. */
+int    phx_sur_param_count(void);
+size_t phx_sur_workspace_bytes(int B, int H, int W);
+int    phx_sur_forward(const float* params, void* ws, const float* c_in, float* c_out,
+                       int B, int H, int W, phx_stream stream);
+/* Loss = mean((c_out - c_target)^2); grad (param layout) <- dLoss/dparams;
+ * loss_dev (device double, may be NULL) <- sum of squared errors (divide by
+ * B*H*W for the loss). Needs the preceding phx_sur_forward on the same ws. */
+int    phx_sur_backward(const float* params, void* ws, const float* c_in, const float* c_out,
+                        const float* c_target, float* grad, double* loss_dev,
+                        int B, int H, int W, phx_stream stream);
+/* Adam (Kingma & Ba 2015) on the master weights, bias-corrected at `step` >= 1. */
+int    phx_sur_adam(float* params, const float* grad, float* m, float* v, int step,
+                    double lr, double beta1, double beta2, double eps, phx_stream stream);
+/* One 32->32 layer, as the network runs it (for parity tests).
+ * x, y, dy, dx, dz, mask: BF16 NHWC [B][H][W][32]; w: FP32 [32][32][3][3];
+ * ws: phx_conv3x3_ws_bytes() bytes of scratch.
+ *   fwd  : y  = relu(conv(x, w) + bias)
+ *   dgrad: dx = conv^T(dy, w) * (mask > 0)
+ *   wgrad: dw = d/dw sum(dz * conv(x, w)),  db = sum over pixels of dz  */
+size_t phx_conv3x3_ws_bytes(void);
+int    phx_conv3x3_bf16_fwd(const void* x, const float* w, const float* bias, void* y, void* ws,
+                            int B, int H, int W, phx_stream stream);
+int    phx_conv3x3_bf16_dgrad(const void* dy, const float* w, const void* mask, void* dx, void* ws,
+                              int B, int H, int W, phx_stream stream);
+int    phx_conv3x3_bf16_wgrad(const void* x, const void* dz, float* dw, float* db, void* ws,
+                              int B, int H, int W, phx_stream stream);
 
 #ifdef __cplusplus
 }

@@ -120,6 +120,9 @@ SIG = {
     "phx_ch_euler_steps": (C.c_int, [GP, C.c_int, VP, VP, C.POINTER(CHParams), C.c_int, VP]),
     "phx_ch_rk4_workspace_bytes": (C.c_size_t, [GP, C.c_int]),
     "phx_ch_rk4_steps": (C.c_int, [GP, C.c_int, VP, VP, C.POINTER(CHParams), C.c_int, VP]),
+    "phx_ch_rk4_set_zchunk": (C.c_int, [GP, C.c_int, C.c_int]),
+    "phx_ch_rk4_autotune": (C.c_int, [GP, C.c_int, VP, VP, C.POINTER(CHParams), C.c_int, C.POINTER(C.c_int),
+                                      C.POINTER(C.c_double), VP]),
     "phx_reduce_workspace_bytes": (C.c_size_t, [GP]),
     "phx_mass_energy": (C.c_int, [GP, C.c_int, VP, C.POINTER(CHParams), VP, VP, VP]),
     "phx_ac_rk4_workspace_bytes": (C.c_size_t, [GP, C.c_int, C.c_int]),
@@ -136,6 +139,15 @@ SIG = {
     "phx_ensemble_get_tiling": (C.c_int, [VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                           C.POINTER(C.c_int)]),
     "phx_ensemble_autotune": (C.c_int, [VP, VP, VP, C.c_double, C.c_int, C.POINTER(C.c_double), VP]),
+    "phx_sur_param_count": (C.c_int, []),
+    "phx_sur_workspace_bytes": (C.c_size_t, [C.c_int, C.c_int, C.c_int]),
+    "phx_sur_forward": (C.c_int, [VP, VP, VP, VP, C.c_int, C.c_int, C.c_int, VP]),
+    "phx_sur_backward": (C.c_int, [VP, VP, VP, VP, VP, VP, VP, C.c_int, C.c_int, C.c_int, VP]),
+    "phx_sur_adam": (C.c_int, [VP, VP, VP, VP, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, VP]),
+    "phx_conv3x3_ws_bytes": (C.c_size_t, []),
+    "phx_conv3x3_bf16_fwd": (C.c_int, [VP, VP, VP, VP, VP, C.c_int, C.c_int, C.c_int, VP]),
+    "phx_conv3x3_bf16_dgrad": (C.c_int, [VP, VP, VP, VP, VP, C.c_int, C.c_int, C.c_int, VP]),
+    "phx_conv3x3_bf16_wgrad": (C.c_int, [VP, VP, VP, VP, VP, C.c_int, C.c_int, C.c_int, VP]),
 }
 
 

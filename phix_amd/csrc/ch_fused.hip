@@ -36,6 +36,9 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 namespace phx {
 
@@ -186,67 +189,188 @@ __global__ __launch_bounds__(256) void k_ch_stage_2d(CHArgs<T> A) {
 }
 
 // ------------------------------------------------------------------- 3D ----
-// One block = TX x TY columns of a z-chunk of A.zc planes. It streams along z
-// with a ring of 4 halo planes of the stage state and 3 mu planes in LDS: each
-// step loads one new plane, computes one mu plane and writes one output plane.
-// Ring reuse is safe with two barriers per step: halo plane p is overwritten
-// by plane p+4, three barriers after its last reader (mu(p+1) and output p);
-// mu plane m is overwritten by m+3 after the barrier that follows output m+1.
+// One block = TX x TY columns of a z-chunk of A.zc planes, streaming along z
+// with a ring of 4 halo planes of the stage state and 3 mu planes in LDS.
+// Software pipelined: while plane o is computed, the halo plane o+3 and the
+// next output plane's c/acc values are already in flight to registers, so the
+// HBM latency overlaps the LDS work instead of sitting behind a barrier.
+// Per-thread load offsets (with the x/y BC maps folded in) are computed once.
+//
+// Ring reuse, with barriers A and B of every iteration o:
+//   halo plane o+2 is stored (before A) into the slot of plane o-2, whose last
+//   readers (mu(o-1) and output o-2) ran before iteration o-1's barriers;
+//   mu(o+1) is written (between A and B) into the slot of mu(o-2), last read
+//   by output o-1 before barrier A of iteration o.
 template <typename T, int TX, int TY, bool EXACT>
 __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
     constexpr int UX = TX + 4, UY = TY + 4, MX = TX + 2, MY = TY + 2;
-    __shared__ T U[4][UY][UX];
-    __shared__ T MU[3][MY][MX];
+    constexpr int NU = (UX * UY + 255) / 256;
+    constexpr int NM = (MX * MY + 255) / 256;
+    constexpr int NO = (TX * TY) / 256;
+    static_assert((TX * TY) % 256 == 0, "tile must be a multiple of 256 cells");
+    __shared__ T U[4][UY * UX];
+    __shared__ T MU[3][MY * MX];
     const int tid = threadIdx.x;
     const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
     const int z0 = blockIdx.z * A.zc;
     const int z1 = min(z0 + A.zc, A.L.n2);
-    const bool xy_interior = (x0 >= 2) && (x0 + TX + 1 < A.L.n0) && (y0 >= 2) && (y0 + TY + 1 < A.L.n1);
+    const int g = A.L.g;
+    const long long sxy = A.L.sxy;
 
-    for (int pz = z0 - 2; pz <= z1 + 1; ++pz) {
-        // ---- load halo plane pz
-        const int us = (pz + 4) & 3;
-        if (xy_interior && pz >= 0 && pz < A.L.n2) {
-            const T* base = A.u + A.L.at(x0 - 2, y0 - 2, pz);
-            for (int t = tid; t < UX * UY; t += 256) {
-                int ly = t / UX, lx = t - ly * UX;
-                U[us][ly][lx] = base[(long long)ly * A.L.sx + lx];
+    // halo slots: in-plane offset (x/y maps applied) or -1 for a FIXED constant
+    int hoff[NU];
+    T hfix[NU];
+#pragma unroll
+    for (int i = 0; i < NU; ++i) {
+        const int t = tid + 256 * i;
+        hoff[i] = 0;
+        hfix[i] = (T)0;
+        if (t < UX * UY) {
+            const int ly = t / UX, lx = t - ly * UX;
+            const int px = x0 - 2 + lx, py = y0 - 2 + ly;
+            const int qy = map_in(py, A.by), qx = map_in(px, A.bx);
+            if (qy < 0) {
+                hoff[i] = -1;
+                hfix[i] = fixed_value(py, A.by);
+            } else if (qx < 0) {
+                hoff[i] = -1;
+                hfix[i] = fixed_value(px, A.bx);
+            } else {
+                hoff[i] = (qy + g) * A.L.sx + (qx + g);
             }
+        }
+    }
+    // output slots
+    int ooff[NO];
+    bool oval[NO];
+#pragma unroll
+    for (int j = 0; j < NO; ++j) {
+        const int t = tid + 256 * j;
+        const int oy = t / TX, ox = t - oy * TX;
+        const int x = x0 + ox, y = y0 + oy;
+        oval[j] = x < A.L.n0 && y < A.L.n1;
+        ooff[j] = (y + g) * A.L.sx + (x + g);
+    }
+
+    auto load_plane = [&](int pz, T (&r)[NU]) {
+        const int qz = map_in(pz, A.bz);
+        if (qz < 0) {
+            const T v = fixed_value(pz, A.bz);
+#pragma unroll
+            for (int i = 0; i < NU; ++i) r[i] = v;
         } else {
-            for (int t = tid; t < UX * UY; t += 256) {
-                int ly = t / UX, lx = t - ly * UX;
-                U[us][ly][lx] = load_halo<T, 3>(A.u, A, x0 - 2 + lx, y0 - 2 + ly, pz);
+            const T* base = A.u + (long long)(qz + g) * sxy;
+#pragma unroll
+            for (int i = 0; i < NU; ++i)
+                if (tid + 256 * i < UX * UY) r[i] = hoff[i] >= 0 ? base[hoff[i]] : hfix[i];
+        }
+    };
+    auto store_plane = [&](int pz, const T (&r)[NU]) {
+        T* dst = U[(pz + 4) & 3];
+#pragma unroll
+        for (int i = 0; i < NU; ++i)
+            if (tid + 256 * i < UX * UY) dst[tid + 256 * i] = r[i];
+    };
+    auto compute_mu = [&](int m) {
+        const T* ul = U[(m + 3) & 3];
+        const T* um = U[(m + 4) & 3];
+        const T* uh = U[(m + 5) & 3];
+        T* dst = MU[(m + 3) % 3];
+#pragma unroll
+        for (int i = 0; i < NM; ++i) {
+            const int t = tid + 256 * i;
+            if (t < MX * MY) {
+                const int my = t / MX, mx = t - my * MX;
+                const int c0 = (my + 1) * UX + mx + 1;
+                const T c = um[c0];
+                dst[t] = mu_at<T, 3>(A, x0 - 1 + mx, y0 - 1 + my, m, c, um[c0 + 1], um[c0 - 1], um[c0 + UX],
+                                     um[c0 - UX], uh[c0], ul[c0]);
             }
         }
-        __syncthreads();
-        if (pz < z0) continue;   // uniform across the block
-        // ---- mu plane m = pz-1 from halo planes pz-2, pz-1, pz
-        const int m = pz - 1;
-        const int ul = (pz + 2) & 3, um = (pz + 3) & 3, uh = us;
-        const int ms = (m + 3) % 3;
-        for (int t = tid; t < MX * MY; t += 256) {
-            int my = t / MX, mx = t - my * MX;
-            T c = U[um][my + 1][mx + 1];
-            MU[ms][my][mx] = mu_at<T, 3>(A, x0 - 1 + mx, y0 - 1 + my, m, c, U[um][my + 1][mx + 2], U[um][my + 1][mx],
-                                         U[um][my + 2][mx + 1], U[um][my][mx + 1], U[uh][my + 1][mx + 1],
-                                         U[ul][my + 1][mx + 1]);
+    };
+    // epilogue operands of one output plane
+    const int st = A.stage;
+    const bool need_c = st == 2 || st == 3 || (EXACT && st == 4);
+    const bool need_a = st >= 2;
+    T pc[NO], pa[NO], nc[NO], na[NO];
+    auto load_epi = [&](int o, T (&rc)[NO], T (&ra)[NO]) {
+        const long long pb = (long long)(o + g) * sxy;
+#pragma unroll
+        for (int j = 0; j < NO; ++j) {
+            if (!oval[j]) continue;
+            if (need_c) rc[j] = A.c[pb + ooff[j]];
+            if (need_a) ra[j] = A.acc[pb + ooff[j]];
         }
-        __syncthreads();
-        // ---- output plane o = pz-2 from mu planes o-1, o, o+1
-        const int o = pz - 2;
-        if (o < z0) continue;
-        const int mlo = (o + 2) % 3, mmid = (o + 3) % 3, mhi = (o + 4) % 3;
-        const int uo = (o + 4) & 3;
-        for (int t = tid; t < TX * TY; t += 256) {
-            int oy = t / TX, ox = t - oy * TX;
-            int x = x0 + ox, y = y0 + oy;
-            if (x >= A.L.n0 || y >= A.L.n1) continue;
-            T mc = MU[mmid][oy + 1][ox + 1];
-            T val = (MU[mmid][oy + 1][ox + 2] - (T)2 * mc + MU[mmid][oy + 1][ox]) * A.idx2;
-            val += (MU[mmid][oy + 2][ox + 1] - (T)2 * mc + MU[mmid][oy][ox + 1]) * A.idy2;
-            val += (MU[mhi][oy + 1][ox + 1] - (T)2 * mc + MU[mlo][oy + 1][ox + 1]) * A.idz2;
-            T k = A.M * val;
-            epilogue<T, EXACT>(A, A.L.at(x, y, o), U[uo][oy + 2][ox + 2], k);
+    };
+
+    // ---- prologue: halo planes z0-2 .. z0+1, mu planes z0-1, z0
+    T pre[NU];
+    load_plane(z0 - 2, pre);
+    store_plane(z0 - 2, pre);
+    load_plane(z0 - 1, pre);
+    store_plane(z0 - 1, pre);
+    load_plane(z0, pre);
+    store_plane(z0, pre);
+    load_plane(z0 + 1, pre);
+    store_plane(z0 + 1, pre);
+    load_plane(z0 + 2, pre);
+    load_epi(z0, pc, pa);
+    __syncthreads();
+    compute_mu(z0 - 1);
+    compute_mu(z0);
+    __syncthreads();
+
+    for (int o = z0; o < z1; ++o) {
+        store_plane(o + 2, pre);
+        __syncthreads();   // A
+        if (o + 3 <= z1 + 1) load_plane(o + 3, pre);
+        if (o + 1 < z1) load_epi(o + 1, nc, na);
+        compute_mu(o + 1);
+        __syncthreads();   // B
+        const T* mlo = MU[(o + 2) % 3];
+        const T* mmid = MU[(o + 3) % 3];
+        const T* mhi = MU[(o + 4) % 3];
+        const T* uo = U[(o + 4) & 3];
+        const long long pb = (long long)(o + g) * sxy;
+#pragma unroll
+        for (int j = 0; j < NO; ++j) {
+            if (!oval[j]) continue;
+            const int t = tid + 256 * j;
+            const int oy = t / TX, ox = t - oy * TX;
+            const int m0 = (oy + 1) * MX + ox + 1;
+            const T mc = mmid[m0];
+            T val = (mmid[m0 + 1] - (T)2 * mc + mmid[m0 - 1]) * A.idx2;
+            val += (mmid[m0 + MX] - (T)2 * mc + mmid[m0 - MX]) * A.idy2;
+            val += (mhi[m0] - (T)2 * mc + mlo[m0]) * A.idz2;
+            const T k = A.M * val;
+            const long long gi = pb + ooff[j];
+            const T uc = uo[(oy + 2) * UX + ox + 2];
+            if (EXACT) {
+                if (st == 1) {
+                    A.acc[gi] = k;
+                    A.out[gi] = uc + A.a * k;
+                } else if (st < 4) {
+                    A.acc[gi] = pa[j] + (T)2 * k;
+                    A.out[gi] = pc[j] + A.a * k;
+                } else {
+                    A.out[gi] = pc[j] + A.dt6 * (pa[j] + k);
+                }
+            } else {
+                if (st == 1) {
+                    A.acc[gi] = uc + A.dt6 * k;
+                    A.out[gi] = uc + A.a * k;
+                } else if (st < 4) {
+                    A.acc[gi] = pa[j] + A.dt3 * k;
+                    A.out[gi] = pc[j] + A.a * k;
+                } else {
+                    A.out[gi] = pa[j] + A.dt6 * k;
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NO; ++j) {
+            pc[j] = nc[j];
+            pa[j] = na[j];
         }
     }
 }
@@ -396,7 +520,8 @@ int bc_fill_impl(const phx_grid* G, int dtype, void* data, hipStream_t st);
 
 // Tile shapes (autotuned later; these are the defaults).
 constexpr int T2X = 64, T2Y = 16;
-constexpr int T3X = 64, T3Y = 8;
+constexpr int T3X = 64;
+template <typename T> struct T3Y { static constexpr int v = sizeof(T) == 4 ? 16 : 8; };
 
 template <typename T, bool EXACT>
 static int rk4_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_params* p, int nsteps, hipStream_t st,
@@ -421,7 +546,7 @@ static int rk4_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_params* p,
     const T* uin[4] = {c, w0, w1, w0};
     T* uout[4] = {w0, w1, w0, c};
     dim3 grid;
-    if (G->dim == 3) grid = dim3(cdiv(A.L.n0, T3X), cdiv(A.L.n1, T3Y), cdiv(A.L.n2, zc));
+    if (G->dim == 3) grid = dim3(cdiv(A.L.n0, T3X), cdiv(A.L.n1, T3Y<T>::v), cdiv(A.L.n2, zc));
     else grid = dim3(cdiv(A.L.n0, T2X), cdiv(A.L.n1, T2Y), 1);
     for (int s = 0; s < nsteps; ++s) {
         if (s == nsteps - 1) {
@@ -433,7 +558,7 @@ static int rk4_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_params* p,
             A.u = uin[stg];
             A.out = uout[stg];
             A.a = shift[stg];
-            if (G->dim == 3) hipLaunchKernelGGL((k_ch_stage_3d<T, T3X, T3Y, EXACT>), grid, dim3(256), 0, st, A);
+            if (G->dim == 3) hipLaunchKernelGGL((k_ch_stage_3d<T, T3X, T3Y<T>::v, EXACT>), grid, dim3(256), 0, st, A);
             else if (G->dim == 2) hipLaunchKernelGGL((k_ch_stage_2d<T, 2, T2X, T2Y, EXACT>), grid, dim3(256), 0, st, A);
             else hipLaunchKernelGGL((k_ch_stage_2d<T, 1, T2X, T2Y, EXACT>), grid, dim3(256), 0, st, A);
         }
@@ -475,6 +600,26 @@ size_t phx_ch_rk4_workspace_bytes(const phx_grid* G, int dtype) {
     return 3 * (size_t)make_layout(G).total() * (dtype == PHX_F64 ? 8 : 4);
 }
 
+// z planes per block of the 3D stage kernel: tuned per (shape, dtype) by
+// phx_ch_rk4_autotune, else 16.
+static std::mutex g_zc_mu;
+static std::map<std::tuple<int, int, int, int>, int> g_zc;
+
+static int zchunk_for(const phx_grid* G, int dtype) {
+    std::lock_guard<std::mutex> lk(g_zc_mu);
+    auto it = g_zc.find(std::make_tuple(G->n[0], G->n[1], G->n[2], dtype));
+    return it == g_zc.end() ? 16 : it->second;
+}
+
+int phx_ch_rk4_set_zchunk(const phx_grid* G, int dtype, int zc) {
+    int rc = check_grid(G, "phx_ch_rk4_set_zchunk");
+    if (rc) return rc;
+    if (zc < 1 || zc > 4096) return fail(PHX_ERR_INVALID, "phx_ch_rk4_set_zchunk: zc must be in [1, 4096]");
+    std::lock_guard<std::mutex> lk(g_zc_mu);
+    g_zc[std::make_tuple(G->n[0], G->n[1], G->n[2], dtype)] = zc;
+    return PHX_OK;
+}
+
 int phx_ch_rk4_steps(const phx_grid* G, int dtype, void* c, void* work, const phx_ch_params* p, int nsteps,
                      phx_stream s) {
     int rc = check_grid_fused(G, "phx_ch_rk4_steps");
@@ -484,10 +629,47 @@ int phx_ch_rk4_steps(const phx_grid* G, int dtype, void* c, void* work, const ph
     if (!c || !work) return fail(PHX_ERR_INVALID, "phx_ch_rk4_steps: NULL buffer");
     if (nsteps < 0) return fail(PHX_ERR_INVALID, "phx_ch_rk4_steps: nsteps < 0");
     if (nsteps == 0) return PHX_OK;
-    const int zc = 32;
+    const int zc = G->dim == 3 ? zchunk_for(G, dtype) : 1;
     return dtype == PHX_F64
                ? rk4_steps_t<double, true>(G, (double*)c, (double*)work, p, nsteps, as_stream(s), zc)
                : rk4_steps_t<float, false>(G, (float*)c, (float*)work, p, nsteps, as_stream(s), zc);
+}
+
+// Times the 3D stage kernel at z-chunks {4, 8, 16, 32, 64} on `steps` steps of
+// the caller's state (advancing it) and keeps the fastest for this shape.
+int phx_ch_rk4_autotune(const phx_grid* G, int dtype, void* c, void* work, const phx_ch_params* p, int steps,
+                        int* best_zc, double* best_ms_per_step, phx_stream s) {
+    int rc = check_grid_fused(G, "phx_ch_rk4_autotune");
+    if (rc) return rc;
+    if (steps < 1) return fail(PHX_ERR_INVALID, "phx_ch_rk4_autotune: steps < 1");
+    hipStream_t st = as_stream(s);
+    const int cand[] = {4, 8, 16, 32, 64};
+    float best = 1e30f;
+    int bz = 16;
+    hipEvent_t e0, e1;
+    PHX_HIP(hipEventCreate(&e0));
+    PHX_HIP(hipEventCreate(&e1));
+    for (int zc : cand) {
+        if (G->dim == 3 && zc > G->n[2]) continue;
+        if ((rc = phx_ch_rk4_set_zchunk(G, dtype, zc))) return rc;
+        if ((rc = phx_ch_rk4_steps(G, dtype, c, work, p, 1, s))) return rc;
+        PHX_HIP(hipEventRecord(e0, st));
+        if ((rc = phx_ch_rk4_steps(G, dtype, c, work, p, steps, s))) return rc;
+        PHX_HIP(hipEventRecord(e1, st));
+        PHX_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        PHX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            bz = zc;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    phx_ch_rk4_set_zchunk(G, dtype, bz);
+    if (best_zc) *best_zc = bz;
+    if (best_ms_per_step) *best_ms_per_step = best / steps;
+    return PHX_OK;
 }
 
 }  // extern "C"

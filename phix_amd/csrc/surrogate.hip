@@ -1,0 +1,607 @@
+// surrogate.hip -- learned surrogate for 2D periodic Cahn-Hilliard on gfx950.
+//
+// No reference counterpart (PhiX has no learned part). The network advances
+// c(t) -> c(t + 100 dt):
+//   x   = c - 0.5
+//   a1  = relu(conv3x3(x;  1 -> 32))            FP32 VALU, BF16 NHWC out
+//   a_l = relu(conv3x3(a_{l-1}; 32 -> 32)), l=2..5   BF16 MFMA implicit GEMM
+//   d   = conv3x3(a5; 32 -> 1)                  VALU, FP32 out
+//   c'  = c + d - mean(d)                       exact mean projection
+// All convolutions use circular padding. Activations are BF16 (NHWC: the 32
+// channels of a pixel are 64 contiguous bytes), accumulation is FP32.
+//
+// The 32->32 layers are implicit GEMMs on v_mfma_f32_16x16x32_bf16: for each
+// of the 9 taps, K = 32 input channels is exactly one MFMA K step. A block
+// computes a 16x16 pixel tile x 32 output channels with 4 waves (4 rows each),
+// the (16+2)^2 input halo tile staged once in LDS, the weights held in VGPRs.
+// Data gradients reuse the same kernel with flipped, transposed weights and a
+// ReLU mask; weight gradients are a K = pixels MFMA GEMM per tap with
+// per-block partials reduced in a fixed order (deterministic, no atomics).
+//
+// MFMA 16x16x32 bf16 lane layout (checked by tests/test_gpu_surrogate.py):
+//   A (16x32): lane l holds A[l%16][8*(l/16) + j], j = 0..7
+//   B (32x16): lane l holds B[8*(l/16) + j][l%16]
+//   D (16x16): lane l holds D[4*(l/16) + r][l%16], r = 0..3
+#include "common.hpp"
+
+#include <hip/hip_bf16.h>
+
+namespace phx {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int CH = 32;
+constexpr int NTAP = 9;
+constexpr int PIXP = 40;          // LDS pixel pitch in bf16 (80 B: 16 B aligned, spreads banks)
+constexpr int WG_BLOCKS = 512;    // persistent grid of the weight-gradient kernels
+constexpr int MID_W = NTAP * CH * CH;   // 9216
+constexpr int PART = MID_W + CH;        // per-block partial: dW + db
+
+// parameter layout (FP32 master, PyTorch-style [out][in][kh][kw])
+constexpr int P_W1 = 0, P_B1 = P_W1 + CH * 9;
+constexpr int P_MID = P_B1 + CH;                 // layer l (0..3): w at P_MID + l*(MID_W+CH), b after it
+constexpr int P_W6 = P_MID + 4 * (MID_W + CH), P_B6 = P_W6 + CH * 9;
+constexpr int P_COUNT = P_B6 + 1;
+
+__device__ __forceinline__ int wrap(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+
+__device__ __forceinline__ float bf2f(__bf16 v) { return (float)v; }
+__device__ __forceinline__ __bf16 f2bf(float v) { return (__bf16)v; }
+
+// ------------------------------------------------------------ weight pack --
+// wfwd[l][tap][co][ci] = w_l[co][ci][kh][kw];  wbwd[l][tap'][ci][co] = w_l[co][ci][2-kh'][2-kw']
+__global__ void k_pack(const float* __restrict__ P, __bf16* __restrict__ wfwd, __bf16* __restrict__ wbwd) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 4 * MID_W) return;
+    int l = t / MID_W, r = t % MID_W;
+    int tap = r / (CH * CH), co = (r / CH) % CH, ci = r % CH;
+    const float* w = P + P_MID + l * (MID_W + CH);
+    wfwd[t] = f2bf(w[(co * CH + ci) * 9 + tap]);
+    // backward: output channel = ci of the forward, input channel = co
+    int tf = 8 - tap;
+    wbwd[l * MID_W + (tap * CH + ci) * CH + co] = f2bf(w[(co * CH + ci) * 9 + tf]);
+}
+
+// --------------------------------------------------------- first layer --
+__global__ __launch_bounds__(256) void k_conv_in_fwd(const float* __restrict__ c, const float* __restrict__ P,
+                                                     __bf16* __restrict__ out, int B, int H, int W) {
+    long long p = blockIdx.x * 256LL + threadIdx.x;
+    if (p >= (long long)B * H * W) return;
+    int x = (int)(p % W), y = (int)((p / W) % H);
+    long long base = p - (long long)y * W - x;
+    float v[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        int yy = wrap(y + t / 3 - 1, H), xx = wrap(x + t % 3 - 1, W);
+        v[t] = c[base + (long long)yy * W + xx] - 0.5f;
+    }
+    __attribute__((aligned(16))) __bf16 o[CH];
+#pragma unroll
+    for (int co = 0; co < CH; ++co) {
+        float s = P[P_B1 + co];
+#pragma unroll
+        for (int t = 0; t < 9; ++t) s += P[P_W1 + co * 9 + t] * v[t];
+        o[co] = f2bf(fmaxf(s, 0.f));
+    }
+    uint4* dst = reinterpret_cast<uint4*>(out + p * CH);
+    const uint4* src = reinterpret_cast<const uint4*>(o);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) dst[q] = src[q];
+}
+
+// ----------------------------------------------------------- last layer --
+__global__ __launch_bounds__(256) void k_conv_out_fwd(const __bf16* __restrict__ a, const float* __restrict__ P,
+                                                      float* __restrict__ d, int B, int H, int W) {
+    long long p = blockIdx.x * 256LL + threadIdx.x;
+    if (p >= (long long)B * H * W) return;
+    int x = (int)(p % W), y = (int)((p / W) % H);
+    long long base = p - (long long)y * W - x;
+    float s = P[P_B6];
+    for (int t = 0; t < 9; ++t) {
+        int yy = wrap(y + t / 3 - 1, H), xx = wrap(x + t % 3 - 1, W);
+        const uint4* src = reinterpret_cast<const uint4*>(a + (base + (long long)yy * W + xx) * CH);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            uint4 u = src[q];
+            const __bf16* e = reinterpret_cast<const __bf16*>(&u);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) s += P[P_W6 + (q * 8 + j) * 9 + t] * bf2f(e[j]);
+        }
+    }
+    d[p] = s;
+}
+
+// dZ5[q][ci] = (sum_tap w6[ci][tap] * dd[q - off(tap)]) * (a5[q][ci] > 0)
+__global__ __launch_bounds__(256) void k_conv_out_bwd(const float* __restrict__ dd, const __bf16* __restrict__ a5,
+                                                      const float* __restrict__ P, __bf16* __restrict__ dz, int B,
+                                                      int H, int W) {
+    long long p = blockIdx.x * 256LL + threadIdx.x;
+    if (p >= (long long)B * H * W) return;
+    int x = (int)(p % W), y = (int)((p / W) % H);
+    long long base = p - (long long)y * W - x;
+    float g[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        int yy = wrap(y - (t / 3 - 1), H), xx = wrap(x - (t % 3 - 1), W);
+        g[t] = dd[base + (long long)yy * W + xx];
+    }
+    const uint4* am = reinterpret_cast<const uint4*>(a5 + p * CH);
+    uint4* dst = reinterpret_cast<uint4*>(dz + p * CH);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        uint4 u = am[q];
+        const __bf16* e = reinterpret_cast<const __bf16*>(&u);
+        __attribute__((aligned(16))) __bf16 o[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            int ci = q * 8 + j;
+            float s = 0.f;
+#pragma unroll
+            for (int t = 0; t < 9; ++t) s += P[P_W6 + ci * 9 + t] * g[t];
+            o[j] = f2bf(bf2f(e[j]) > 0.f ? s : 0.f);
+        }
+        dst[q] = *reinterpret_cast<const uint4*>(o);
+    }
+}
+
+// ------------------------------------------------ 32 -> 32 MFMA conv -----
+// mode 0: out = relu(conv + bias)   mode 1: out = conv * (mask > 0)
+__global__ __launch_bounds__(256) void k_conv_mid(const __bf16* __restrict__ in, const __bf16* __restrict__ wpk,
+                                                  const float* __restrict__ bias, const __bf16* __restrict__ mask,
+                                                  __bf16* __restrict__ out, int H, int W, int mode) {
+    __shared__ __attribute__((aligned(16))) __bf16 S[18 * 18 * PIXP];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int x0 = blockIdx.x * 16, y0 = blockIdx.y * 16, n = blockIdx.z;
+    const long long img = (long long)n * H * W;
+    for (int t = tid; t < 18 * 18 * 4; t += 256) {
+        int pi = t >> 2, q = t & 3;
+        int yy = wrap(y0 - 1 + pi / 18, H), xx = wrap(x0 - 1 + pi % 18, W);
+        uint4 v = *reinterpret_cast<const uint4*>(in + (img + (long long)yy * W + xx) * CH + q * 8);
+        *reinterpret_cast<uint4*>(&S[pi * PIXP + q * 8]) = v;
+    }
+    bf16x8 bw[NTAP][2];
+#pragma unroll
+    for (int tap = 0; tap < NTAP; ++tap)
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb)
+            bw[tap][nb] = *reinterpret_cast<const bf16x8*>(wpk + (tap * CH + (lane & 15) + 16 * nb) * CH + 8 * (lane >> 4));
+    __syncthreads();
+    f32x4 acc[4][2];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) acc[mb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int tap = 0; tap < NTAP; ++tap) {
+        const int dh = tap / 3, dw = tap % 3;
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+            const int row = 4 * wv + mb;
+            const bf16x8 a = *reinterpret_cast<const bf16x8*>(&S[((row + dh) * 18 + (lane & 15) + dw) * PIXP + 8 * (lane >> 4)]);
+#pragma unroll
+            for (int nb = 0; nb < 2; ++nb)
+                acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[tap][nb], acc[mb][nb], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+        const int y = y0 + 4 * wv + mb;
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+            const int co = (lane & 15) + 16 * nb;
+            const float bco = mode == 0 ? bias[co] : 0.f;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int x = x0 + 4 * (lane >> 4) + r;
+                const long long pix = img + (long long)y * W + x;
+                float v = acc[mb][nb][r];
+                if (mode == 0) v = fmaxf(v + bco, 0.f);
+                else v = bf2f(mask[pix * CH + co]) > 0.f ? v : 0.f;
+                out[pix * CH + co] = f2bf(v);
+            }
+        }
+    }
+}
+
+// ----------------------------------------- 32 -> 32 weight gradient -------
+// part[blk][tap][co][ci] = sum over the block's tiles of sum_p dz[p][co]*x[p+off(tap)][ci]
+// part[blk][MID_W + co]  = sum over the block's tiles of sum_p dz[p][co]
+__global__ __launch_bounds__(256) void k_conv_mid_wgrad(const __bf16* __restrict__ x, const __bf16* __restrict__ dz,
+                                                        float* __restrict__ part, int B, int H, int W) {
+    constexpr int XP = 18 * 18 + 4;   // row pitch of the transposed halo tile
+    constexpr int ZP = 256 + 8;       // row pitch of the transposed dz tile (16 B aligned)
+    __shared__ __attribute__((aligned(16))) __bf16 Xt[CH * XP];
+    __shared__ __attribute__((aligned(16))) __bf16 Zt[CH * ZP];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int tx = W / 16, ty = H / 16, ntiles = B * tx * ty;
+    int taps[3] = {wv, wv + 4, wv == 0 ? 8 : -1};
+    f32x4 acc[3][2][2];
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
+#pragma unroll
+        for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+            for (int nb = 0; nb < 2; ++nb) acc[s][mb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float dbs = 0.f;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int n = tile / (tx * ty), rem = tile % (tx * ty);
+        const int y0 = (rem / tx) * 16, x0 = (rem % tx) * 16;
+        const long long img = (long long)n * H * W;
+        for (int t = tid; t < 18 * 18 * 4; t += 256) {
+            int pi = t >> 2, q = t & 3;
+            int yy = wrap(y0 - 1 + pi / 18, H), xx = wrap(x0 - 1 + pi % 18, W);
+            uint4 v = *reinterpret_cast<const uint4*>(x + (img + (long long)yy * W + xx) * CH + q * 8);
+            const __bf16* e = reinterpret_cast<const __bf16*>(&v);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) Xt[(q * 8 + j) * XP + pi] = e[j];
+        }
+        for (int t = tid; t < 256 * 4; t += 256) {
+            int pi = t >> 2, q = t & 3;
+            const long long pix = img + (long long)(y0 + pi / 16) * W + x0 + pi % 16;
+            uint4 v = *reinterpret_cast<const uint4*>(dz + pix * CH + q * 8);
+            const __bf16* e = reinterpret_cast<const __bf16*>(&v);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) Zt[(q * 8 + j) * ZP + pi] = e[j];
+        }
+        __syncthreads();
+        if (tid < CH) {
+            float s = 0.f;
+            for (int p = 0; p < 256; ++p) s += bf2f(Zt[tid * ZP + p]);
+            dbs += s;
+        }
+        for (int kk = 0; kk < 256; kk += 32) {
+            const int pbase = kk + 8 * (lane >> 4);        // 8 consecutive pixels in one tile row
+            const int prow = pbase >> 4, pcol = pbase & 15;
+            bf16x8 a[2];
+#pragma unroll
+            for (int mb = 0; mb < 2; ++mb)
+                a[mb] = *reinterpret_cast<const bf16x8*>(&Zt[((lane & 15) + 16 * mb) * ZP + pbase]);
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                const int tap = taps[s];
+                if (tap < 0) continue;
+                const int dh = tap / 3, dw = tap % 3;
+#pragma unroll
+                for (int nb = 0; nb < 2; ++nb) {
+                    const __bf16* src = &Xt[((lane & 15) + 16 * nb) * XP + (prow + dh) * 18 + pcol + dw];
+                    bf16x8 b;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) b[j] = src[j];
+#pragma unroll
+                    for (int mb = 0; mb < 2; ++mb)
+                        acc[s][mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mb], b, acc[s][mb][nb], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    float* pp = part + (long long)blockIdx.x * PART;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        const int tap = taps[s];
+        if (tap < 0) continue;
+#pragma unroll
+        for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+            for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int co = 16 * mb + 4 * (lane >> 4) + r, ci = 16 * nb + (lane & 15);
+                    pp[(tap * CH + co) * CH + ci] = acc[s][mb][nb][r];
+                }
+    }
+    if (tid < CH) pp[MID_W + tid] = dbs;
+}
+
+// dW of layer l in the master layout [co][ci][kh][kw], db after it: fixed-order sum of partials.
+__global__ void k_reduce_mid(const float* __restrict__ part, int nblk, float* __restrict__ g) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= PART) return;
+    float s = 0.f;
+    for (int b = 0; b < nblk; ++b) s += part[(long long)b * PART + t];
+    if (t < MID_W) {
+        int tap = t / (CH * CH), co = (t / CH) % CH, ci = t % CH;
+        g[(co * CH + ci) * 9 + tap] = s;
+    } else {
+        g[MID_W + (t - MID_W)] = s;
+    }
+}
+
+// ------------------------------------- first/last layer weight gradients --
+// mode 0 (first layer): part[blk][co*9+tap] = sum_p x[p+off(tap)] * dz1[p][co]; part[blk][288+co] = sum_p dz1[p][co]
+// mode 1 (last layer):  part[blk][ci*9+tap] = sum_p a5[p+off(tap)][ci] * dd[p]; part[blk][288] = sum_p dd[p]
+__global__ __launch_bounds__(320) void k_edge_wgrad(const float* __restrict__ c, const __bf16* __restrict__ act,
+                                                    const float* __restrict__ dd, float* __restrict__ part, int B,
+                                                    int H, int W, int mode) {
+    const int t = threadIdx.x;   // 0..319: (channel, tap) pairs 0..287, bias 288..319
+    const long long npix = (long long)B * H * W;
+    const long long chunk = (npix + gridDim.x - 1) / gridDim.x;
+    const long long p0 = blockIdx.x * chunk, p1 = min(npix, p0 + chunk);
+    float s = 0.f;
+    const int ch = t < 288 ? t / 9 : t - 288, tap = t < 288 ? t % 9 : 0;
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    if (mode == 0 || t <= 288) {
+        for (long long p = p0; p < p1; ++p) {
+            const int x = (int)(p % W), y = (int)((p / W) % H);
+            const long long base = p - (long long)y * W - x;
+            const long long q = base + (long long)wrap(y + dy, H) * W + wrap(x + dx, W);
+            if (mode == 0) {
+                const float g = bf2f(act[p * CH + ch]);
+                s += t < 288 ? (c[q] - 0.5f) * g : g;
+            } else {
+                s += t < 288 ? bf2f(act[q * CH + ch]) * dd[p] : dd[p];
+            }
+        }
+    }
+    const int nout = mode == 0 ? 320 : 289;
+    if (t < nout) part[(long long)blockIdx.x * 320 + t] = s;
+}
+
+__global__ void k_reduce_edge(const float* __restrict__ part, int nblk, int nout, float* __restrict__ g) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nout) return;
+    float s = 0.f;
+    for (int b = 0; b < nblk; ++b) s += part[(long long)b * 320 + t];
+    g[t] = s;
+}
+
+// --------------------------------------------- projection, loss, optimiser --
+// per-sample FP64 compensated sums of v over H*W pixels (one block per sample)
+__global__ __launch_bounds__(256) void k_sample_sum(const float* __restrict__ v, int HW, double* __restrict__ out) {
+    const float* s0 = v + (long long)blockIdx.x * HW;
+    double s = 0.0, cmp = 0.0;
+    for (int q = threadIdx.x; q < HW; q += 256) {
+        double x = s0[q], t = s + x;
+        cmp += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+        s = t;
+    }
+    __shared__ double S[256], Cc[256];
+    S[threadIdx.x] = s;
+    Cc[threadIdx.x] = cmp;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            double a = S[threadIdx.x], b = S[threadIdx.x + h], t = a + b;
+            double e = fabs(a) >= fabs(b) ? (a - t) + b : (b - t) + a;
+            S[threadIdx.x] = t;
+            Cc[threadIdx.x] += Cc[threadIdx.x + h] + e;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = S[0] + Cc[0];
+}
+
+__global__ void k_project(const float* __restrict__ c, const float* __restrict__ d, const double* __restrict__ dsum,
+                          int HW, long long n, float* __restrict__ out) {
+    long long p = blockIdx.x * 256LL + threadIdx.x;
+    if (p >= n) return;
+    const double mean = dsum[p / HW] / HW;
+    out[p] = (float)((double)c[p] + ((double)d[p] - mean));
+}
+
+// g = 2*(out - tgt)/n; e2 = (out - tgt)^2 for the loss
+__global__ void k_loss_grad(const float* __restrict__ out, const float* __restrict__ tgt, long long n,
+                            float* __restrict__ g, float* __restrict__ e2) {
+    long long p = blockIdx.x * 256LL + threadIdx.x;
+    if (p >= n) return;
+    float r = out[p] - tgt[p];
+    g[p] = 2.f * r / (float)n;
+    e2[p] = r * r;
+}
+
+// dd = g - mean_sample(g)  (adjoint of the mean projection)
+__global__ void k_unproject(const float* __restrict__ g, const double* __restrict__ gsum, int HW, long long n,
+                            float* __restrict__ dd) {
+    long long p = blockIdx.x * 256LL + threadIdx.x;
+    if (p >= n) return;
+    dd[p] = (float)((double)g[p] - gsum[p / HW] / HW);
+}
+
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       int n, float lr, float b1, float b2, float eps, float c1, float c2) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    float gg = g[t];
+    float mm = b1 * m[t] + (1.f - b1) * gg;
+    float vv = b2 * v[t] + (1.f - b2) * gg * gg;
+    m[t] = mm;
+    v[t] = vv;
+    p[t] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+}
+
+// --------------------------------------------------------------- host ----
+struct SurWS {
+    // carved from the caller's workspace
+    __bf16* act[5];   // a1..a5
+    __bf16* dz[2];    // gradient ping-pong (BF16 NHWC)
+    float* d;         // last-layer output / its gradient
+    float* g;         // dL/dout
+    float* e2;
+    double* sums;     // per-sample sums (B)
+    double* loss;     // 1
+    float* part;      // WG_BLOCKS * PART
+    __bf16* wfwd;     // 4 * MID_W
+    __bf16* wbwd;     // 4 * MID_W
+};
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+static size_t ws_layout(int B, int H, int W, SurWS* w, char* base) {
+    const size_t npix = (size_t)B * H * W;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char* p = base ? base + off : nullptr;
+        off += align256(bytes);
+        return p;
+    };
+    SurWS s;
+    for (int l = 0; l < 5; ++l) s.act[l] = (__bf16*)take(npix * CH * 2);
+    for (int l = 0; l < 2; ++l) s.dz[l] = (__bf16*)take(npix * CH * 2);
+    s.d = (float*)take(npix * 4);
+    s.g = (float*)take(npix * 4);
+    s.e2 = (float*)take(npix * 4);
+    s.sums = (double*)take((size_t)B * 8 + 8);
+    s.loss = (double*)take(16);
+    s.part = (float*)take((size_t)WG_BLOCKS * PART * 4);
+    s.wfwd = (__bf16*)take(4 * MID_W * 2);
+    s.wbwd = (__bf16*)take(4 * MID_W * 2);
+    if (w) *w = s;
+    return off;
+}
+
+static int check_shape(int B, int H, int W, const char* who) {
+    if (B < 1 || H < 16 || W < 16 || H % 16 || W % 16)
+        return fail(PHX_ERR_INVALID, std::string(who) + ": need B >= 1 and H, W positive multiples of 16");
+    if ((long long)B * H * W > (1LL << 31) / CH)
+        return fail(PHX_ERR_UNSUPPORTED, std::string(who) + ": batch too large");
+    return PHX_OK;
+}
+
+static int sur_forward(const float* P, SurWS& S, const float* c, float* out, int B, int H, int W, hipStream_t st) {
+    const long long n = (long long)B * H * W;
+    const int nb = (int)((n + 255) / 256);
+    hipLaunchKernelGGL(k_pack, dim3((4 * MID_W + 255) / 256), dim3(256), 0, st, P, S.wfwd, S.wbwd);
+    hipLaunchKernelGGL(k_conv_in_fwd, dim3(nb), dim3(256), 0, st, c, P, S.act[0], B, H, W);
+    for (int l = 0; l < 4; ++l)
+        hipLaunchKernelGGL(k_conv_mid, dim3(W / 16, H / 16, B), dim3(256), 0, st, S.act[l], S.wfwd + l * MID_W,
+                           P + P_MID + l * (MID_W + CH) + MID_W, (const __bf16*)nullptr, S.act[l + 1], H, W, 0);
+    hipLaunchKernelGGL(k_conv_out_fwd, dim3(nb), dim3(256), 0, st, S.act[4], P, S.d, B, H, W);
+    hipLaunchKernelGGL(k_sample_sum, dim3(B), dim3(256), 0, st, S.d, H * W, S.sums);
+    hipLaunchKernelGGL(k_project, dim3(nb), dim3(256), 0, st, c, S.d, S.sums, H * W, n, out);
+    PHX_LAUNCH_CHECK("phx_sur_forward");
+    return PHX_OK;
+}
+
+}  // namespace phx
+
+using namespace phx;
+
+extern "C" {
+
+int phx_sur_param_count(void) { return P_COUNT; }
+
+size_t phx_sur_workspace_bytes(int B, int H, int W) {
+    if (check_shape(B, H, W, "phx_sur_workspace_bytes")) return 0;
+    return ws_layout(B, H, W, nullptr, nullptr);
+}
+
+int phx_sur_forward(const float* params, void* ws, const float* c_in, float* c_out, int B, int H, int W,
+                    phx_stream s) {
+    int rc = check_shape(B, H, W, "phx_sur_forward");
+    if (rc) return rc;
+    if (!params || !ws || !c_in || !c_out) return fail(PHX_ERR_INVALID, "phx_sur_forward: NULL argument");
+    SurWS S;
+    ws_layout(B, H, W, &S, (char*)ws);
+    return sur_forward(params, S, c_in, c_out, B, H, W, as_stream(s));
+}
+
+int phx_sur_backward(const float* params, void* ws, const float* c_in, const float* c_out, const float* c_target,
+                     float* grad, double* loss_dev, int B, int H, int W, phx_stream s) {
+    int rc = check_shape(B, H, W, "phx_sur_backward");
+    if (rc) return rc;
+    if (!params || !ws || !c_in || !c_out || !c_target || !grad)
+        return fail(PHX_ERR_INVALID, "phx_sur_backward: NULL argument");
+    SurWS S;
+    ws_layout(B, H, W, &S, (char*)ws);
+    hipStream_t st = as_stream(s);
+    const long long n = (long long)B * H * W;
+    const int nb = (int)((n + 255) / 256);
+    hipLaunchKernelGGL(k_loss_grad, dim3(nb), dim3(256), 0, st, c_out, c_target, n, S.g, S.e2);
+    if (loss_dev) {
+        // loss = mean of e2 = (sum over samples of per-sample sums) / n
+        hipLaunchKernelGGL(k_sample_sum, dim3(1), dim3(256), 0, st, S.e2, (int)n, S.loss);
+    }
+    hipLaunchKernelGGL(k_sample_sum, dim3(B), dim3(256), 0, st, S.g, H * W, S.sums);
+    hipLaunchKernelGGL(k_unproject, dim3(nb), dim3(256), 0, st, S.g, S.sums, H * W, n, S.d);
+    // last layer: weight grads, then dz5
+    hipLaunchKernelGGL(k_edge_wgrad, dim3(WG_BLOCKS), dim3(320), 0, st, (const float*)nullptr, S.act[4], S.d, S.part,
+                       B, H, W, 1);
+    hipLaunchKernelGGL(k_reduce_edge, dim3(2), dim3(256), 0, st, S.part, WG_BLOCKS, 289, grad + P_W6);
+    hipLaunchKernelGGL(k_conv_out_bwd, dim3(nb), dim3(256), 0, st, S.d, S.act[4], params, S.dz[0], B, H, W);
+    int cur = 0;
+    for (int l = 3; l >= 0; --l) {
+        // layer l maps act[l] -> act[l+1]; dz[cur] is the gradient at its pre-activation
+        hipLaunchKernelGGL(k_conv_mid_wgrad, dim3(WG_BLOCKS), dim3(256), 0, st, S.act[l], S.dz[cur], S.part, B, H, W);
+        hipLaunchKernelGGL(k_reduce_mid, dim3((PART + 255) / 256), dim3(256), 0, st, S.part, WG_BLOCKS,
+                           grad + P_MID + l * (MID_W + CH));
+        hipLaunchKernelGGL(k_conv_mid, dim3(W / 16, H / 16, B), dim3(256), 0, st, S.dz[cur], S.wbwd + l * MID_W,
+                           (const float*)nullptr, S.act[l], S.dz[cur ^ 1], H, W, 1);
+        cur ^= 1;
+    }
+    // first layer: weight grads from x and dz1
+    hipLaunchKernelGGL(k_edge_wgrad, dim3(WG_BLOCKS), dim3(320), 0, st, c_in, S.dz[cur], (const float*)nullptr,
+                       S.part, B, H, W, 0);
+    hipLaunchKernelGGL(k_reduce_edge, dim3(2), dim3(256), 0, st, S.part, WG_BLOCKS, 320, grad + P_W1);
+    if (loss_dev) hipMemcpyAsync(loss_dev, S.loss, sizeof(double), hipMemcpyDeviceToDevice, st);
+    PHX_LAUNCH_CHECK("phx_sur_backward");
+    return PHX_OK;
+}
+
+int phx_sur_adam(float* params, const float* grad, float* m, float* v, int step, double lr, double b1, double b2,
+                 double eps, phx_stream s) {
+    if (!params || !grad || !m || !v || step < 1) return fail(PHX_ERR_INVALID, "phx_sur_adam: bad arguments");
+    const float c1 = (float)(1.0 - std::pow(b1, step)), c2 = (float)(1.0 - std::pow(b2, step));
+    hipLaunchKernelGGL(k_adam, dim3((P_COUNT + 255) / 256), dim3(256), 0, as_stream(s), params, grad, m, v, P_COUNT,
+                       (float)lr, (float)b1, (float)b2, (float)eps, c1, c2);
+    PHX_LAUNCH_CHECK("phx_sur_adam");
+    return PHX_OK;
+}
+
+// Single 32->32 layer entry points (what the network runs), exposed for
+// parity tests. x, y, dz: BF16 NHWC [B][H][W][32]; w: FP32 [32][32][3][3].
+int phx_conv3x3_bf16_fwd(const void* x, const float* w, const float* bias, void* y, void* wpk_ws, int B, int H, int W,
+                         phx_stream s) {
+    int rc = check_shape(B, H, W, "phx_conv3x3_bf16_fwd");
+    if (rc) return rc;
+    if (!x || !w || !bias || !y || !wpk_ws) return fail(PHX_ERR_INVALID, "phx_conv3x3_bf16_fwd: NULL argument");
+    hipStream_t st = as_stream(s);
+    // pack a one-layer parameter block: reuse k_pack by laying w at layer 0
+    float* Pl = (float*)wpk_ws;  // [P_COUNT] floats, then packed weights
+    __bf16* wf = (__bf16*)(Pl + P_COUNT);
+    __bf16* wb = wf + 4 * MID_W;
+    PHX_HIP(hipMemcpyAsync(Pl + P_MID, w, MID_W * sizeof(float), hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_pack, dim3((4 * MID_W + 255) / 256), dim3(256), 0, st, Pl, wf, wb);
+    hipLaunchKernelGGL(k_conv_mid, dim3(W / 16, H / 16, B), dim3(256), 0, st, (const __bf16*)x, wf, bias,
+                       (const __bf16*)nullptr, (__bf16*)y, H, W, 0);
+    PHX_LAUNCH_CHECK("phx_conv3x3_bf16_fwd");
+    return PHX_OK;
+}
+
+size_t phx_conv3x3_ws_bytes(void) { return (size_t)P_COUNT * 4 + 8 * MID_W * 2 + (size_t)WG_BLOCKS * PART * 4; }
+
+int phx_conv3x3_bf16_dgrad(const void* dy, const float* w, const void* mask, void* dx, void* wpk_ws, int B, int H,
+                           int W, phx_stream s) {
+    int rc = check_shape(B, H, W, "phx_conv3x3_bf16_dgrad");
+    if (rc) return rc;
+    if (!dy || !w || !mask || !dx || !wpk_ws) return fail(PHX_ERR_INVALID, "phx_conv3x3_bf16_dgrad: NULL argument");
+    hipStream_t st = as_stream(s);
+    float* Pl = (float*)wpk_ws;
+    __bf16* wf = (__bf16*)(Pl + P_COUNT);
+    __bf16* wb = wf + 4 * MID_W;
+    PHX_HIP(hipMemcpyAsync(Pl + P_MID, w, MID_W * sizeof(float), hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_pack, dim3((4 * MID_W + 255) / 256), dim3(256), 0, st, Pl, wf, wb);
+    hipLaunchKernelGGL(k_conv_mid, dim3(W / 16, H / 16, B), dim3(256), 0, st, (const __bf16*)dy, wb,
+                       (const float*)nullptr, (const __bf16*)mask, (__bf16*)dx, H, W, 1);
+    PHX_LAUNCH_CHECK("phx_conv3x3_bf16_dgrad");
+    return PHX_OK;
+}
+
+int phx_conv3x3_bf16_wgrad(const void* x, const void* dz, float* dw, float* db, void* wpk_ws, int B, int H, int W,
+                           phx_stream s) {
+    int rc = check_shape(B, H, W, "phx_conv3x3_bf16_wgrad");
+    if (rc) return rc;
+    if (!x || !dz || !dw || !db || !wpk_ws) return fail(PHX_ERR_INVALID, "phx_conv3x3_bf16_wgrad: NULL argument");
+    hipStream_t st = as_stream(s);
+    float* part = (float*)((char*)wpk_ws + (size_t)P_COUNT * 4 + 8 * MID_W * 2);
+    float* g = (float*)wpk_ws;   // reuse the parameter area for the reduced result
+    hipLaunchKernelGGL(k_conv_mid_wgrad, dim3(WG_BLOCKS), dim3(256), 0, st, (const __bf16*)x, (const __bf16*)dz, part,
+                       B, H, W);
+    hipLaunchKernelGGL(k_reduce_mid, dim3((PART + 255) / 256), dim3(256), 0, st, part, WG_BLOCKS, g);
+    PHX_HIP(hipMemcpyAsync(dw, g, MID_W * sizeof(float), hipMemcpyDeviceToDevice, st));
+    PHX_HIP(hipMemcpyAsync(db, g + MID_W, CH * sizeof(float), hipMemcpyDeviceToDevice, st));
+    PHX_LAUNCH_CHECK("phx_conv3x3_bf16_wgrad");
+    return PHX_OK;
+}
+
+}  // extern "C"

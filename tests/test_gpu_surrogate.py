@@ -1,0 +1,217 @@
+"""GPU parity of the surrogate's kernels against the FP64 CPU conv oracle
+(oracle/conv_oracle.c), and a whole-network gradient check.
+
+BF16 inputs are drawn on a BF16-exact grid, so a kernel's only rounding is its
+FP32 accumulation and the final BF16 store; tolerances say which."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from phix_amd import capi
+from oracle import oracle as O
+
+from helpers import stream
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16_grid(rng, shape, scale=1.0):
+    # multiples of 1/32 in [-1, 1): exactly representable in BF16
+    return (rng.integers(-32, 32, size=shape) / 32.0 * scale).astype(np.float32)
+
+
+def to_bf16(t):
+    import torch
+    return t.to(torch.bfloat16).contiguous()
+
+
+def nchw_to_nhwc(a):
+    return np.ascontiguousarray(np.transpose(a, (0, 2, 3, 1)))
+
+
+def nhwc_to_nchw(a):
+    return np.ascontiguousarray(np.transpose(a, (0, 3, 1, 2)))
+
+
+def bf16_round(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(torch.bfloat16).float().numpy()
+
+
+def test_mfma_conv_forward(dev):
+    import torch
+    rng = np.random.default_rng(0)
+    B, H, W = 2, 32, 48
+    x = bf16_grid(rng, (B, 32, H, W))
+    w = bf16_grid(rng, (32, 32, 3, 3), 0.25)
+    b = bf16_grid(rng, (32,), 0.5)
+    tx = to_bf16(torch.from_numpy(nchw_to_nhwc(x)).to(dev))
+    ty = torch.empty_like(tx)
+    ws = torch.empty(capi.lib().phx_conv3x3_ws_bytes(), dtype=torch.uint8, device=dev)
+    tw = torch.from_numpy(w).to(dev)
+    tb = torch.from_numpy(b).to(dev)
+    capi.call("phx_conv3x3_bf16_fwd", capi.ptr(tx), capi.ptr(tw), capi.ptr(tb), capi.ptr(ty), capi.ptr(ws), B, H, W,
+              stream())
+    got = nhwc_to_nchw(ty.float().cpu().numpy())
+    ref = np.maximum(O.conv3x3_fwd(x.astype(np.float64), w.astype(np.float64), b.astype(np.float64)), 0.0)
+    # one BF16 rounding of the output (rel 2^-8) on top of FP32 accumulation
+    err = np.abs(got - ref)
+    assert np.all(err <= 2 ** -8 * np.abs(ref) + 1e-5), err.max()
+
+
+def test_mfma_conv_dgrad(dev):
+    import torch
+    rng = np.random.default_rng(1)
+    B, H, W = 2, 32, 32
+    dy = bf16_grid(rng, (B, 32, H, W))
+    w = bf16_grid(rng, (32, 32, 3, 3), 0.25)
+    mask = bf16_grid(rng, (B, 32, H, W))
+    ws = torch.empty(capi.lib().phx_conv3x3_ws_bytes(), dtype=torch.uint8, device=dev)
+    tdy = to_bf16(torch.from_numpy(nchw_to_nhwc(dy)).to(dev))
+    tm = to_bf16(torch.from_numpy(nchw_to_nhwc(mask)).to(dev))
+    tdx = torch.empty_like(tdy)
+    capi.call("phx_conv3x3_bf16_dgrad", capi.ptr(tdy), capi.ptr(torch.from_numpy(w).to(dev)), capi.ptr(tm),
+              capi.ptr(tdx), capi.ptr(ws), B, H, W, stream())
+    got = nhwc_to_nchw(tdx.float().cpu().numpy())
+    ref = O.conv3x3_dgrad(dy.astype(np.float64), w.astype(np.float64)) * (mask > 0)
+    err = np.abs(got - ref)
+    assert np.all(err <= 2 ** -8 * np.abs(ref) + 1e-5), err.max()
+
+
+def test_mfma_conv_wgrad(dev):
+    import torch
+    rng = np.random.default_rng(2)
+    B, H, W = 3, 48, 32
+    x = bf16_grid(rng, (B, 32, H, W))
+    dz = bf16_grid(rng, (B, 32, H, W))
+    ws = torch.empty(capi.lib().phx_conv3x3_ws_bytes(), dtype=torch.uint8, device=dev)
+    tx = to_bf16(torch.from_numpy(nchw_to_nhwc(x)).to(dev))
+    tz = to_bf16(torch.from_numpy(nchw_to_nhwc(dz)).to(dev))
+    dw = torch.empty((32, 32, 3, 3), dtype=torch.float32, device=dev)
+    db = torch.empty(32, dtype=torch.float32, device=dev)
+    capi.call("phx_conv3x3_bf16_wgrad", capi.ptr(tx), capi.ptr(tz), capi.ptr(dw), capi.ptr(db), capi.ptr(ws), B, H, W,
+              stream())
+    rdw, rdb = O.conv3x3_wgrad(x.astype(np.float64), dz.astype(np.float64))
+    gdw = dw.cpu().numpy()
+    # exact BF16 products; FP32 accumulation over 4608 pixels
+    scale = np.abs(x).mean() * np.abs(dz).mean() * B * H * W
+    assert np.abs(gdw - rdw).max() <= 1e-5 * scale + 1e-6 * np.abs(rdw).max()
+    assert np.abs(db.cpu().numpy() - rdb).max() <= 1e-5 * scale
+
+
+# ------------------------------------------------------------ whole network
+def unpack(p):
+    """FP32 master vector -> list of (w, b) per layer (oracle layout)."""
+    out, o = [], 0
+    shapes = [(32, 1), (32, 32), (32, 32), (32, 32), (32, 32), (1, 32)]
+    for co, ci in shapes:
+        n = co * ci * 9
+        w = p[o:o + n].reshape(co, ci, 3, 3)
+        o += n
+        b = p[o:o + co]
+        o += co
+        out.append((w, b))
+    assert o == len(p)
+    return out
+
+
+def ref_forward_backward(p, c, tgt):
+    """FP64 network with the GPU's BF16 roundings (weights of the 32->32 layers,
+    every activation and every activation gradient), built on the conv oracle.
+    Returns (out, loss, grad vector)."""
+    L = unpack(p.astype(np.float64))
+    B, H, W = c.shape
+    x = (c - 0.5)[:, None].astype(np.float64)
+    acts, pre = [x], []
+    for i, (w, b) in enumerate(L):
+        wq = bf16_round(w).astype(np.float64) if 1 <= i <= 4 else w
+        z = O.conv3x3_fwd(acts[-1], wq, b)
+        pre.append(z)
+        if i < 5:
+            acts.append(bf16_round(np.maximum(z, 0.0)).astype(np.float64))
+    d = pre[-1][:, 0]
+    out = c + d - d.mean(axis=(1, 2), keepdims=True)
+    n = c.size
+    g = 2 * (out - tgt) / n
+    dd = (g - g.mean(axis=(1, 2), keepdims=True))[:, None]
+    grads = [None] * 6
+    dz = dd
+    for i in range(5, -1, -1):
+        w, b = L[i]
+        wq = bf16_round(w).astype(np.float64) if 1 <= i <= 4 else w
+        gw, gb = O.conv3x3_wgrad(acts[i], dz)
+        grads[i] = (gw, gb)
+        if i > 0:
+            da = O.conv3x3_dgrad(dz, wq) * (acts[i] > 0)
+            dz = bf16_round(da).astype(np.float64)
+    vec = np.concatenate([np.concatenate([gw.ravel(), gb.ravel()]) for gw, gb in grads])
+    return out, float(((out - tgt) ** 2).mean()), vec
+
+
+def test_network_forward_backward_matches_fp64(dev):
+    import torch
+    rng = np.random.default_rng(3)
+    B, H, W = 2, 32, 32
+    npar = capi.lib().phx_sur_param_count()
+    assert npar == 32 * 9 + 32 + 4 * (9216 + 32) + 32 * 9 + 1
+    p = (rng.standard_normal(npar) * 0.1).astype(np.float32)
+    c = (0.5 + 0.2 * rng.standard_normal((B, H, W))).astype(np.float32)
+    tgt = (c + 0.05 * rng.standard_normal((B, H, W))).astype(np.float32)
+    tp = torch.from_numpy(p).to(dev)
+    tc = torch.from_numpy(c).to(dev)
+    tt = torch.from_numpy(tgt).to(dev)
+    out = torch.empty_like(tc)
+    ws = torch.empty(capi.lib().phx_sur_workspace_bytes(B, H, W), dtype=torch.uint8, device=dev)
+    grad = torch.zeros(npar, dtype=torch.float32, device=dev)
+    loss = torch.zeros(1, dtype=torch.float64, device=dev)
+    capi.call("phx_sur_forward", capi.ptr(tp), capi.ptr(ws), capi.ptr(tc), capi.ptr(out), B, H, W, stream())
+    capi.call("phx_sur_backward", capi.ptr(tp), capi.ptr(ws), capi.ptr(tc), capi.ptr(out), capi.ptr(tt),
+              capi.ptr(grad), capi.ptr(loss), B, H, W, stream())
+    torch.cuda.synchronize()
+    rout, rloss, rgrad = ref_forward_backward(p, c.astype(np.float64), tgt.astype(np.float64))
+    gout = out.cpu().numpy()
+    # mean projection is exact per sample
+    np.testing.assert_allclose(gout.mean(axis=(1, 2)), c.mean(axis=(1, 2)), atol=2e-7)
+    assert np.abs(gout - rout).max() <= 1e-3 * np.abs(rout - c).max() + 1e-5
+    assert abs(loss.item() / c.size - rloss) <= 1e-3 * rloss
+    g = grad.cpu().numpy()
+    rel = np.linalg.norm(g - rgrad) / np.linalg.norm(rgrad)
+    print(f"network gradient rel-L2 vs FP64 (BF16-rounded) reference: {rel:.3e}")
+    assert rel <= 2e-2
+    # per-layer: every layer's gradient direction agrees
+    o = 0
+    for co, ci in [(32, 1), (32, 32), (32, 32), (32, 32), (32, 32), (1, 32)]:
+        n = co * ci * 9 + co
+        a, r = g[o:o + n], rgrad[o:o + n]
+        assert np.dot(a, r) / (np.linalg.norm(a) * np.linalg.norm(r)) > 0.99
+        o += n
+
+
+def test_adam_and_training_reduces_loss(dev):
+    import torch
+    rng = np.random.default_rng(4)
+    B, H, W = 4, 32, 32
+    npar = capi.lib().phx_sur_param_count()
+    p = (rng.standard_normal(npar) * 0.05).astype(np.float32)
+    tp = torch.from_numpy(p).to(dev)
+    m = torch.zeros_like(tp)
+    v = torch.zeros_like(tp)
+    c = torch.from_numpy((0.5 + 0.2 * rng.standard_normal((B, H, W))).astype(np.float32)).to(dev)
+    # target: a fixed smoothing of c (learnable by a conv net)
+    tgt = c + 0.1 * (torch.roll(c, 1, 1) + torch.roll(c, -1, 1) + torch.roll(c, 1, 2) + torch.roll(c, -1, 2) - 4 * c)
+    out = torch.empty_like(c)
+    ws = torch.empty(capi.lib().phx_sur_workspace_bytes(B, H, W), dtype=torch.uint8, device=dev)
+    grad = torch.zeros_like(tp)
+    loss = torch.zeros(1, dtype=torch.float64, device=dev)
+    losses = []
+    for step in range(1, 201):
+        capi.call("phx_sur_forward", capi.ptr(tp), capi.ptr(ws), capi.ptr(c), capi.ptr(out), B, H, W, stream())
+        capi.call("phx_sur_backward", capi.ptr(tp), capi.ptr(ws), capi.ptr(c), capi.ptr(out), capi.ptr(tgt),
+                  capi.ptr(grad), capi.ptr(loss), B, H, W, stream())
+        capi.call("phx_sur_adam", capi.ptr(tp), capi.ptr(grad), capi.ptr(m), capi.ptr(v), step, 1e-3, 0.9, 0.999,
+                  1e-8, stream())
+        if step % 50 == 1 or step == 200:
+            losses.append(loss.item() / c.numel())
+    print("training losses:", losses)
+    assert losses[-1] < 0.2 * losses[0]
