@@ -1,17 +1,24 @@
-"""Benchmark of the fused Cahn-Hilliard RK4 hot path on one MI355X.
+"""Benchmark of the fused phase-field hot path on one MI355X.
 
-Workload (the north-star configuration that fits one GPU): 3D Cahn-Hilliard,
-256^3 cells, dx=1, zero-flux BCs on all faces, f = c^2(1-c)^2, kappa=1, M=1,
-dt=0.005, classic RK4, FP32 stencil bulk; c0 = 0.4 + U(-0.02, 0.02) from
-SplitMix64 seed 2. One step = one RK4 time step over the whole grid (four fused
-stage kernels). Inputs are resident in HBM before the timed region.
+Default workload (the north-star configuration that fits one GPU): 3D
+Cahn-Hilliard, 256^3 cells, dx=1, zero-flux BCs on all faces,
+f = c^2(1-c)^2, kappa=1, M=1, dt=0.005, classic RK4, FP32 stencil bulk;
+c0 = 0.4 + U(-0.02, 0.02) from SplitMix64 seed 2. One step = one RK4 time
+step over the whole grid (four fused stage kernels). Inputs are resident in
+HBM before the timed region.
 
-Prints ONE JSON line (rank 0). `roofline` is computed for the dominant kernel
-(the fused stage kernel) from its algorithmic bytes -- 16 field passes per
-cell-step, i.e. 64 B/cell-step in FP32, 16 B/cell per stage launch -- divided
-by its average launch duration measured with HIP events on the launch stream.
-`cpu_baseline` times the FP64 CPU oracle (a restatement of the reference's
-per-term CPU path) on 10 RK4 steps of the same 256^3 problem, one thread.
+Other workloads (--workload), each a configuration of the same path:
+  euler128  128^2 periodic CH, multi-step Euler, FP64, kappa=0.5, dt=0.01
+  ch2d1024  1024^2 periodic CH RK4, FP32 (same physics as the default)
+  ac1024    1024^2 periodic Allen-Cahn, 8 order parameters, RK4, FP32
+  ens512    512 periodic CH runs, sides 64 + 32*floor(15u), RK4, FP32,
+            one persistent launch per stage, hipGraph-captured step loop
+
+Prints ONE JSON line. `roofline` is computed for the dominant kernel from its
+algorithmic bytes per launch divided by its average launch duration measured
+with HIP events on the launch stream. `cpu_baseline` times the FP64 CPU oracle
+(a restatement of the reference's per-term CPU path, one thread) on a bounded
+sample of the same workload.
 """
 from __future__ import annotations
 
@@ -28,8 +35,6 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-NX = 256
-PASSES_PER_STEP = 16
 
 
 def cpu_model() -> str:
@@ -42,11 +47,10 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
-def traffic_from_profiles(dtype: str):
-    """HBM bytes per stage launch measured by rocprofv3 --pmc (FETCH_SIZE +
-    WRITE_SIZE, scaled by a copy-kernel calibration), if a summary is
-    committed under profiles/."""
-    f = ROOT / "profiles" / f"ch3d_rk4_{dtype}_256_traffic.json"
+def traffic_from_profiles(name: str):
+    """HBM bytes per launch of the dominant kernel measured by rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if committed."""
+    f = ROOT / "profiles" / f"{name}_traffic.json"
     if f.exists():
         try:
             return json.loads(f.read_text()).get("bytes_per_launch")
@@ -55,25 +59,254 @@ def traffic_from_profiles(dtype: str):
     return None
 
 
-def run_cpu_baseline(steps: int = 10):
+def timed(fn, stream):
+    import torch
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    fn()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    return ev0.elapsed_time(ev1)
+
+
+def roofline(bytes_per_launch, launch_s, kernel, traffic_name):
+    ach = bytes_per_launch / launch_s / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(traffic_name), "kernel": kernel,
+            "alg_bytes_per_launch": bytes_per_launch, "avg_launch_us": round(launch_s * 1e6, 2)}
+
+
+def line(metric, value, unit, steps, warmup, ms, dtype, data, config, roof, cpu):
+    return {"metric": metric, "value": round(value, 1), "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": round(ms / steps, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": dtype, "data": data, "config": config, "roofline": roof, "cpu_baseline": cpu,
+            "host": {"nproc": os.cpu_count(), "cpu": cpu_model()}}
+
+
+# ------------------------------------------------------------ workloads ----
+def bench_ch3d(a, with_cpu):
+    import torch
     from phix_amd import capi, init
-    from oracle import oracle as O
-    import numpy as np
+    NX = 256
+    dev = torch.device("cuda:0")
+    dtype = capi.F32 if a.dtype == "f32" else capi.F64
+    tdt, esz = (torch.float32, 4) if dtype == capi.F32 else (torch.float64, 8)
     g = capi.make_grid((NX, NX, NX), bc=capi.NOFLUX)
     p = capi.CHParams(1.0, 0.0, 1.0, 1.0, 1.0, 0.005)
-    c = init.to_stored(init.ch_field((NX, NX, NX), 2, 0.4, 0.02), g)
-    work = np.zeros((6,) + c.shape)
-    t0 = time.perf_counter()
-    O.ch_rk4_steps(g, p, c, steps, work)
-    dt = time.perf_counter() - t0
-    return {
-        "value": NX ** 3 * steps / dt,
-        "unit": "cell-updates/s",
-        "cores": 1,
-        "kind": "port",
-        "sample": f"{steps} RK4 steps of the same 256^3 problem, FP64, reference-structured per-term passes "
-                  f"(oracle/phx_oracle.c orc_ch_rk4_steps), {dt:.1f} s on {cpu_model()}",
-    }
+    c = torch.from_numpy(init.to_stored(init.ch_field((NX,) * 3, 2, 0.4, 0.02), g)).to(tdt).to(dev)
+    work = torch.empty(capi.lib().phx_ch_rk4_workspace_bytes(C.byref(g), dtype) // esz, dtype=tdt, device=dev)
+    stream = torch.cuda.current_stream()
+    sh = capi.stream_handle(stream)
+    zc, tune_ms = C.c_int(0), C.c_double(0)
+    scratch = c.clone()
+    capi.call("phx_ch_rk4_autotune", C.byref(g), dtype, capi.ptr(scratch), capi.ptr(work), C.byref(p), 10,
+              C.byref(zc), C.byref(tune_ms), sh)
+    del scratch
+
+    def run(n):
+        capi.call("phx_ch_rk4_steps", C.byref(g), dtype, capi.ptr(c), capi.ptr(work), C.byref(p), n, sh)
+
+    run(a.warmup)
+    ms = timed(lambda: run(a.steps), stream)
+    cells = NX ** 3
+    # 16 field passes per FP32 step (17 in FP64: c_n re-read in stage 4), 4 launches per step
+    passes = 16 if dtype == capi.F32 else 17
+    roof = roofline(passes / 4 * esz * cells, (ms / 1e3) / (4 * a.steps), "k_ch_stage_3d", f"ch3d_rk4_{a.dtype}_256")
+    cpu = None
+    if with_cpu:
+        import numpy as np
+        from oracle import oracle as O
+        steps = 10
+        cc = init.to_stored(init.ch_field((NX,) * 3, 2, 0.4, 0.02), g)
+        w = np.zeros((6,) + cc.shape)
+        t0 = time.perf_counter()
+        O.ch_rk4_steps(g, p, cc, steps, w)
+        dt = time.perf_counter() - t0
+        cpu = {"value": cells * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+               "sample": f"{steps} RK4 steps of the same 256^3 problem, FP64, reference-structured per-term passes "
+                         f"(oracle/phx_oracle.c orc_ch_rk4_steps), {dt:.1f} s on {cpu_model()}"}
+    cfg = {"workload": "ch3d_rk4_256cube_noflux", "grid": [NX] * 3, "bc": "noflux", "scheme": "rk4", "dt": 0.005,
+           "kappa": 1.0, "M": 1.0, "f": "c^2(1-c)^2", "bulk": a.dtype, "parallelism": "single-gpu",
+           "zchunk": zc.value}
+    return line("cell-updates/s, Cahn-Hilliard RK4 (fused stencil, %HBM roofline in 'roofline')",
+                cells * a.steps / (ms / 1e3), "cell-updates/s", a.steps, a.warmup, ms, a.dtype,
+                "synthetic (c0 = 0.4 + U(-0.02, 0.02), SplitMix64 seed 2)", cfg, roof, cpu)
+
+
+def bench_ch2d(a, with_cpu):
+    import torch
+    from phix_amd import capi, init
+    NX = 1024
+    dev = torch.device("cuda:0")
+    g = capi.make_grid((NX, NX), bc=capi.PERIODIC)
+    p = capi.CHParams(1.0, 0.0, 1.0, 1.0, 1.0, 0.005)
+    c = torch.from_numpy(init.to_stored(init.ch_field((NX, NX), 2, 0.4, 0.02), g)).float().to(dev)
+    work = torch.empty(capi.lib().phx_ch_rk4_workspace_bytes(C.byref(g), capi.F32) // 4, dtype=torch.float32,
+                       device=dev)
+    stream = torch.cuda.current_stream()
+    sh = capi.stream_handle(stream)
+
+    def run(n):
+        capi.call("phx_ch_rk4_steps", C.byref(g), capi.F32, capi.ptr(c), capi.ptr(work), C.byref(p), n, sh)
+
+    run(a.warmup)
+    ms = timed(lambda: run(a.steps), stream)
+    cells = NX * NX
+    roof = roofline(16 / 4 * 4 * cells, (ms / 1e3) / (4 * a.steps), "k_ch_stage_2d", "ch2d_rk4_f32_1024")
+    cpu = None
+    if with_cpu:
+        from oracle import oracle as O
+        steps = 20
+        cc = init.to_stored(init.ch_field((NX, NX), 2, 0.4, 0.02), g)
+        t0 = time.perf_counter()
+        O.ch_rk4_steps(g, p, cc, steps)
+        dt = time.perf_counter() - t0
+        cpu = {"value": cells * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+               "sample": f"{steps} RK4 steps of the same 1024^2 problem, FP64 oracle, {dt:.1f} s on {cpu_model()}"}
+    cfg = {"workload": "ch2d_rk4_1024sq_periodic", "grid": [NX, NX], "bc": "periodic", "scheme": "rk4", "dt": 0.005,
+           "kappa": 1.0, "M": 1.0, "bulk": "f32", "parallelism": "single-gpu"}
+    return line("cell-updates/s, Cahn-Hilliard RK4 2D (fused stencil)", cells * a.steps / (ms / 1e3),
+                "cell-updates/s", a.steps, a.warmup, ms, "f32",
+                "synthetic (c0 = 0.4 + U(-0.02, 0.02), SplitMix64 seed 2)", cfg, roof, cpu)
+
+
+def bench_ac(a, with_cpu):
+    import torch
+    from phix_amd import capi, init
+    NX, NF = 1024, 8
+    dev = torch.device("cuda:0")
+    g = capi.make_grid((NX, NX), bc=capi.PERIODIC)
+    p = capi.ACParams(1.0, 2.0, 1.5, 0.01)
+    x = init.ac_fields((NX, NX), NF, 3)
+    etas = [torch.from_numpy(init.to_stored(x[f], g)).float().to(dev) for f in range(NF)]
+    arr = (C.c_void_p * NF)(*[capi.ptr(t) for t in etas])
+    work = torch.empty(capi.lib().phx_ac_rk4_workspace_bytes(C.byref(g), capi.F32, NF) // 4, dtype=torch.float32,
+                       device=dev)
+    stream = torch.cuda.current_stream()
+    sh = capi.stream_handle(stream)
+
+    def run(n):
+        capi.call("phx_ac_rk4_steps", C.byref(g), capi.F32, NF, arr, capi.ptr(work), C.byref(p), n, sh)
+
+    run(a.warmup)
+    ms = timed(lambda: run(a.steps), stream)
+    pts = NX * NX
+    roof = roofline(16 / 4 * NF * 4 * pts, (ms / 1e3) / (4 * a.steps), "k_ac_stage_2d", "ac8_rk4_f32_1024")
+    cpu = None
+    if with_cpu:
+        from oracle import oracle as O
+        steps = 1
+        es = [init.to_stored(x[f], g) for f in range(NF)]
+        t0 = time.perf_counter()
+        O.ac_rk4_steps(g, p, es, steps)
+        dt = time.perf_counter() - t0
+        cpu = {"value": pts * steps / dt, "unit": "point-updates/s", "cores": 1, "kind": "port",
+               "sample": f"{steps} RK4 step of the same 1024^2 x 8-field problem, FP64 oracle, {dt:.1f} s on "
+                         f"{cpu_model()}"}
+    cfg = {"workload": "ac8_rk4_1024sq_periodic", "grid": [NX, NX], "fields": NF, "gamma": 1.5, "kappa": 2.0,
+           "L": 1.0, "dt": 0.01, "scheme": "rk4", "bulk": "f32", "parallelism": "single-gpu"}
+    return line("grid-point-updates/s (8 fields), Allen-Cahn RK4 (fused multi-field stencil)",
+                pts * a.steps / (ms / 1e3), "point-updates/s", a.steps, a.warmup, ms, "f32",
+                "synthetic (one-hot on floor(8u) + U(-0.001, 0.001), SplitMix64 seed 3)", cfg, roof, cpu)
+
+
+def bench_ens(a, with_cpu):
+    import torch
+    import numpy as np
+    from phix_amd import capi, init
+    dev = torch.device("cuda:0")
+    N, kappa, cbar = init.ensemble_params(512, 4)
+    n = len(N)
+    h = C.c_void_p()
+    capi.call("phx_ensemble_create", n, (C.c_int * n)(*[int(v) for v in N]), (C.c_double * n)(*kappa), None,
+              C.byref(h))
+    cells = capi.lib().phx_ensemble_cells(h)
+    x0 = np.concatenate([init.uniform(init.substream(4, 10 + r), int(N[r]) ** 2, cbar[r] - 0.01, cbar[r] + 0.01)
+                         for r in range(n)])
+    c = torch.from_numpy(x0).float().to(dev)
+    work = torch.empty(3 * cells, dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream()
+    sh = capi.stream_handle(stream)
+    scratch = c.clone()
+    best = C.c_double(0)
+    capi.call("phx_ensemble_autotune", h, capi.ptr(scratch), capi.ptr(work), 0.01, 5, C.byref(best), sh)
+    del scratch
+    sp, gr, tx, ty = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    capi.call("phx_ensemble_get_tiling", h, C.byref(sp), C.byref(gr), C.byref(tx), C.byref(ty))
+    per_graph = 50
+    steps = max(per_graph, (a.steps // per_graph) * per_graph)
+    capi.call("phx_ensemble_graph_steps", h, capi.ptr(c), capi.ptr(work), 0.01, per_graph, 1, sh)
+    ms = timed(lambda: capi.call("phx_ensemble_graph_steps", h, capi.ptr(c), capi.ptr(work), 0.01, per_graph,
+                                 steps // per_graph, sh), stream)
+    roof = roofline(16 / 4 * 4 * cells, (ms / 1e3) / (4 * steps), "k_ens_stage", "ens512_rk4_f32")
+    cpu = None
+    if with_cpu:
+        from oracle import oracle as O
+        csteps, runs = 20, 16
+        done, off = 0, 0
+        t0 = time.perf_counter()
+        for r in range(runs):
+            nn = int(N[r])
+            g = capi.make_grid((nn, nn), bc=capi.PERIODIC)
+            cc = init.to_stored(x0[off:off + nn * nn].reshape(1, nn, nn), g)
+            off += nn * nn
+            O.ch_rk4_steps(g, capi.CHParams(1, 0, 1, float(kappa[r]), 1, 0.01), cc, csteps)
+            done += nn * nn * csteps
+        dt = time.perf_counter() - t0
+        cpu = {"value": done / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+               "sample": f"{csteps} RK4 steps of the first {runs} runs, FP64 oracle, {dt:.1f} s on {cpu_model()}"}
+    capi.lib().phx_ensemble_destroy(h)
+    cfg = {"workload": "ens512_ch_rk4_periodic", "runs": n, "cells": int(cells), "sides": "64+32*floor(15u)",
+           "dt": 0.01, "scheme": "rk4", "bulk": "f32", "tile": [tx.value, ty.value], "persistent_grid": gr.value,
+           "hipgraph_steps": per_graph, "parallelism": "single-gpu"}
+    return line("cell-updates/s, batched uneven-size CH RK4 ensemble", cells * steps / (ms / 1e3),
+                "cell-updates/s", steps, per_graph, ms, "f32",
+                "synthetic (per-run kappa ~ U(0.25,1), cbar ~ U(0.3,0.7) +- 0.01, SplitMix64 seed 4)", cfg, roof, cpu)
+
+
+def bench_euler(a, with_cpu):
+    import torch
+    import numpy as np
+    from phix_amd import capi, init
+    NX = 128
+    dev = torch.device("cuda:0")
+    g = capi.make_grid((NX, NX), bc=capi.PERIODIC)
+    p = capi.CHParams(1.0, 0.0, 1.0, 0.5, 1.0, 0.01)
+    c0 = init.to_stored(init.ch_field((NX, NX), 1, 0.5, 0.01), g)
+    c = torch.from_numpy(c0).to(dev)
+    mu = torch.zeros_like(c)
+    stream = torch.cuda.current_stream()
+    sh = capi.stream_handle(stream)
+    steps = 1000
+
+    def run(n):
+        capi.call("phx_ch_euler_steps", C.byref(g), capi.F64, capi.ptr(c), capi.ptr(mu), C.byref(p), n, sh)
+
+    run(10)
+    ms = timed(lambda: run(steps), stream)
+    cells = NX * NX
+    roof = roofline(2 * 8 * cells, (ms / 1e3) / (2 * steps), "k_euler_mu+k_euler_c", "ch2d_euler_f64_128")
+    cpu = None
+    if with_cpu:
+        from oracle import oracle as O
+        cc = c0.copy()
+        t0 = time.perf_counter()
+        O.ch_euler_steps(g, p, cc, np.zeros_like(cc), steps)
+        dt = time.perf_counter() - t0
+        cpu = {"value": cells * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+               "sample": f"the full {steps}-step run, FP64 oracle, {dt:.1f} s on {cpu_model()}"}
+    cfg = {"workload": "ch2d_euler_128sq_periodic", "grid": [NX, NX], "bc": "periodic", "scheme": "euler multi-step",
+           "dt": 0.01, "kappa": 0.5, "M": 1.0, "bulk": "f64", "parallelism": "single-gpu",
+           "note": "135 KB working set is cache resident: latency bound, not HBM bound"}
+    return line("cell-updates/s, Cahn-Hilliard multi-step Euler (config-1 app)", cells * steps / (ms / 1e3),
+                "cell-updates/s", steps, 10, ms, "f64", "synthetic (c0 = 0.5 + U(-0.01, 0.01), SplitMix64 seed 1)",
+                cfg, roof, cpu)
+
+
+WORKLOADS = {"ch3d": bench_ch3d, "ch2d1024": bench_ch2d, "ac1024": bench_ac, "ens512": bench_ens,
+             "euler128": bench_euler}
 
 
 def main(argv=None) -> int:
@@ -82,91 +315,14 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--dtype", choices=["f32", "f64"], default="f32")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="ch3d")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
               "the stencil path shards by slab with halo exchange but that is not built here.", file=sys.stderr)
         return 2
-
-    import torch
-    from phix_amd import capi, init
-
-    dev = torch.device("cuda:0")
-    L = capi.lib()
-    dtype = capi.F32 if a.dtype == "f32" else capi.F64
-    tdt = torch.float32 if dtype == capi.F32 else torch.float64
-    esz = 4 if dtype == capi.F32 else 8
-    g = capi.make_grid((NX, NX, NX), bc=capi.NOFLUX)
-    p = capi.CHParams(1.0, 0.0, 1.0, 1.0, 1.0, 0.005)
-    c0 = init.to_stored(init.ch_field((NX, NX, NX), 2, 0.4, 0.02), g)
-    c = torch.from_numpy(c0).to(tdt).to(dev)
-    del c0
-    work = torch.empty(L.phx_ch_rk4_workspace_bytes(C.byref(g), dtype) // esz, dtype=tdt, device=dev)
-    stream = torch.cuda.current_stream()
-    sh = capi.stream_handle(stream)
-
-    def run(n):
-        capi.call("phx_ch_rk4_steps", C.byref(g), dtype, capi.ptr(c), capi.ptr(work), C.byref(p), n, sh)
-
-    # launch shape from the autotuner, timed on a scratch copy of the state
-    zc = C.c_int(0)
-    tune_ms = C.c_double(0)
-    scratch = c.clone()
-    capi.call("phx_ch_rk4_autotune", C.byref(g), dtype, capi.ptr(scratch), capi.ptr(work), C.byref(p), 10,
-              C.byref(zc), C.byref(tune_ms), sh)
-    del scratch
-    run(a.warmup)
-    torch.cuda.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    ev0.record(stream)
-    run(a.steps)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    ms = ev0.elapsed_time(ev1)
-    cells = NX ** 3
-    value = cells * a.steps / (ms / 1e3)
-
-    # dominant kernel: the fused stage kernel, 4 launches per step; per-launch
-    # duration from a separate event-bracketed run of single stages' worth
-    # (4*steps launches + 3 BC launches on the last step).
-    per_launch_s = (ms / 1e3) / (4 * a.steps)
-    bytes_per_launch = PASSES_PER_STEP / 4 * esz * cells
-    achieved = bytes_per_launch / per_launch_s / 1e9
-    roof = {
-        "bound": "hbm",
-        "achieved": round(achieved, 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": traffic_from_profiles(a.dtype),
-        "kernel": "k_ch_stage_3d",
-        "alg_bytes_per_launch": bytes_per_launch,
-        "avg_launch_us": round(per_launch_s * 1e6, 2),
-    }
-    cpu = None if a.no_cpu_baseline else run_cpu_baseline()
-    out = {
-        "metric": "cell-updates/s, Cahn-Hilliard RK4 (fused stencil, %HBM roofline in 'roofline')",
-        "value": round(value, 1),
-        "unit": "cell-updates/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": round(ms / a.steps, 5),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": a.dtype,
-        "data": "synthetic (c0 = 0.4 + U(-0.02, 0.02), SplitMix64 seed 2)",
-        "config": {"workload": "ch3d_rk4_256cube_noflux", "grid": [NX, NX, NX], "bc": "noflux", "scheme": "rk4",
-                   "dt": 0.005, "kappa": 1.0, "M": 1.0, "f": "c^2(1-c)^2", "bulk": a.dtype,
-                   "parallelism": "single-gpu", "zchunk": zc.value},
-        "roofline": roof,
-        "cpu_baseline": cpu,
-        "host": {"nproc": os.cpu_count(), "cpu": cpu_model()},
-    }
+    out = WORKLOADS[a.workload](a, not a.no_cpu_baseline)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -169,6 +169,9 @@ int phx_ch_rk4_steps(const phx_grid* grid, int dtype, void* c, void* work,
  * steps of the given state (advancing it: pass a scratch copy), keeps the
  * fastest and reports it. Both are host-side settings; no device allocation. */
 int phx_ch_rk4_set_zchunk(const phx_grid* grid, int dtype, int zc);
+/* XCD-aware workgroup order of the 3D stage kernel (on by default): each of
+ * the 8 XCDs gets a contiguous range of tiles so halo re-reads hit its L2. */
+int phx_ch_set_xcd_remap(int on);
 int phx_ch_rk4_autotune(const phx_grid* grid, int dtype, void* c, void* work,
                         const phx_ch_params* p, int steps, int* best_zc,
                         double* best_ms_per_step, phx_stream stream);

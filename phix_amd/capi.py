@@ -121,6 +121,7 @@ SIG = {
     "phx_ch_rk4_workspace_bytes": (C.c_size_t, [GP, C.c_int]),
     "phx_ch_rk4_steps": (C.c_int, [GP, C.c_int, VP, VP, C.POINTER(CHParams), C.c_int, VP]),
     "phx_ch_rk4_set_zchunk": (C.c_int, [GP, C.c_int, C.c_int]),
+    "phx_ch_set_xcd_remap": (C.c_int, [C.c_int]),
     "phx_ch_rk4_autotune": (C.c_int, [GP, C.c_int, VP, VP, C.POINTER(CHParams), C.c_int, C.POINTER(C.c_int),
                                       C.POINTER(C.c_double), VP]),
     "phx_reduce_workspace_bytes": (C.c_size_t, [GP]),

@@ -6,8 +6,7 @@
 // (include/equation/TermPW.inl:212-268), so an N-field coupling costs ~N^2
 // launches per stage. Here one kernel evaluates every component's RHS for a
 // tile and applies the RK4 stage epilogue to all of them:
-//   for each field: halo tile -> LDS (double buffered), lap and centre value
-//                   kept in registers;
+//   all fields' halo tiles -> LDS behind one barrier (their loads overlap);
 //   then per field: rhs_i = L*kappa*lap_i + L*(e_i - e_i^3)
 //                           + sum_{j!=i} (-2*L*gamma)*(e_i*e_j*e_j)
 //                   in that order (the order of the VectorEquation terms),
@@ -52,56 +51,64 @@ template <typename T, int DIM, int NF, int TX, int TY, bool EXACT>
 __global__ __launch_bounds__(256) void k_ac_stage_2d(ACArgs<T> A) {
     constexpr int UX = TX + 2, UY = TY + 2;
     constexpr int R = (TX * TY) / 256;   // outputs per thread
-    __shared__ T U[2][UY][UX];
+    constexpr int NS = (UX * UY + 255) / 256;
+    __shared__ T U[NF][UY * UX];
     const int tid = threadIdx.x;
     const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
     const bool interior = (x0 >= 1) && (x0 + TX < A.L.n0) && (DIM < 2 || ((y0 >= 1) && (y0 + TY < A.L.n1)));
 
-    T e[NF][R], lp[NF][R];
+    // every field's halo tile is requested before the single barrier, so the
+    // NF global latencies overlap
+    if (interior) {
+        const long long b0 = A.L.at(x0 - 1, y0 - 1, 0);
 #pragma unroll
-    for (int f = 0; f < NF; ++f) {
-        const int b = f & 1;
-        if (interior) {
-            const T* base = A.u[f] + A.L.at(x0 - 1, y0 - 1, 0);
-            for (int t = tid; t < UX * UY; t += 256) {
-                int ly = t / UX, lx = t - ly * UX;
-                U[b][ly][lx] = base[(long long)ly * A.L.sx + lx];
-            }
-        } else {
-            for (int t = tid; t < UX * UY; t += 256) {
-                int ly = t / UX, lx = t - ly * UX;
-                U[b][ly][lx] = ac_load<T, DIM>(A.u[f], A, x0 - 1 + lx, DIM >= 2 ? y0 - 1 + ly : 0);
+        for (int i = 0; i < NS; ++i) {
+            const int t = tid + 256 * i;
+            if (t < UX * UY) {
+                const int ly = t / UX, lx = t - ly * UX;
+                const long long off = b0 + (long long)ly * A.L.sx + lx;
+#pragma unroll
+                for (int f = 0; f < NF; ++f) U[f][t] = A.u[f][off];
             }
         }
-        __syncthreads();
+    } else {
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            int t = tid + r * 256;
-            int oy = t / TX, ox = t - oy * TX;
-            T cc = U[b][oy + 1][ox + 1];
-            T val = (U[b][oy + 1][ox + 2] - (T)2 * cc + U[b][oy + 1][ox]) * A.idx2;
-            if (DIM >= 2) val += (U[b][oy + 2][ox + 1] - (T)2 * cc + U[b][oy][ox + 1]) * A.idy2;
-            e[f][r] = cc;
-            lp[f][r] = val;
+        for (int i = 0; i < NS; ++i) {
+            const int t = tid + 256 * i;
+            if (t < UX * UY) {
+                const int ly = t / UX, lx = t - ly * UX;
+#pragma unroll
+                for (int f = 0; f < NF; ++f) U[f][t] = ac_load<T, DIM>(A.u[f], A, x0 - 1 + lx, DIM >= 2 ? y0 - 1 + ly : 0);
+            }
         }
-        // buffer b is rewritten by field f+2, after the barrier of field f+1
     }
+    __syncthreads();
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        int t = tid + r * 256;
-        int oy = t / TX, ox = t - oy * TX;
-        int x = x0 + ox, y = y0 + oy;
+        const int t = tid + r * 256;
+        const int oy = t / TX, ox = t - oy * TX;
+        const int x = x0 + ox, y = y0 + oy;
         if (x >= A.L.n0 || y >= A.L.n1) continue;
         const long long gi = A.L.at(x, y, 0);
+        const int c0 = (oy + 1) * UX + ox + 1;
+        T e[NF], lp[NF];
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
-            T ef = e[f][r];
-            T k = A.cl * lp[f][r];
+            const T cc = U[f][c0];
+            T val = (U[f][c0 + 1] - (T)2 * cc + U[f][c0 - 1]) * A.idx2;
+            if (DIM >= 2) val += (U[f][c0 + UX] - (T)2 * cc + U[f][c0 - UX]) * A.idy2;
+            e[f] = cc;
+            lp[f] = val;
+        }
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            const T ef = e[f];
+            T k = A.cl * lp[f];
             k = k + A.cp * (ef - ef * ef * ef);
 #pragma unroll
             for (int h = 0; h < NF; ++h) {
                 if (h == f) continue;
-                k = k + A.cg * (ef * e[h][r] * e[h][r]);
+                k = k + A.cg * (ef * e[h] * e[h]);
             }
             const int s = A.stage;
             if (EXACT) {

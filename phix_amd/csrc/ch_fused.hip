@@ -57,6 +57,7 @@ struct CHArgs {
     T dt6, dt3;
     int stage;      // 1..4
     int zc;         // z planes per block (3D)
+    int xcd_remap;  // 1: XCD-aware tile order (grid size a multiple of 8)
 };
 
 template <typename T>
@@ -211,8 +212,21 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
     __shared__ T U[4][UY * UX];
     __shared__ T MU[3][MY * MX];
     const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
-    const int z0 = blockIdx.z * A.zc;
+    // XCD-aware tile order: workgroups are dispatched round-robin over the 8
+    // XCDs by linear id, so hand each XCD a contiguous range of tiles (whole
+    // xy slabs of consecutive z-chunks) -- tiles sharing halos then share an L2.
+    int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+    if (A.xcd_remap) {
+        const int gx = gridDim.x, gy = gridDim.y;
+        const int nb = gx * gy * gridDim.z;
+        const int h = bx + gx * (by + gy * bz);
+        const int lb = (h & 7) * (nb >> 3) + (h >> 3);
+        bx = lb % gx;
+        by = (lb / gx) % gy;
+        bz = lb / (gx * gy);
+    }
+    const int x0 = bx * TX, y0 = by * TY;
+    const int z0 = bz * A.zc;
     const int z1 = min(z0 + A.zc, A.L.n2);
     const int g = A.L.g;
     const long long sxy = A.L.sxy;
@@ -271,20 +285,45 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
         for (int i = 0; i < NU; ++i)
             if (tid + 256 * i < UX * UY) dst[tid + 256 * i] = r[i];
     };
+    // mu slots: LDS centre offset in the halo plane, and halo position
+    int mc0[NM], mpx[NM], mpy[NM];
+#pragma unroll
+    for (int i = 0; i < NM; ++i) {
+        const int t = tid + 256 * i;
+        const int my = t / MX, mx = t - my * MX;
+        mc0[i] = (my + 1) * UX + mx + 1;
+        mpx[i] = x0 - 1 + mx;
+        mpy[i] = y0 - 1 + my;
+    }
+    // tiles whose mu ring lies inside the domain in x and y need no BC logic
+    const bool xy_in = x0 >= 1 && x0 + TX < A.L.n0 && y0 >= 1 && y0 + TY < A.L.n1;
     auto compute_mu = [&](int m) {
         const T* ul = U[(m + 3) & 3];
         const T* um = U[(m + 4) & 3];
         const T* uh = U[(m + 5) & 3];
         T* dst = MU[(m + 3) % 3];
+        if (xy_in && m >= 0 && m < A.L.n2) {
 #pragma unroll
-        for (int i = 0; i < NM; ++i) {
-            const int t = tid + 256 * i;
-            if (t < MX * MY) {
-                const int my = t / MX, mx = t - my * MX;
-                const int c0 = (my + 1) * UX + mx + 1;
-                const T c = um[c0];
-                dst[t] = mu_at<T, 3>(A, x0 - 1 + mx, y0 - 1 + my, m, c, um[c0 + 1], um[c0 - 1], um[c0 + UX],
-                                     um[c0 - UX], uh[c0], ul[c0]);
+            for (int i = 0; i < NM; ++i) {
+                const int t = tid + 256 * i;
+                if (t < MX * MY) {
+                    const int c0 = mc0[i];
+                    const T c = um[c0];
+                    T val = (um[c0 + 1] - (T)2 * c + um[c0 - 1]) * A.idx2;
+                    val += (um[c0 + UX] - (T)2 * c + um[c0 - UX]) * A.idy2;
+                    val += (uh[c0] - (T)2 * c + ul[c0]) * A.idz2;
+                    dst[t] = dwell(c, A.two_rho, A.ca, A.cb) + A.nkappa * val;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < NM; ++i) {
+                const int t = tid + 256 * i;
+                if (t < MX * MY) {
+                    const int c0 = mc0[i];
+                    dst[t] = mu_at<T, 3>(A, mpx[i], mpy[i], m, um[c0], um[c0 + 1], um[c0 - 1], um[c0 + UX],
+                                         um[c0 - UX], uh[c0], ul[c0]);
+                }
             }
         }
     };
@@ -523,6 +562,8 @@ constexpr int T2X = 64, T2Y = 16;
 constexpr int T3X = 64;
 template <typename T> struct T3Y { static constexpr int v = sizeof(T) == 4 ? 16 : 8; };
 
+static int g_xcd_remap = 1;
+
 template <typename T, bool EXACT>
 static int rk4_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_params* p, int nsteps, hipStream_t st,
                        int zc) {
@@ -542,11 +583,15 @@ static int rk4_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_params* p,
     A.c = c;
     A.acc = acc;
     A.zc = zc;
+    A.xcd_remap = 0;
     const T shift[4] = {(T)(dt * 0.5), (T)(dt * 0.5), (T)dt, (T)0};
     const T* uin[4] = {c, w0, w1, w0};
     T* uout[4] = {w0, w1, w0, c};
     dim3 grid;
-    if (G->dim == 3) grid = dim3(cdiv(A.L.n0, T3X), cdiv(A.L.n1, T3Y<T>::v), cdiv(A.L.n2, zc));
+    if (G->dim == 3) {
+        grid = dim3(cdiv(A.L.n0, T3X), cdiv(A.L.n1, T3Y<T>::v), cdiv(A.L.n2, zc));
+        A.xcd_remap = g_xcd_remap && (grid.x * grid.y * grid.z) % 8 == 0;
+    }
     else grid = dim3(cdiv(A.L.n0, T2X), cdiv(A.L.n1, T2Y), 1);
     for (int s = 0; s < nsteps; ++s) {
         if (s == nsteps - 1) {
@@ -609,6 +654,11 @@ static int zchunk_for(const phx_grid* G, int dtype) {
     std::lock_guard<std::mutex> lk(g_zc_mu);
     auto it = g_zc.find(std::make_tuple(G->n[0], G->n[1], G->n[2], dtype));
     return it == g_zc.end() ? 16 : it->second;
+}
+
+int phx_ch_set_xcd_remap(int on) {
+    g_xcd_remap = on ? 1 : 0;
+    return PHX_OK;
 }
 
 int phx_ch_rk4_set_zchunk(const phx_grid* G, int dtype, int zc) {
