@@ -100,10 +100,18 @@ def bench_ch3d(a, with_cpu):
     stream = torch.cuda.current_stream()
     sh = capi.stream_handle(stream)
     zc, tune_ms = C.c_int(0), C.c_double(0)
-    scratch = c.clone()
-    capi.call("phx_ch_rk4_autotune", C.byref(g), dtype, capi.ptr(scratch), capi.ptr(work), C.byref(p), 10,
-              C.byref(zc), C.byref(tune_ms), sh)
-    del scratch
+    if a.tile is None and a.zc is None:
+        scratch = c.clone()
+        capi.call("phx_ch_rk4_autotune", C.byref(g), dtype, capi.ptr(scratch), capi.ptr(work), C.byref(p), 10,
+                  C.byref(zc), C.byref(tune_ms), sh)
+        del scratch
+    else:
+        if a.tile is not None:
+            capi.call("phx_ch_rk4_set_tile", C.byref(g), dtype, a.tile)
+        if a.zc is not None:
+            capi.call("phx_ch_rk4_set_zchunk", C.byref(g), dtype, a.zc)
+    tile = C.c_int(0)
+    capi.call("phx_ch_rk4_get_launch", C.byref(g), dtype, C.byref(zc), C.byref(tile))
 
     def run(n):
         capi.call("phx_ch_rk4_steps", C.byref(g), dtype, capi.ptr(c), capi.ptr(work), C.byref(p), n, sh)
@@ -129,7 +137,7 @@ def bench_ch3d(a, with_cpu):
                          f"(oracle/phx_oracle.c orc_ch_rk4_steps), {dt:.1f} s on {cpu_model()}"}
     cfg = {"workload": "ch3d_rk4_256cube_noflux", "grid": [NX] * 3, "bc": "noflux", "scheme": "rk4", "dt": 0.005,
            "kappa": 1.0, "M": 1.0, "f": "c^2(1-c)^2", "bulk": a.dtype, "parallelism": "single-gpu",
-           "zchunk": zc.value}
+           "zchunk": zc.value, "tile": ["64x16", "64x8", "128x8", "32x16", "64x4"][tile.value]}
     return line("cell-updates/s, Cahn-Hilliard RK4 (fused stencil, %HBM roofline in 'roofline')",
                 cells * a.steps / (ms / 1e3), "cell-updates/s", a.steps, a.warmup, ms, a.dtype,
                 "synthetic (c0 = 0.4 + U(-0.02, 0.02), SplitMix64 seed 2)", cfg, roof, cpu)
@@ -305,7 +313,25 @@ def bench_euler(a, with_cpu):
                 cfg, roof, cpu)
 
 
-WORKLOADS = {"ch3d": bench_ch3d, "ch2d1024": bench_ch2d, "ac1024": bench_ac, "ens512": bench_ens,
+def bench_copy(a, with_cpu):
+    """Plain device copy of 2 GiB (the STREAM-copy bandwidth of this box, and the
+    known-size kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters)."""
+    import torch
+    n = (2 << 30) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    stream = torch.cuda.current_stream()
+    dst.copy_(src)
+    reps = 10
+    ms = timed(lambda: [dst.copy_(src) for _ in range(reps)], stream)
+    per = ms / 1e3 / reps
+    roof = roofline(2.0 * 4 * n, per, "copy (torch)", "copy_2gib")
+    cfg = {"workload": "copy_2gib", "bytes_read": 4 * n, "bytes_written": 4 * n, "parallelism": "single-gpu"}
+    return line("GB/s, device-to-device copy (read + write bytes / time)", 2.0 * 4 * n / per / 1e9, "GB/s", reps, 1,
+                ms, "f32", "constant buffer", cfg, roof, None)
+
+
+WORKLOADS = {"copycal": bench_copy, "ch3d": bench_ch3d, "ch2d1024": bench_ch2d, "ac1024": bench_ac, "ens512": bench_ens,
              "euler128": bench_euler}
 
 
@@ -317,6 +343,8 @@ def main(argv=None) -> int:
     ap.add_argument("--dtype", choices=["f32", "f64"], default="f32")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="ch3d")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tile", type=int, default=None, help="3D stage tile index (skips the autotuner)")
+    ap.add_argument("--zc", type=int, default=None, help="3D stage z-chunk (skips the autotuner)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "

@@ -165,10 +165,15 @@ int phx_ch_rk4_steps(const phx_grid* grid, int dtype, void* c, void* work,
                      const phx_ch_params* p, int nsteps, phx_stream stream);
 
 /* Launch shape of the 3D stage kernel: z planes streamed per workgroup
- * (default 16), per (n, dtype). The autotuner times {4,8,16,32,64} on `steps`
+ * (default 16), per (n, dtype). The autotuner times every tile shape x
+ * z-chunk {4,8,16,32,64} on `steps`
  * steps of the given state (advancing it: pass a scratch copy), keeps the
  * fastest and reports it. Both are host-side settings; no device allocation. */
 int phx_ch_rk4_set_zchunk(const phx_grid* grid, int dtype, int zc);
+/* Tile shape of the 3D stage kernel: 0: 64x16, 1: 64x8, 2: 128x8, 3: 32x16,
+ * 4: 64x4 (x by y columns per 256-thread workgroup). */
+int phx_ch_rk4_set_tile(const phx_grid* grid, int dtype, int tile);
+int phx_ch_rk4_get_launch(const phx_grid* grid, int dtype, int* zc, int* tile);
 /* XCD-aware workgroup order of the 3D stage kernel (on by default): each of
  * the 8 XCDs gets a contiguous range of tiles so halo re-reads hit its L2. */
 int phx_ch_set_xcd_remap(int on);
@@ -245,8 +250,11 @@ int phx_ensemble_autotune(phx_ensemble* ens, float* c, float* work, double dt, i
  * there for the following backward. */
 int    phx_sur_param_count(void);
 size_t phx_sur_workspace_bytes(int B, int H, int W);
+/* flags: PHX_SUR_REUSE_WEIGHTS skips re-packing the BF16 weights (valid when
+ * params are unchanged since the last call on this ws). */
+#define PHX_SUR_REUSE_WEIGHTS 1
 int    phx_sur_forward(const float* params, void* ws, const float* c_in, float* c_out,
-                       int B, int H, int W, phx_stream stream);
+                       int B, int H, int W, int flags, phx_stream stream);
 /* Loss = mean((c_out - c_target)^2); grad (param layout) <- dLoss/dparams;
  * loss_dev (device double, may be NULL) <- sum of squared errors (divide by
  * B*H*W for the loss). Needs the preceding phx_sur_forward on the same ws. */

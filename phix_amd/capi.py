@@ -122,6 +122,8 @@ SIG = {
     "phx_ch_rk4_steps": (C.c_int, [GP, C.c_int, VP, VP, C.POINTER(CHParams), C.c_int, VP]),
     "phx_ch_rk4_set_zchunk": (C.c_int, [GP, C.c_int, C.c_int]),
     "phx_ch_set_xcd_remap": (C.c_int, [C.c_int]),
+    "phx_ch_rk4_get_launch": (C.c_int, [GP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "phx_ch_rk4_set_tile": (C.c_int, [GP, C.c_int, C.c_int]),
     "phx_ch_rk4_autotune": (C.c_int, [GP, C.c_int, VP, VP, C.POINTER(CHParams), C.c_int, C.POINTER(C.c_int),
                                       C.POINTER(C.c_double), VP]),
     "phx_reduce_workspace_bytes": (C.c_size_t, [GP]),
@@ -142,7 +144,7 @@ SIG = {
     "phx_ensemble_autotune": (C.c_int, [VP, VP, VP, C.c_double, C.c_int, C.POINTER(C.c_double), VP]),
     "phx_sur_param_count": (C.c_int, []),
     "phx_sur_workspace_bytes": (C.c_size_t, [C.c_int, C.c_int, C.c_int]),
-    "phx_sur_forward": (C.c_int, [VP, VP, VP, VP, C.c_int, C.c_int, C.c_int, VP]),
+    "phx_sur_forward": (C.c_int, [VP, VP, VP, VP, C.c_int, C.c_int, C.c_int, C.c_int, VP]),
     "phx_sur_backward": (C.c_int, [VP, VP, VP, VP, VP, VP, VP, C.c_int, C.c_int, C.c_int, VP]),
     "phx_sur_adam": (C.c_int, [VP, VP, VP, VP, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, VP]),
     "phx_conv3x3_ws_bytes": (C.c_size_t, []),

@@ -559,14 +559,29 @@ int bc_fill_impl(const phx_grid* G, int dtype, void* data, hipStream_t st);
 
 // Tile shapes (autotuned later; these are the defaults).
 constexpr int T2X = 64, T2Y = 16;
-constexpr int T3X = 64;
-template <typename T> struct T3Y { static constexpr int v = sizeof(T) == 4 ? 16 : 8; };
+// 3D tile shapes the autotuner chooses from (TX x TY columns, 256 threads).
+struct Tile3 {
+    int tx, ty;
+};
+static const Tile3 kTiles3[] = {{64, 16}, {64, 8}, {128, 8}, {32, 16}, {64, 4}};
+constexpr int kNumTiles3 = sizeof(kTiles3) / sizeof(kTiles3[0]);
+
+template <typename T, bool EXACT>
+static void launch3(int tile, dim3 grid, const CHArgs<T>& A, hipStream_t st) {
+    switch (tile) {
+        case 0: hipLaunchKernelGGL((k_ch_stage_3d<T, 64, 16, EXACT>), grid, dim3(256), 0, st, A); break;
+        case 1: hipLaunchKernelGGL((k_ch_stage_3d<T, 64, 8, EXACT>), grid, dim3(256), 0, st, A); break;
+        case 2: hipLaunchKernelGGL((k_ch_stage_3d<T, 128, 8, EXACT>), grid, dim3(256), 0, st, A); break;
+        case 3: hipLaunchKernelGGL((k_ch_stage_3d<T, 32, 16, EXACT>), grid, dim3(256), 0, st, A); break;
+        default: hipLaunchKernelGGL((k_ch_stage_3d<T, 64, 4, EXACT>), grid, dim3(256), 0, st, A); break;
+    }
+}
 
 static int g_xcd_remap = 1;
 
 template <typename T, bool EXACT>
 static int rk4_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_params* p, int nsteps, hipStream_t st,
-                       int zc) {
+                       int zc, int tile) {
     CHArgs<T> A;
     A.L = make_layout(G);
     const long long N = A.L.total();
@@ -589,7 +604,7 @@ static int rk4_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_params* p,
     T* uout[4] = {w0, w1, w0, c};
     dim3 grid;
     if (G->dim == 3) {
-        grid = dim3(cdiv(A.L.n0, T3X), cdiv(A.L.n1, T3Y<T>::v), cdiv(A.L.n2, zc));
+        grid = dim3(cdiv(A.L.n0, kTiles3[tile].tx), cdiv(A.L.n1, kTiles3[tile].ty), cdiv(A.L.n2, zc));
         A.xcd_remap = g_xcd_remap && (grid.x * grid.y * grid.z) % 8 == 0;
     }
     else grid = dim3(cdiv(A.L.n0, T2X), cdiv(A.L.n1, T2Y), 1);
@@ -603,7 +618,7 @@ static int rk4_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_params* p,
             A.u = uin[stg];
             A.out = uout[stg];
             A.a = shift[stg];
-            if (G->dim == 3) hipLaunchKernelGGL((k_ch_stage_3d<T, T3X, T3Y<T>::v, EXACT>), grid, dim3(256), 0, st, A);
+            if (G->dim == 3) launch3<T, EXACT>(tile, grid, A, st);
             else if (G->dim == 2) hipLaunchKernelGGL((k_ch_stage_2d<T, 2, T2X, T2Y, EXACT>), grid, dim3(256), 0, st, A);
             else hipLaunchKernelGGL((k_ch_stage_2d<T, 1, T2X, T2Y, EXACT>), grid, dim3(256), 0, st, A);
         }
@@ -647,13 +662,22 @@ size_t phx_ch_rk4_workspace_bytes(const phx_grid* G, int dtype) {
 
 // z planes per block of the 3D stage kernel: tuned per (shape, dtype) by
 // phx_ch_rk4_autotune, else 16.
+struct Launch3 {
+    int zc, tile;
+};
 static std::mutex g_zc_mu;
-static std::map<std::tuple<int, int, int, int>, int> g_zc;
+static std::map<std::tuple<int, int, int, int>, Launch3> g_zc;
 
-static int zchunk_for(const phx_grid* G, int dtype) {
+static Launch3 launch_for(const phx_grid* G, int dtype) {
     std::lock_guard<std::mutex> lk(g_zc_mu);
     auto it = g_zc.find(std::make_tuple(G->n[0], G->n[1], G->n[2], dtype));
-    return it == g_zc.end() ? 16 : it->second;
+    if (it != g_zc.end()) return it->second;
+    return Launch3{16, dtype == PHX_F32 ? 0 : 1};
+}
+
+static void set_launch(const phx_grid* G, int dtype, Launch3 l) {
+    std::lock_guard<std::mutex> lk(g_zc_mu);
+    g_zc[std::make_tuple(G->n[0], G->n[1], G->n[2], dtype)] = l;
 }
 
 int phx_ch_set_xcd_remap(int on) {
@@ -665,8 +689,28 @@ int phx_ch_rk4_set_zchunk(const phx_grid* G, int dtype, int zc) {
     int rc = check_grid(G, "phx_ch_rk4_set_zchunk");
     if (rc) return rc;
     if (zc < 1 || zc > 4096) return fail(PHX_ERR_INVALID, "phx_ch_rk4_set_zchunk: zc must be in [1, 4096]");
-    std::lock_guard<std::mutex> lk(g_zc_mu);
-    g_zc[std::make_tuple(G->n[0], G->n[1], G->n[2], dtype)] = zc;
+    Launch3 l = launch_for(G, dtype);
+    l.zc = zc;
+    set_launch(G, dtype, l);
+    return PHX_OK;
+}
+
+int phx_ch_rk4_get_launch(const phx_grid* G, int dtype, int* zc, int* tile) {
+    int rc = check_grid(G, "phx_ch_rk4_get_launch");
+    if (rc) return rc;
+    Launch3 l = launch_for(G, dtype);
+    if (zc) *zc = l.zc;
+    if (tile) *tile = l.tile;
+    return PHX_OK;
+}
+
+int phx_ch_rk4_set_tile(const phx_grid* G, int dtype, int tile) {
+    int rc = check_grid(G, "phx_ch_rk4_set_tile");
+    if (rc) return rc;
+    if (tile < 0 || tile >= kNumTiles3) return fail(PHX_ERR_INVALID, "phx_ch_rk4_set_tile: tile index out of range");
+    Launch3 l = launch_for(G, dtype);
+    l.tile = tile;
+    set_launch(G, dtype, l);
     return PHX_OK;
 }
 
@@ -679,10 +723,10 @@ int phx_ch_rk4_steps(const phx_grid* G, int dtype, void* c, void* work, const ph
     if (!c || !work) return fail(PHX_ERR_INVALID, "phx_ch_rk4_steps: NULL buffer");
     if (nsteps < 0) return fail(PHX_ERR_INVALID, "phx_ch_rk4_steps: nsteps < 0");
     if (nsteps == 0) return PHX_OK;
-    const int zc = G->dim == 3 ? zchunk_for(G, dtype) : 1;
+    const Launch3 l = G->dim == 3 ? launch_for(G, dtype) : Launch3{1, 0};
     return dtype == PHX_F64
-               ? rk4_steps_t<double, true>(G, (double*)c, (double*)work, p, nsteps, as_stream(s), zc)
-               : rk4_steps_t<float, false>(G, (float*)c, (float*)work, p, nsteps, as_stream(s), zc);
+               ? rk4_steps_t<double, true>(G, (double*)c, (double*)work, p, nsteps, as_stream(s), l.zc, l.tile)
+               : rk4_steps_t<float, false>(G, (float*)c, (float*)work, p, nsteps, as_stream(s), l.zc, l.tile);
 }
 
 // Times the 3D stage kernel at z-chunks {4, 8, 16, 32, 64} on `steps` steps of
@@ -695,13 +739,14 @@ int phx_ch_rk4_autotune(const phx_grid* G, int dtype, void* c, void* work, const
     hipStream_t st = as_stream(s);
     const int cand[] = {4, 8, 16, 32, 64};
     float best = 1e30f;
-    int bz = 16;
+    int bz = 16, bt = 0;
     hipEvent_t e0, e1;
     PHX_HIP(hipEventCreate(&e0));
     PHX_HIP(hipEventCreate(&e1));
+    for (int tile = 0; tile < (G->dim == 3 ? kNumTiles3 : 1); ++tile)
     for (int zc : cand) {
         if (G->dim == 3 && zc > G->n[2]) continue;
-        if ((rc = phx_ch_rk4_set_zchunk(G, dtype, zc))) return rc;
+        set_launch(G, dtype, Launch3{zc, tile});
         if ((rc = phx_ch_rk4_steps(G, dtype, c, work, p, 1, s))) return rc;
         PHX_HIP(hipEventRecord(e0, st));
         if ((rc = phx_ch_rk4_steps(G, dtype, c, work, p, steps, s))) return rc;
@@ -712,11 +757,12 @@ int phx_ch_rk4_autotune(const phx_grid* G, int dtype, void* c, void* work, const
         if (ms < best) {
             best = ms;
             bz = zc;
+            bt = tile;
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    phx_ch_rk4_set_zchunk(G, dtype, bz);
+    set_launch(G, dtype, Launch3{bz, bt});
     if (best_zc) *best_zc = bz;
     if (best_ms_per_step) *best_ms_per_step = best / steps;
     return PHX_OK;

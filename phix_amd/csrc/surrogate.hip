@@ -458,10 +458,12 @@ static int check_shape(int B, int H, int W, const char* who) {
     return PHX_OK;
 }
 
-static int sur_forward(const float* P, SurWS& S, const float* c, float* out, int B, int H, int W, hipStream_t st) {
+static int sur_forward(const float* P, SurWS& S, const float* c, float* out, int B, int H, int W, int flags,
+                       hipStream_t st) {
     const long long n = (long long)B * H * W;
     const int nb = (int)((n + 255) / 256);
-    hipLaunchKernelGGL(k_pack, dim3((4 * MID_W + 255) / 256), dim3(256), 0, st, P, S.wfwd, S.wbwd);
+    if (!(flags & PHX_SUR_REUSE_WEIGHTS))
+        hipLaunchKernelGGL(k_pack, dim3((4 * MID_W + 255) / 256), dim3(256), 0, st, P, S.wfwd, S.wbwd);
     hipLaunchKernelGGL(k_conv_in_fwd, dim3(nb), dim3(256), 0, st, c, P, S.act[0], B, H, W);
     for (int l = 0; l < 4; ++l)
         hipLaunchKernelGGL(k_conv_mid, dim3(W / 16, H / 16, B), dim3(256), 0, st, S.act[l], S.wfwd + l * MID_W,
@@ -486,14 +488,14 @@ size_t phx_sur_workspace_bytes(int B, int H, int W) {
     return ws_layout(B, H, W, nullptr, nullptr);
 }
 
-int phx_sur_forward(const float* params, void* ws, const float* c_in, float* c_out, int B, int H, int W,
+int phx_sur_forward(const float* params, void* ws, const float* c_in, float* c_out, int B, int H, int W, int flags,
                     phx_stream s) {
     int rc = check_shape(B, H, W, "phx_sur_forward");
     if (rc) return rc;
     if (!params || !ws || !c_in || !c_out) return fail(PHX_ERR_INVALID, "phx_sur_forward: NULL argument");
     SurWS S;
     ws_layout(B, H, W, &S, (char*)ws);
-    return sur_forward(params, S, c_in, c_out, B, H, W, as_stream(s));
+    return sur_forward(params, S, c_in, c_out, B, H, W, flags, as_stream(s));
 }
 
 int phx_sur_backward(const float* params, void* ws, const float* c_in, const float* c_out, const float* c_target,

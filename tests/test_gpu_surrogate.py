@@ -165,7 +165,7 @@ def test_network_forward_backward_matches_fp64(dev):
     ws = torch.empty(capi.lib().phx_sur_workspace_bytes(B, H, W), dtype=torch.uint8, device=dev)
     grad = torch.zeros(npar, dtype=torch.float32, device=dev)
     loss = torch.zeros(1, dtype=torch.float64, device=dev)
-    capi.call("phx_sur_forward", capi.ptr(tp), capi.ptr(ws), capi.ptr(tc), capi.ptr(out), B, H, W, stream())
+    capi.call("phx_sur_forward", capi.ptr(tp), capi.ptr(ws), capi.ptr(tc), capi.ptr(out), B, H, W, 0, stream())
     capi.call("phx_sur_backward", capi.ptr(tp), capi.ptr(ws), capi.ptr(tc), capi.ptr(out), capi.ptr(tt),
               capi.ptr(grad), capi.ptr(loss), B, H, W, stream())
     torch.cuda.synchronize()
@@ -206,7 +206,7 @@ def test_adam_and_training_reduces_loss(dev):
     loss = torch.zeros(1, dtype=torch.float64, device=dev)
     losses = []
     for step in range(1, 201):
-        capi.call("phx_sur_forward", capi.ptr(tp), capi.ptr(ws), capi.ptr(c), capi.ptr(out), B, H, W, stream())
+        capi.call("phx_sur_forward", capi.ptr(tp), capi.ptr(ws), capi.ptr(c), capi.ptr(out), B, H, W, 0, stream())
         capi.call("phx_sur_backward", capi.ptr(tp), capi.ptr(ws), capi.ptr(c), capi.ptr(out), capi.ptr(tgt),
                   capi.ptr(grad), capi.ptr(loss), B, H, W, stream())
         capi.call("phx_sur_adam", capi.ptr(tp), capi.ptr(grad), capi.ptr(m), capi.ptr(v), step, 1e-3, 0.9, 0.999,
@@ -215,3 +215,32 @@ def test_adam_and_training_reduces_loss(dev):
             losses.append(loss.item() / c.numel())
     print("training losses:", losses)
     assert losses[-1] < 0.2 * losses[0]
+
+
+# ---------------------------------------------------- learned part: the gain
+def test_trained_surrogate_gains_over_untrained(dev):
+    """With the committed trained weights the solver-judged hybrid accepts
+    surrogate blocks (fewer solver steps than the plain solver) and still meets
+    the 1e-2 relative-L2 bound against the FP64 solver; the same hybrid with an
+    untrained (identity) network is rejected on every block and gains nothing."""
+    import torch
+    from phix_amd import surrogate as S
+    if not S.WEIGHTS.exists():
+        pytest.skip("no trained surrogate weights committed yet")
+    params = np.load(S.WEIGHTS)["params"]
+    nsteps = 6000
+    c0 = S.init.uniform(S.init.substream(4321, 7), 256 * 256, 0.45, 0.55).reshape(256, 256)
+    ref = S.fp64_reference(dev, c0, nsteps)
+    x0 = torch.from_numpy(c0).float().to(dev)
+    hyb = S.HybridSolver(dev, params)
+    out, rep = hyb.run(x0, nsteps)
+    err = float(np.linalg.norm(out.cpu().numpy() - ref) / np.linalg.norm(ref))
+    untrained = S.Surrogate(dev, 1).init_params(0)
+    hyb0 = S.HybridSolver(dev, untrained)
+    out0, rep0 = hyb0.run(x0, nsteps)
+    err0 = float(np.linalg.norm(out0.cpu().numpy() - ref) / np.linalg.norm(ref))
+    print(f"trained: {rep.solver_steps} solver steps, {rep.rejected}/{rep.blocks} blocks rejected, rel-L2 {err:.3e}; "
+          f"untrained: {rep0.solver_steps} solver steps, {rep0.rejected}/{rep0.blocks} rejected, rel-L2 {err0:.3e}")
+    assert err <= 1e-2 and err0 <= 1e-2
+    assert rep0.rejected == rep0.blocks
+    assert rep.solver_steps < rep0.solver_steps
