@@ -4,8 +4,9 @@ Default workload (the north-star configuration that fits one GPU): 3D
 Cahn-Hilliard, 256^3 cells, dx=1, zero-flux BCs on all faces,
 f = c^2(1-c)^2, kappa=1, M=1, dt=0.005, classic RK4, FP32 stencil bulk;
 c0 = 0.4 + U(-0.02, 0.02) from SplitMix64 seed 2. One step = one RK4 time
-step over the whole grid (four fused stage kernels). Inputs are resident in
-HBM before the timed region.
+step over the whole grid, with the launch (one-stage or two-stage kernel,
+tile, z-chunk) chosen by the library's autotuner unless --tile/--zc are given.
+Inputs are resident in HBM before the timed region.
 
 Other workloads (--workload), each a configuration of the same path:
   euler128  128^2 periodic CH, multi-step Euler, FP64, kappa=0.5, dt=0.01
@@ -15,8 +16,9 @@ Other workloads (--workload), each a configuration of the same path:
             one persistent launch per stage, hipGraph-captured step loop
 
 Prints ONE JSON line. `roofline` is computed for the dominant kernel from its
-algorithmic bytes per launch divided by its average launch duration measured
-with HIP events on the launch stream. `cpu_baseline` times the FP64 CPU oracle
+algorithmic bytes per launch divided by its average launch duration, measured
+with HIP events bracketing every launch on the launch stream over a second run
+of the same steps (the events would add ~9% to the timed region itself). `cpu_baseline` times the FP64 CPU oracle
 (a restatement of the reference's per-term CPU path, one thread) on a bounded
 sample of the same workload.
 """
@@ -35,6 +37,9 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# launch tile indices of the 3D CH kernels: one-stage 0-4, two-stage 5-8
+TILE_NAMES = ["64x16", "64x8", "128x8", "32x16", "64x4", "two-stage 64x8", "two-stage 32x16", "two-stage 32x8",
+              "two-stage 64x4"]
 
 
 def cpu_model() -> str:
@@ -69,6 +74,24 @@ def timed(fn, stream):
     ev1.record(stream)
     torch.cuda.synchronize()
     return ev0.elapsed_time(ev1)
+
+
+def kernel_times(run, steps):
+    """Per-kind launch durations of the CH RK4 kernels: the same `steps` steps
+    as the timed region, run again with every launch bracketed by HIP events on
+    its stream (kept out of the timed region: the events add ~9% there).
+    Returns {kind: (launches, total_ms)}."""
+    from phix_amd import capi
+    capi.call("phx_ch_timing", 1)
+    run(steps)
+    kinds = {}
+    for k in range(4):
+        n, t = C.c_int(), C.c_double()
+        capi.call("phx_ch_timing_read", k, C.byref(n), C.byref(t))
+        if n.value:
+            kinds[k] = (n.value, t.value)
+    capi.call("phx_ch_timing", 0)
+    return kinds
 
 
 def roofline(bytes_per_launch, launch_s, kernel, traffic_name):
@@ -118,10 +141,20 @@ def bench_ch3d(a, with_cpu):
 
     run(a.warmup)
     ms = timed(lambda: run(a.steps), stream)
+    kinds = kernel_times(run, a.steps)
     cells = NX ** 3
-    # 16 field passes per FP32 step (17 in FP64: c_n re-read in stage 4), 4 launches per step
-    passes = 16 if dtype == capi.F32 else 17
-    roof = roofline(passes / 4 * esz * cells, (ms / 1e3) / (4 * a.steps), "k_ch_stage_3d", f"ch3d_rk4_{a.dtype}_256")
+    # algorithmic field passes per launch: one-stage kernel 16/4 (FP32) or 17/4
+    # (FP64: c_n re-read in stage 4) on average over its 4 stages; two-stage
+    # (1,2): read c_n, write acc and the stage-3 state; (3,4): read the stage-3
+    # state, c_n and acc, write c_{n+1}
+    passes = {0: (16 if dtype == capi.F32 else 17) / 4, 1: 3, 2: 4}
+    names = {0: "k_ch_stage_3d", 1: "k_ch_pair_3d (stages 1+2)", 2: "k_ch_pair_3d (stages 3+4)"}
+    tags = {0: "ch3d_rk4", 1: "ch3d_pair12", 2: "ch3d_pair34"}
+    dom = max(kinds, key=lambda k: kinds[k][1])
+    roof = roofline(passes[dom] * esz * cells, kinds[dom][1] / 1e3 / kinds[dom][0], names[dom],
+                    f"{tags[dom]}_{a.dtype}_256")
+    roof["launches"] = {names[k]: {"count": v[0], "avg_us": round(v[1] * 1e3 / v[0], 2),
+                                   "alg_bytes": passes[k] * esz * cells} for k, v in kinds.items()}
     cpu = None
     if with_cpu:
         import numpy as np
@@ -137,7 +170,7 @@ def bench_ch3d(a, with_cpu):
                          f"(oracle/phx_oracle.c orc_ch_rk4_steps), {dt:.1f} s on {cpu_model()}"}
     cfg = {"workload": "ch3d_rk4_256cube_noflux", "grid": [NX] * 3, "bc": "noflux", "scheme": "rk4", "dt": 0.005,
            "kappa": 1.0, "M": 1.0, "f": "c^2(1-c)^2", "bulk": a.dtype, "parallelism": "single-gpu",
-           "zchunk": zc.value, "tile": ["64x16", "64x8", "128x8", "32x16", "64x4"][tile.value]}
+           "zchunk": zc.value, "tile": TILE_NAMES[tile.value]}
     return line("cell-updates/s, Cahn-Hilliard RK4 (fused stencil, %HBM roofline in 'roofline')",
                 cells * a.steps / (ms / 1e3), "cell-updates/s", a.steps, a.warmup, ms, a.dtype,
                 "synthetic (c0 = 0.4 + U(-0.02, 0.02), SplitMix64 seed 2)", cfg, roof, cpu)
@@ -155,14 +188,32 @@ def bench_ch2d(a, with_cpu):
                        device=dev)
     stream = torch.cuda.current_stream()
     sh = capi.stream_handle(stream)
+    if a.tile is None:
+        scratch = c.clone()
+        zc, tune_ms = C.c_int(0), C.c_double(0)
+        capi.call("phx_ch_rk4_autotune", C.byref(g), capi.F32, capi.ptr(scratch), capi.ptr(work), C.byref(p), 20,
+                  C.byref(zc), C.byref(tune_ms), sh)
+        del scratch
+    else:
+        capi.call("phx_ch_rk4_set_tile", C.byref(g), capi.F32, a.tile)
+    zc, tile = C.c_int(0), C.c_int(0)
+    capi.call("phx_ch_rk4_get_launch", C.byref(g), capi.F32, C.byref(zc), C.byref(tile))
 
     def run(n):
         capi.call("phx_ch_rk4_steps", C.byref(g), capi.F32, capi.ptr(c), capi.ptr(work), C.byref(p), n, sh)
 
     run(a.warmup)
     ms = timed(lambda: run(a.steps), stream)
+    kinds = kernel_times(run, a.steps)
     cells = NX * NX
-    roof = roofline(16 / 4 * 4 * cells, (ms / 1e3) / (4 * a.steps), "k_ch_stage_2d", "ch2d_rk4_f32_1024")
+    # passes per launch: one-stage 16/4 on average; whole step: read c_n, write c_{n+1}
+    passes = {0: 4, 3: 2}
+    names = {0: "k_ch_stage_2d", 3: "k_ch_step_2d"}
+    dom = max(kinds, key=lambda k: kinds[k][1])
+    roof = roofline(passes[dom] * 4 * cells, kinds[dom][1] / 1e3 / kinds[dom][0], names[dom],
+                    f"{'ch2d_rk4' if dom == 0 else 'ch2d_step'}_f32_1024")
+    roof["launches"] = {names[k]: {"count": v[0], "avg_us": round(v[1] * 1e3 / v[0], 2),
+                                   "alg_bytes": passes[k] * 4 * cells} for k, v in kinds.items()}
     cpu = None
     if with_cpu:
         from oracle import oracle as O
@@ -174,7 +225,9 @@ def bench_ch2d(a, with_cpu):
         cpu = {"value": cells * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
                "sample": f"{steps} RK4 steps of the same 1024^2 problem, FP64 oracle, {dt:.1f} s on {cpu_model()}"}
     cfg = {"workload": "ch2d_rk4_1024sq_periodic", "grid": [NX, NX], "bc": "periodic", "scheme": "rk4", "dt": 0.005,
-           "kappa": 1.0, "M": 1.0, "bulk": "f32", "parallelism": "single-gpu"}
+           "kappa": 1.0, "M": 1.0, "bulk": "f32", "parallelism": "single-gpu",
+           "tile": ["64x16 one-stage", "whole-step 64x16", "whole-step 64x32", "whole-step 128x16",
+                    "whole-step 32x32"][tile.value]}
     return line("cell-updates/s, Cahn-Hilliard RK4 2D (fused stencil)", cells * a.steps / (ms / 1e3),
                 "cell-updates/s", a.steps, a.warmup, ms, "f32",
                 "synthetic (c0 = 0.4 + U(-0.02, 0.02), SplitMix64 seed 2)", cfg, roof, cpu)

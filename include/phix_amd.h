@@ -171,7 +171,10 @@ int phx_ch_rk4_steps(const phx_grid* grid, int dtype, void* c, void* work,
  * fastest and reports it. Both are host-side settings; no device allocation. */
 int phx_ch_rk4_set_zchunk(const phx_grid* grid, int dtype, int zc);
 /* Tile shape of the 3D stage kernel: 0: 64x16, 1: 64x8, 2: 128x8, 3: 32x16,
- * 4: 64x4 (x by y columns per 256-thread workgroup). */
+ * 4: 64x4 (x by y columns per 256-thread workgroup). Tiles 5: 64x8, 6: 32x16,
+ * 7: 32x8, 8: 64x4 select the two-stage kernel (stages 1+2 and 3+4 each in one
+ * launch: 7 field passes per step instead of 16/17; same results bit for bit;
+ * needs n >= 4 on every axis, else PHX_ERR_UNSUPPORTED). */
 int phx_ch_rk4_set_tile(const phx_grid* grid, int dtype, int tile);
 int phx_ch_rk4_get_launch(const phx_grid* grid, int dtype, int* zc, int* tile);
 /* XCD-aware workgroup order of the 3D stage kernel (on by default): each of
@@ -180,6 +183,14 @@ int phx_ch_set_xcd_remap(int on);
 int phx_ch_rk4_autotune(const phx_grid* grid, int dtype, void* c, void* work,
                         const phx_ch_params* p, int steps, int* best_zc,
                         double* best_ms_per_step, phx_stream stream);
+
+/* Benchmark timing (no reference counterpart): while on, each launch of the
+ * fused RK4 kernels is bracketed by HIP events on its stream. phx_ch_timing(1)
+ * also clears the record. phx_ch_timing_read sums the durations of the
+ * recorded launches of one kind (0: one-stage kernel, 1: two-stage (1,2),
+ * 2: two-stage (3,4)), waiting for them. One thread at a time. */
+int phx_ch_timing(int on);
+int phx_ch_timing_read(int kind, int* launches, double* total_ms);
 
 /* ------------------------------------------ FP64 diagnostics ---- */
 /* Workspace in bytes for phx_mass_energy. */

@@ -39,6 +39,8 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
+#include <vector>
 
 namespace phx {
 
@@ -414,6 +416,594 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
     }
 }
 
+// ------------------------------------------------- 3D, two stages fused --
+// Temporal blocking: one launch evaluates two consecutive RK4 stages, (1,2) or
+// (3,4), so a step costs 7 field passes over HBM instead of 16 (17 in FP64):
+//   (1,2): read c_n;                write acc and the stage-3 state;
+//   (3,4): read stage-3 state, c_n, acc;  write c_{n+1} (to a buffer other than
+//          c_n: the (3,4) launch reads c_n in a halo around its tile).
+// Per z plane the block computes, in LDS, five levels over shrinking regions
+// (TX+8)x(TY+8) -> ... -> TX x TY:
+//   UA = first stage's input state, MA = its mu, UB = second stage's input
+//   state (c_n + a1*k_first), MB = its mu, and the output cells.
+// The stage-internal ghost values are the same as in the one-stage kernels: a
+// level's value at a ghost position is its value at the BC image position
+// (mirror for NOFLUX with the +/- neighbours of the ghost axis exchanged,
+// wrap for PERIODIC) or the FIXED constant, decided z, then y, then x. The
+// image must lie in the domain for ghost depths up to 4, hence n >= 4 per axis.
+// Every value is formed by the operations, in the order, of the one-stage
+// kernels, so results match them bit for bit in FP32 and FP64.
+//
+// Pipeline, iteration o (output plane o), barriers B1..B4:
+//   P1 store UA(o+4) [regs loaded one iteration earlier]         | B1
+//   P2 MA(o+3) from UA(o+2..o+4)                                  | B2
+//   P3 UB(o+2) from MA(o+1..o+3); keep k and c of the tile cells  | B3
+//   P4 MB(o+1) from UB(o..o+2)                                    | B4
+//   P5 output plane o from MB(o-1..o+1)
+// Every level keeps 3 planes. The slot written in a phase held the plane
+// whose last reader ran before an earlier barrier: UA(o+1) last read in P3 of
+// iteration o-1, MA(o) in P3(o-1), UB(o-1) in P4(o-1), MB(o-2) and the tile
+// ring's plane o-1 in P5(o-1), which B1(o) separates from P3/P4(o).
+template <typename T>
+struct PairArgs {
+    const T* u;    // first stage's input state: c_n for (1,2), the stage-3 state for (3,4)
+    const T* c;    // c_n (read by (3,4))
+    T* acc;        // RK4 accumulator: written by (1,2), read by (3,4)
+    T* out;        // (1,2): stage-3 state; (3,4): c_{n+1}
+    Layout L;
+    AxisBC<T> bx, by, bz;
+    T idx2, idy2, idz2;
+    T two_rho, ca, cb, nkappa, M;
+    T a1, a2;      // stage shifts: (dt/2, dt/2) for (1,2); (dt, unused) for (3,4)
+    T dt6, dt3;
+    int zc;
+    int xcd_remap;
+};
+
+// BC image of position p for ghost depths up to D (needs n >= D); -1: FIXED.
+// Positions beyond depth D only occur in partial tiles and feed no output;
+// they are clamped to keep every load in bounds.
+template <typename T, int D = 4>
+__device__ __forceinline__ int map_deep(int p, const AxisBC<T>& b) {
+    p = p < -D ? -D : (p > b.n + D - 1 ? b.n + D - 1 : p);
+    if (p >= 0 && p < b.n) return p;
+    if (p < 0) {
+        if (b.klo == PHX_BC_PERIODIC) return p + b.n;
+        if (b.klo == PHX_BC_FIXED) return -1;
+        return -1 - p;
+    }
+    if (b.khi == PHX_BC_PERIODIC) return p - b.n;
+    if (b.khi == PHX_BC_FIXED) return -1;
+    return 2 * b.n - 1 - p;
+}
+
+// In-plane ghost rule of a position (y decides before x):
+// FIX -> the FIXED constant of axis FY ? y : x, side HI ? high : low;
+// SWX/SWY -> exchange the +/- neighbours of that axis (NOFLUX image).
+enum : int { GC_FIX = 1, GC_SWX = 2, GC_SWY = 4, GC_FY = 8, GC_HI = 16 };
+
+template <typename T>
+__device__ __forceinline__ int ghost_code_xy(int px, int py, const PairArgs<T>& A) {
+    int code = 0;
+    if (py < 0 || py >= A.L.n1) {
+        const int k = py < 0 ? A.by.klo : A.by.khi;
+        if (k == PHX_BC_FIXED) return GC_FIX | GC_FY | (py < 0 ? 0 : GC_HI);
+        if (k == PHX_BC_NOFLUX) code |= GC_SWY;
+    }
+    if (px < 0 || px >= A.L.n0) {
+        const int k = px < 0 ? A.bx.klo : A.bx.khi;
+        if (k == PHX_BC_FIXED) return GC_FIX | (px < 0 ? 0 : GC_HI);
+        if (k == PHX_BC_NOFLUX) code |= GC_SWX;
+    }
+    return code;
+}
+
+template <typename T>
+__device__ __forceinline__ T ghost_value_xy(int code, const PairArgs<T>& A) {
+    return (code & GC_FY) ? ((code & GC_HI) ? A.by.vhi : A.by.vlo) : ((code & GC_HI) ? A.bx.vhi : A.bx.vlo);
+}
+
+// (xp - 2c + xm)*idx2 + (yp - 2c + ym)*idy2 + (zp - 2c + zm)*idz2, in this order
+template <typename T>
+__device__ __forceinline__ T lap7(const PairArgs<T>& A, T c, T xp, T xm, T yp, T ym, T zp, T zm) {
+    T val = (xp - (T)2 * c + xm) * A.idx2;
+    val += (yp - (T)2 * c + ym) * A.idy2;
+    val += (zp - (T)2 * c + zm) * A.idz2;
+    return val;
+}
+
+// Seven-point star around LDS index i of a level with row width W, with the
+// neighbours of NOFLUX ghost axes exchanged.
+template <typename T, int W, bool GEN>
+__device__ __forceinline__ T star_lap(const PairArgs<T>& A, const T* lo, const T* mid, const T* hi, int i, int code,
+                                      bool zsw, T& c) {
+    c = mid[i];
+    T xp = mid[i + 1], xm = mid[i - 1], yp = mid[i + W], ym = mid[i - W], zp = hi[i], zm = lo[i];
+    if (GEN) {
+        if (code & GC_SWX) { const T t = xp; xp = xm; xm = t; }
+        if (code & GC_SWY) { const T t = yp; yp = ym; ym = t; }
+        if (zsw) { const T t = zp; zp = zm; zm = t; }
+    }
+    return lap7(A, c, xp, xm, yp, ym, zp, zm);
+}
+
+template <typename T, int TX, int TY, bool EXACT, bool SECOND>
+__global__ __launch_bounds__(256) void k_ch_pair_3d(PairArgs<T> A) {
+    constexpr int W4 = TX + 8, N4 = W4 * (TY + 8);
+    constexpr int W3 = TX + 6, N3 = W3 * (TY + 6);
+    constexpr int W2 = TX + 4, N2 = W2 * (TY + 4);
+    constexpr int W1 = TX + 2, N1 = W1 * (TY + 2);
+    constexpr int N0 = TX * TY;
+    constexpr int S4 = (N4 + 255) / 256, S3 = (N3 + 255) / 256, S2 = (N2 + 255) / 256, S1 = (N1 + 255) / 256;
+    constexpr int S0 = N0 / 256;
+    static_assert(N0 % 256 == 0, "tile must be a multiple of 256 cells");
+    constexpr bool NEED_CR = !SECOND || EXACT;   // c_n of the tile cells needed by the epilogue
+    __shared__ T UA[3][N4];
+    __shared__ T MA[3][N3];
+    __shared__ T UB[3][N2];
+    __shared__ T MB[3][N1];
+    __shared__ T KR[3][N0];
+    __shared__ T CR[NEED_CR ? 3 : 1][NEED_CR ? N0 : 1];
+    const int tid = threadIdx.x;
+    int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+    if (A.xcd_remap) {
+        const int gx = gridDim.x, gy = gridDim.y;
+        const int nb = gx * gy * gridDim.z;
+        const int h = bx + gx * (by + gy * bz);
+        const int lb = (h & 7) * (nb >> 3) + (h >> 3);
+        bx = lb % gx;
+        by = (lb / gx) % gy;
+        bz = lb / (gx * gy);
+    }
+    const int x0 = bx * TX, y0 = by * TY;
+    const int z0 = bz * A.zc;
+    const int z1 = min(z0 + A.zc, A.L.n2);
+    const int g = A.L.g;
+    const long long sxy = A.L.sxy;
+    const bool xy_in = x0 >= 4 && x0 + TX + 4 <= A.L.n0 && y0 >= 4 && y0 + TY + 4 <= A.L.n1;
+    auto rs = [](int p) { return (p + 12) % 3; };   // ring slot, planes >= -4
+
+    // UA halo slots: in-plane offset with the x/y images, or -1 for a FIXED constant
+    int hoff[S4];
+    T hfix[S4];
+#pragma unroll
+    for (int i = 0; i < S4; ++i) {
+        const int t = tid + 256 * i;
+        hoff[i] = 0;
+        hfix[i] = (T)0;
+        if (t < N4) {
+            const int ly = t / W4, lx = t - ly * W4;
+            const int px = x0 - 4 + lx, py = y0 - 4 + ly;
+            const int qy = map_deep(py, A.by), qx = map_deep(px, A.bx);
+            if (qy < 0) {
+                hoff[i] = -1;
+                hfix[i] = fixed_value(py, A.by);
+            } else if (qx < 0) {
+                hoff[i] = -1;
+                hfix[i] = fixed_value(px, A.bx);
+            } else {
+                hoff[i] = (qy + g) * A.L.sx + (qx + g);
+            }
+        }
+    }
+    // in-plane ghost codes of the level slots (all zero for interior tiles)
+    int code3[S3], code2[S2], code1[S1];
+#pragma unroll
+    for (int i = 0; i < S3; ++i) {
+        const int t = tid + 256 * i, ly = t / W3, lx = t - ly * W3;
+        code3[i] = xy_in ? 0 : ghost_code_xy(x0 - 3 + lx, y0 - 3 + ly, A);
+    }
+#pragma unroll
+    for (int i = 0; i < S2; ++i) {
+        const int t = tid + 256 * i, ly = t / W2, lx = t - ly * W2;
+        code2[i] = xy_in ? 0 : ghost_code_xy(x0 - 2 + lx, y0 - 2 + ly, A);
+    }
+#pragma unroll
+    for (int i = 0; i < S1; ++i) {
+        const int t = tid + 256 * i, ly = t / W1, lx = t - ly * W1;
+        code1[i] = xy_in ? 0 : ghost_code_xy(x0 - 1 + lx, y0 - 1 + ly, A);
+    }
+    // c_n at the UB slots ((3,4) only): in-plane offsets with the images
+    int coff[SECOND ? S2 : 1];
+    T creg[SECOND ? S2 : 1];
+    if (SECOND) {
+#pragma unroll
+        for (int i = 0; i < S2; ++i) {
+            const int t = tid + 256 * i, ly = t / W2, lx = t - ly * W2;
+            const int qy = map_deep(y0 - 2 + ly, A.by), qx = map_deep(x0 - 2 + lx, A.bx);
+            coff[i] = (t < N2 && qy >= 0 && qx >= 0) ? (qy + g) * A.L.sx + (qx + g) : -1;
+            creg[i] = (T)0;
+        }
+    }
+    // output slots
+    int ooff[S0];
+    bool oval[S0];
+    T areg[SECOND ? S0 : 1];
+#pragma unroll
+    for (int j = 0; j < S0; ++j) {
+        const int t = tid + 256 * j;
+        const int oy = t / TX, ox = t - oy * TX;
+        const int x = x0 + ox, y = y0 + oy;
+        oval[j] = x < A.L.n0 && y < A.L.n1;
+        ooff[j] = (y + g) * A.L.sx + (x + g);
+    }
+
+    auto load_ua = [&](int pz, T (&r)[S4]) {
+        const int qz = map_deep(pz, A.bz);
+        if (qz < 0) {
+            const T v = fixed_value(pz, A.bz);
+#pragma unroll
+            for (int i = 0; i < S4; ++i) r[i] = v;
+        } else {
+            const T* base = A.u + (long long)(qz + g) * sxy;
+#pragma unroll
+            for (int i = 0; i < S4; ++i)
+                if (tid + 256 * i < N4) r[i] = hoff[i] >= 0 ? base[hoff[i]] : hfix[i];
+        }
+    };
+    auto store_ua = [&](int pz, const T (&r)[S4]) {
+        T* dst = UA[rs(pz)];
+#pragma unroll
+        for (int i = 0; i < S4; ++i)
+            if (tid + 256 * i < N4) dst[tid + 256 * i] = r[i];
+    };
+    auto load_c = [&](int pz) {
+        const int qz = map_deep(pz, A.bz);
+        if (qz < 0) return;   // FIXED plane: UB is the constant there
+        const T* base = A.c + (long long)(qz + g) * sxy;
+#pragma unroll
+        for (int i = 0; i < (SECOND ? S2 : 1); ++i)
+            if (coff[i] >= 0) creg[i] = base[coff[i]];
+    };
+    auto load_acc = [&](int o) {
+        const long long pb = (long long)(o + g) * sxy;
+#pragma unroll
+        for (int j = 0; j < (SECOND ? S0 : 1); ++j)
+            if (oval[j]) areg[j] = A.acc[pb + ooff[j]];
+    };
+    // ghost rule of plane pz along z (decides before y and x)
+    auto zrule = [&](int pz, bool& zfix, bool& zsw, T& zv) {
+        zfix = false;
+        zsw = false;
+        zv = (T)0;
+        if (pz < 0 || pz >= A.L.n2) {
+            const int k = pz < 0 ? A.bz.klo : A.bz.khi;
+            if (k == PHX_BC_FIXED) {
+                zfix = true;
+                zv = fixed_value(pz, A.bz);
+            } else if (k == PHX_BC_NOFLUX) {
+                zsw = true;
+            }
+        }
+    };
+
+    // P2: mu of the first stage on plane m
+    auto eval_ma = [&](int m) {
+        const T* lo = UA[rs(m - 1)];
+        const T* mid = UA[rs(m)];
+        const T* hi = UA[rs(m + 1)];
+        T* dst = MA[rs(m)];
+        bool zfix, zsw;
+        T zv;
+        zrule(m, zfix, zsw, zv);
+        auto body = [&](auto gen) {
+            constexpr bool GEN = decltype(gen)::value;
+#pragma unroll
+            for (int i = 0; i < S3; ++i) {
+                const int t = tid + 256 * i;
+                if (t < N3) {
+                    const int ly = t / W3, lx = t - ly * W3;
+                    const int ci = (ly + 1) * W4 + lx + 1;
+                    T v;
+                    if (GEN && zfix) v = zv;
+                    else if (GEN && (code3[i] & GC_FIX)) v = ghost_value_xy(code3[i], A);
+                    else {
+                        T c;
+                        const T l = star_lap<T, W4, GEN>(A, lo, mid, hi, ci, code3[i], zsw, c);
+                        v = dwell(c, A.two_rho, A.ca, A.cb) + A.nkappa * l;
+                    }
+                    dst[t] = v;
+                }
+            }
+        };
+        if (xy_in && !zfix && !zsw) body(std::false_type{});
+        else body(std::true_type{});
+    };
+    // P3: second stage's input state on plane b, and k / c_n of the tile cells
+    auto eval_ub = [&](int b) {
+        const T* lo = MA[rs(b - 1)];
+        const T* mid = MA[rs(b)];
+        const T* hi = MA[rs(b + 1)];
+        const T* uc = UA[rs(b)];
+        T* dst = UB[rs(b)];
+        T* kr = KR[rs(b)];
+        T* cr = CR[NEED_CR ? rs(b) : 0];
+        bool zfix, zsw;
+        T zv;
+        zrule(b, zfix, zsw, zv);
+        auto body = [&](auto gen) {
+            constexpr bool GEN = decltype(gen)::value;
+#pragma unroll
+            for (int i = 0; i < S2; ++i) {
+                const int t = tid + 256 * i;
+                if (t < N2) {
+                    const int ly = t / W2, lx = t - ly * W2;
+                    const int mi = (ly + 1) * W3 + lx + 1;
+                    T cc;
+                    if constexpr (SECOND) cc = creg[i];
+                    else cc = uc[(ly + 2) * W4 + lx + 2];
+                    T v, k = (T)0;
+                    if (GEN && zfix) v = zv;
+                    else if (GEN && (code2[i] & GC_FIX)) v = ghost_value_xy(code2[i], A);
+                    else {
+                        T m;
+                        k = A.M * star_lap<T, W3, GEN>(A, lo, mid, hi, mi, code2[i], zsw, m);
+                        v = cc + A.a1 * k;
+                    }
+                    dst[t] = v;
+                    const int ox = lx - 2, oy = ly - 2;
+                    if (ox >= 0 && ox < TX && oy >= 0 && oy < TY) {
+                        // stage 1 (FP32 form) keeps c + dt/6*k1, the others k
+                        kr[oy * TX + ox] = (!SECOND && !EXACT) ? cc + A.dt6 * k : k;
+                        if constexpr (NEED_CR) cr[oy * TX + ox] = cc;
+                    }
+                }
+            }
+        };
+        if (xy_in && !zfix && !zsw) body(std::false_type{});
+        else body(std::true_type{});
+    };
+    // P4: mu of the second stage on plane m
+    auto eval_mb = [&](int m) {
+        const T* lo = UB[rs(m - 1)];
+        const T* mid = UB[rs(m)];
+        const T* hi = UB[rs(m + 1)];
+        T* dst = MB[rs(m)];
+        bool zfix, zsw;
+        T zv;
+        zrule(m, zfix, zsw, zv);
+        auto body = [&](auto gen) {
+            constexpr bool GEN = decltype(gen)::value;
+#pragma unroll
+            for (int i = 0; i < S1; ++i) {
+                const int t = tid + 256 * i;
+                if (t < N1) {
+                    const int ly = t / W1, lx = t - ly * W1;
+                    const int ci = (ly + 1) * W2 + lx + 1;
+                    T v;
+                    if (GEN && zfix) v = zv;
+                    else if (GEN && (code1[i] & GC_FIX)) v = ghost_value_xy(code1[i], A);
+                    else {
+                        T c;
+                        const T l = star_lap<T, W2, GEN>(A, lo, mid, hi, ci, code1[i], zsw, c);
+                        v = dwell(c, A.two_rho, A.ca, A.cb) + A.nkappa * l;
+                    }
+                    dst[t] = v;
+                }
+            }
+        };
+        if (xy_in && !zfix && !zsw) body(std::false_type{});
+        else body(std::true_type{});
+    };
+    // P5: second stage's k on output plane o and the stage epilogues
+    auto out_plane = [&](int o) {
+        const T* lo = MB[rs(o - 1)];
+        const T* mid = MB[rs(o)];
+        const T* hi = MB[rs(o + 1)];
+        const T* kr = KR[rs(o)];
+        const T* cr = CR[NEED_CR ? rs(o) : 0];
+        const long long pb = (long long)(o + g) * sxy;
+#pragma unroll
+        for (int j = 0; j < S0; ++j) {
+            if (!oval[j]) continue;
+            const int t = tid + 256 * j;
+            const int oy = t / TX, ox = t - oy * TX;
+            const int mi = (oy + 1) * W1 + ox + 1;
+            T m;
+            const T k = A.M * star_lap<T, W1, false>(A, lo, mid, hi, mi, 0, false, m);
+            const long long gi = pb + ooff[j];
+            if constexpr (!SECOND) {
+                // stage 2: acc = k1 + 2*k2 (FP64) or (c + dt/6*k1) + dt/3*k2; state c + dt/2*k2
+                if (EXACT) A.acc[gi] = kr[t] + (T)2 * k;
+                else A.acc[gi] = kr[t] + A.dt3 * k;
+                A.out[gi] = cr[t] + A.a2 * k;
+            } else {
+                // stage 3 accumulates k3, stage 4 finishes the step
+                if (EXACT) {
+                    const T ac = areg[j] + (T)2 * kr[t];
+                    A.out[gi] = cr[t] + A.dt6 * (ac + k);
+                } else {
+                    const T ac = areg[j] + A.dt3 * kr[t];
+                    A.out[gi] = ac + A.dt6 * k;
+                }
+            }
+        }
+    };
+
+    // prologue: UA planes z0-4, z0-3 stored, z0-2 in flight
+    T pre[S4];
+    load_ua(z0 - 4, pre);
+    store_ua(z0 - 4, pre);
+    load_ua(z0 - 3, pre);
+    store_ua(z0 - 3, pre);
+    load_ua(z0 - 2, pre);
+    if (SECOND) {
+        load_c(z0 - 2);
+        load_acc(z0);
+    }
+    for (int o = z0 - 6; o < z1; ++o) {
+        store_ua(o + 4, pre);
+        __syncthreads();   // B1
+        if (o + 5 <= z1 + 3) load_ua(o + 5, pre);
+        if (o + 3 >= z0 - 3) eval_ma(o + 3);
+        __syncthreads();   // B2
+        if (o + 2 >= z0 - 2) {
+            eval_ub(o + 2);
+            if (SECOND && o + 3 <= z1 + 1) load_c(o + 3);
+        }
+        __syncthreads();   // B3
+        if (o + 1 >= z0 - 1) eval_mb(o + 1);
+        __syncthreads();   // B4
+        if (o >= z0) {
+            out_plane(o);
+            if (SECOND && o + 1 < z1) load_acc(o + 1);
+        }
+    }
+}
+
+// ------------------------------------------------ 2D, whole step fused --
+// All four RK4 stages of a 2D step in one launch: the block loads c_n on its
+// tile plus an 8-cell ring (HBM -> LDS once) and evaluates, in LDS, the eight
+// levels mu1 (ring 7), state 2 (ring 6), mu2 (5), state 3 (4), mu3 (3),
+// state 4 (2), mu4 (1) and the output; the RK4 accumulator of the tile cells
+// stays in LDS. A step costs 2 field passes over HBM (read c_n, write c_{n+1})
+// and one launch instead of 16 passes and four launches. c_{n+1} goes to a
+// buffer other than c_n (blocks read c_n in a ring around their tile). The
+// ghost rules and operation order are those of the one-stage kernels (see the
+// two-stage 3D kernel above), for ghost depths up to 8, hence n >= 8 per axis;
+// results equal the one-stage kernels bit for bit.
+//
+// LDS: SC = c_n (ring 8, kept for the stage-state centres), SA = mu levels,
+// SB = stage states 2..4, SACC = accumulator. A level reads the other buffer
+// and is separated by a barrier from the level that last read the buffer it
+// writes.
+template <typename T>
+__device__ __forceinline__ T lap5(const PairArgs<T>& A, T c, T xp, T xm, T yp, T ym) {
+    T val = (xp - (T)2 * c + xm) * A.idx2;
+    val += (yp - (T)2 * c + ym) * A.idy2;
+    return val;
+}
+
+template <typename T, int TX, int TY, bool EXACT>
+__global__ __launch_bounds__(256) void k_ch_step_2d(PairArgs<T> A) {
+    constexpr int WC = TX + 16, NC = WC * (TY + 16);
+    constexpr int NA = (TX + 14) * (TY + 14);
+    constexpr int NB = (TX + 12) * (TY + 12);
+    constexpr int N0 = TX * TY;
+    __shared__ T SC[NC];
+    __shared__ T SA[NA];
+    __shared__ T SB[NB];
+    __shared__ T SACC[N0];
+    const int tid = threadIdx.x;
+    int bx = blockIdx.x, by = blockIdx.y;
+    if (A.xcd_remap) {
+        const int gx = gridDim.x, nb = gx * gridDim.y;
+        const int h = bx + gx * by;
+        const int lb = (h & 7) * (nb >> 3) + (h >> 3);
+        bx = lb % gx;
+        by = lb / gx;
+    }
+    const int x0 = bx * TX, y0 = by * TY;
+    const int g = A.L.g;
+    const long long zb = (long long)g * A.L.sxy;
+    const bool xy_in = x0 >= 8 && x0 + TX + 8 <= A.L.n0 && y0 >= 8 && y0 + TY + 8 <= A.L.n1;
+
+    // c_n on the tile and an 8-cell ring, x/y BC images applied
+    if (xy_in) {
+        const T* base = A.u + zb + (long long)(y0 - 8 + g) * A.L.sx + (x0 - 8 + g);
+        for (int t = tid; t < NC; t += 256) {
+            const int ly = t / WC, lx = t - ly * WC;
+            SC[t] = base[(long long)ly * A.L.sx + lx];
+        }
+    } else {
+        for (int t = tid; t < NC; t += 256) {
+            const int ly = t / WC, lx = t - ly * WC;
+            const int px = x0 - 8 + lx, py = y0 - 8 + ly;
+            const int qy = map_deep<T, 8>(py, A.by), qx = map_deep<T, 8>(px, A.bx);
+            T v;
+            if (qy < 0) v = fixed_value(py, A.by);
+            else if (qx < 0) v = fixed_value(px, A.bx);
+            else v = A.u[zb + (long long)(qy + g) * A.L.sx + (qx + g)];
+            SC[t] = v;
+        }
+    }
+    __syncthreads();
+
+    // mu on ring R from the state level on ring R+1 (row width TX+2R+2)
+    auto eval_mu = [&](auto rr, const T* src) {
+        constexpr int R = decltype(rr)::value;
+        constexpr int WS = TX + 2 * R + 2, WD = TX + 2 * R, ND = WD * (TY + 2 * R);
+        for (int t = tid; t < ND; t += 256) {
+            const int ly = t / WD, lx = t - ly * WD;
+            const int si = (ly + 1) * WS + lx + 1;
+            const int code = xy_in ? 0 : ghost_code_xy(x0 - R + lx, y0 - R + ly, A);
+            T v;
+            if (code & GC_FIX) {
+                v = ghost_value_xy(code, A);
+            } else {
+                const T c = src[si];
+                T xp = src[si + 1], xm = src[si - 1], yp = src[si + WS], ym = src[si - WS];
+                if (code & GC_SWX) { const T q = xp; xp = xm; xm = q; }
+                if (code & GC_SWY) { const T q = yp; yp = ym; ym = q; }
+                v = dwell(c, A.two_rho, A.ca, A.cb) + A.nkappa * lap5(A, c, xp, xm, yp, ym);
+            }
+            SA[t] = v;
+        }
+    };
+    // stage state on ring R from the mu level on ring R+1: c_n + a*k, and the
+    // accumulator update of stage S on the tile cells
+    auto eval_state = [&](auto rr, auto ss, T a) {
+        constexpr int R = decltype(rr)::value;
+        constexpr int S = decltype(ss)::value;
+        constexpr int WS = TX + 2 * R + 2, WD = TX + 2 * R, ND = WD * (TY + 2 * R);
+        for (int t = tid; t < ND; t += 256) {
+            const int ly = t / WD, lx = t - ly * WD;
+            const int si = (ly + 1) * WS + lx + 1;
+            const int code = xy_in ? 0 : ghost_code_xy(x0 - R + lx, y0 - R + ly, A);
+            const T cc = SC[(ly + 8 - R) * WC + lx + 8 - R];
+            T v, k = (T)0;
+            if (code & GC_FIX) {
+                v = ghost_value_xy(code, A);
+            } else {
+                const T m = SA[si];
+                T xp = SA[si + 1], xm = SA[si - 1], yp = SA[si + WS], ym = SA[si - WS];
+                if (code & GC_SWX) { const T q = xp; xp = xm; xm = q; }
+                if (code & GC_SWY) { const T q = yp; yp = ym; ym = q; }
+                k = A.M * lap5(A, m, xp, xm, yp, ym);
+                v = cc + a * k;
+            }
+            SB[t] = v;
+            const int ox = lx - R, oy = ly - R;
+            if (ox >= 0 && ox < TX && oy >= 0 && oy < TY) {
+                T& ac = SACC[oy * TX + ox];
+                if (S == 1) ac = EXACT ? k : cc + A.dt6 * k;
+                else ac = EXACT ? ac + (T)2 * k : ac + A.dt3 * k;
+            }
+        }
+    };
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using I3 = std::integral_constant<int, 3>;
+    const T half = A.a1, full = A.a2;
+    eval_mu(std::integral_constant<int, 7>{}, SC);
+    __syncthreads();
+    eval_state(std::integral_constant<int, 6>{}, I1{}, half);
+    __syncthreads();
+    eval_mu(std::integral_constant<int, 5>{}, SB);
+    __syncthreads();
+    eval_state(std::integral_constant<int, 4>{}, I2{}, half);
+    __syncthreads();
+    eval_mu(I3{}, SB);
+    __syncthreads();
+    eval_state(I2{}, I3{}, full);
+    __syncthreads();
+    eval_mu(I1{}, SB);
+    __syncthreads();
+    // stage 4 on the tile cells
+    constexpr int W1 = TX + 2;
+    for (int t = tid; t < N0; t += 256) {
+        const int oy = t / TX, ox = t - oy * TX;
+        const int x = x0 + ox, y = y0 + oy;
+        if (x >= A.L.n0 || y >= A.L.n1) continue;
+        const int si = (oy + 1) * W1 + ox + 1;
+        const T m = SA[si];
+        const T k = A.M * lap5(A, m, SA[si + 1], SA[si - 1], SA[si + W1], SA[si - W1]);
+        const long long gi = zb + (long long)(y + g) * A.L.sx + (x + g);
+        if (EXACT) A.out[gi] = SC[(oy + 8) * WC + ox + 8] + A.dt6 * (SACC[t] + k);
+        else A.out[gi] = SACC[t] + A.dt6 * k;
+    }
+}
+
 // ---------------------------------------------------------- Euler pieces --
 // Multi-step Euler for the CH app (Solver.cu:317-336):
 //   K_mu : mu on every stored cell of the active region (physical: from c with
@@ -577,11 +1167,83 @@ static void launch3(int tile, dim3 grid, const CHArgs<T>& A, hipStream_t st) {
     }
 }
 
+// Two-stage kernel tiles; launch tile indices kNumTiles3.. select them.
+static const Tile3 kTilesP[] = {{64, 8}, {32, 16}, {32, 8}, {64, 4}};
+constexpr int kNumTilesP = sizeof(kTilesP) / sizeof(kTilesP[0]);
+
+template <typename T, bool EXACT, bool SECOND>
+static void launch_pair(int tile, dim3 grid, const PairArgs<T>& P, hipStream_t st) {
+    switch (tile) {
+        case 0: hipLaunchKernelGGL((k_ch_pair_3d<T, 64, 8, EXACT, SECOND>), grid, dim3(256), 0, st, P); break;
+        case 1: hipLaunchKernelGGL((k_ch_pair_3d<T, 32, 16, EXACT, SECOND>), grid, dim3(256), 0, st, P); break;
+        case 2: hipLaunchKernelGGL((k_ch_pair_3d<T, 32, 8, EXACT, SECOND>), grid, dim3(256), 0, st, P); break;
+        default: hipLaunchKernelGGL((k_ch_pair_3d<T, 64, 4, EXACT, SECOND>), grid, dim3(256), 0, st, P); break;
+    }
+}
+
+// The two-stage kernel needs every axis of a 3D grid to hold its depth-4 BC
+// images inside the domain.
+static bool pair_ok(const phx_grid* G) { return G->dim == 3 && G->n[0] >= 4 && G->n[1] >= 4 && G->n[2] >= 4; }
+
+// 2D whole-step kernel tiles; launch tile indices 1.. select them in 2D
+// (0 is the one-stage kernel).
+static const Tile3 kTilesS2[] = {{64, 16}, {64, 32}, {128, 16}, {32, 32}};
+constexpr int kNumTilesS2 = sizeof(kTilesS2) / sizeof(kTilesS2[0]);
+
+template <typename T, bool EXACT>
+static void launch_step2d(int tile, dim3 grid, const PairArgs<T>& P, hipStream_t st) {
+    switch (tile) {
+        case 0: hipLaunchKernelGGL((k_ch_step_2d<T, 64, 16, EXACT>), grid, dim3(256), 0, st, P); break;
+        case 1: hipLaunchKernelGGL((k_ch_step_2d<T, 64, 32, EXACT>), grid, dim3(256), 0, st, P); break;
+        case 2: hipLaunchKernelGGL((k_ch_step_2d<T, 128, 16, EXACT>), grid, dim3(256), 0, st, P); break;
+        default: hipLaunchKernelGGL((k_ch_step_2d<T, 32, 32, EXACT>), grid, dim3(256), 0, st, P); break;
+    }
+}
+
+// The 2D whole-step kernel needs depth-8 BC images inside the domain.
+static bool step2d_ok(const phx_grid* G) { return G->dim == 2 && G->n[0] >= 8 && G->n[1] >= 8; }
+
+// Optional per-launch timing for benchmarks: while enabled, every launch of
+// the fused RK4 kernels is bracketed by HIP events on its stream, tagged with
+// its kind (0: one-stage kernel, 1: two-stage (1,2), 2: two-stage (3,4),
+// 3: 2D whole step).
+// Not thread safe; meant for one benchmarking thread.
+struct LaunchTimer {
+    int on = 0;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> kind;
+    size_t used = 0;
+};
+static LaunchTimer g_lt;
+
+static void lt_begin(hipStream_t st) {
+    if (!g_lt.on) return;
+    if (2 * g_lt.used + 2 > g_lt.ev.size()) {
+        for (int i = 0; i < 2; ++i) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) {
+                g_lt.on = 0;
+                return;
+            }
+            g_lt.ev.push_back(e);
+        }
+        g_lt.kind.push_back(0);
+    }
+    (void)hipEventRecord(g_lt.ev[2 * g_lt.used], st);
+}
+
+static void lt_end(int kind, hipStream_t st) {
+    if (!g_lt.on) return;
+    (void)hipEventRecord(g_lt.ev[2 * g_lt.used + 1], st);
+    g_lt.kind[g_lt.used] = kind;
+    ++g_lt.used;
+}
+
 static int g_xcd_remap = 1;
 
 template <typename T, bool EXACT>
 static int rk4_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_params* p, int nsteps, hipStream_t st,
-                       int zc, int tile) {
+                       int zc, int tile, int skip_fill = 0) {
     CHArgs<T> A;
     A.L = make_layout(G);
     const long long N = A.L.total();
@@ -609,7 +1271,7 @@ static int rk4_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_params* p,
     }
     else grid = dim3(cdiv(A.L.n0, T2X), cdiv(A.L.n1, T2Y), 1);
     for (int s = 0; s < nsteps; ++s) {
-        if (s == nsteps - 1) {
+        if (s == nsteps - 1 && !skip_fill) {
             int rc = bc_fill_impl(G, sizeof(T) == 8 ? PHX_F64 : PHX_F32, c, st);
             if (rc) return rc;
         }
@@ -618,13 +1280,181 @@ static int rk4_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_params* p,
             A.u = uin[stg];
             A.out = uout[stg];
             A.a = shift[stg];
+            lt_begin(st);
             if (G->dim == 3) launch3<T, EXACT>(tile, grid, A, st);
             else if (G->dim == 2) hipLaunchKernelGGL((k_ch_stage_2d<T, 2, T2X, T2Y, EXACT>), grid, dim3(256), 0, st, A);
             else hipLaunchKernelGGL((k_ch_stage_2d<T, 1, T2X, T2Y, EXACT>), grid, dim3(256), 0, st, A);
+            lt_end(0, st);
         }
     }
     PHX_LAUNCH_CHECK("phx_ch_rk4_steps");
     return PHX_OK;
+}
+
+// RK4 steps with the two-stage kernel (3D, tile index ptile into kTilesP).
+// The (3,4) launch cannot update c_n in place, so the state alternates between
+// c and the work buffer w1; the last step runs (1,2) and then stages 3 and 4
+// with the one-stage kernel, which ends in c. When the steps before the last
+// are odd in number, the first step runs with the one-stage kernel so that the
+// alternation ends in c. As in the one-stage path the ghost layer of c is the
+// BC fill of the state before the last step.
+template <typename T, bool EXACT>
+static int rk4_pair_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_params* p, int nsteps, hipStream_t st,
+                            int zc, int ptile, int pure_pairs) {
+    const Layout L = make_layout(G);
+    const long long N = L.total();
+    T* w0 = work;
+    T* w1 = work + N;
+    T* acc = work + 2 * N;
+    PairArgs<T> P;
+    P.L = L;
+    P.bx = axis_bc<T>(G, 0);
+    P.by = axis_bc<T>(G, 1);
+    P.bz = axis_bc<T>(G, 2);
+    fill_common<T>(G, p, P.idx2, P.idy2, P.idz2, P.two_rho, P.ca, P.cb, P.nkappa, P.M);
+    const double dt = p->dt, dt6 = dt / 6.0;
+    P.dt6 = (T)dt6;
+    P.dt3 = (T)(2.0 * dt6);
+    P.acc = acc;
+    P.zc = zc;
+    const Tile3 tl = kTilesP[ptile];
+    const dim3 grid(cdiv(L.n0, tl.tx), cdiv(L.n1, tl.ty), cdiv(L.n2, zc));
+    P.xcd_remap = g_xcd_remap && (grid.x * grid.y * grid.z) % 8 == 0;
+    auto pair12 = [&](const T* x) {
+        P.u = x;
+        P.c = x;
+        P.out = w0;
+        P.a1 = (T)(dt * 0.5);
+        P.a2 = (T)(dt * 0.5);
+        lt_begin(st);
+        launch_pair<T, EXACT, false>(ptile, grid, P, st);
+        lt_end(1, st);
+    };
+    auto pair34 = [&](const T* x, T* y) {
+        P.u = w0;
+        P.c = x;
+        P.out = y;
+        P.a1 = (T)dt;
+        P.a2 = (T)0;
+        lt_begin(st);
+        launch_pair<T, EXACT, true>(ptile, grid, P, st);
+        lt_end(2, st);
+    };
+    if (pure_pairs) {
+        // timing helper: nsteps (even) two-stage steps, state ends in c
+        T* x = c;
+        T* y = w1;
+        for (int s = 0; s < nsteps; ++s) {
+            pair12(x);
+            pair34(x, y);
+            std::swap(x, y);
+        }
+        PHX_LAUNCH_CHECK("phx_ch_rk4_steps");
+        return PHX_OK;
+    }
+    int rest = nsteps - 1;
+    if (rest % 2) {
+        // one one-stage step, in place
+        int rc = rk4_steps_t<T, EXACT>(G, c, work, p, 1, st, 16, sizeof(T) == 8 ? 1 : 0, /*skip_fill=*/1);
+        if (rc) return rc;
+        --rest;
+    }
+    T* x = c;
+    T* y = w1;
+    for (int s = 0; s < rest; ++s) {
+        pair12(x);
+        pair34(x, y);
+        std::swap(x, y);
+    }
+    int rc = bc_fill_impl(G, sizeof(T) == 8 ? PHX_F64 : PHX_F32, c, st);
+    if (rc) return rc;
+    pair12(c);
+    // stages 3 and 4 with the one-stage kernel: w0 -> w1 -> c
+    CHArgs<T> A;
+    A.L = L;
+    A.bx = P.bx;
+    A.by = P.by;
+    A.bz = P.bz;
+    fill_common<T>(G, p, A.idx2, A.idy2, A.idz2, A.two_rho, A.ca, A.cb, A.nkappa, A.M);
+    A.dt6 = P.dt6;
+    A.dt3 = P.dt3;
+    A.c = c;
+    A.acc = acc;
+    A.zc = 16;
+    const int stile = sizeof(T) == 8 ? 1 : 0;
+    const dim3 sgrid(cdiv(L.n0, kTiles3[stile].tx), cdiv(L.n1, kTiles3[stile].ty), cdiv(L.n2, 16));
+    A.xcd_remap = g_xcd_remap && (sgrid.x * sgrid.y * sgrid.z) % 8 == 0;
+    A.stage = 3;
+    A.u = w0;
+    A.out = w1;
+    A.a = (T)dt;
+    lt_begin(st);
+    launch3<T, EXACT>(stile, sgrid, A, st);
+    lt_end(0, st);
+    A.stage = 4;
+    A.u = w1;
+    A.out = c;
+    A.a = (T)0;
+    lt_begin(st);
+    launch3<T, EXACT>(stile, sgrid, A, st);
+    lt_end(0, st);
+    PHX_LAUNCH_CHECK("phx_ch_rk4_steps");
+    return PHX_OK;
+}
+
+// RK4 steps with the 2D whole-step kernel (tile index stile into kTilesS2).
+// Same schedule as the two-stage 3D path: the state alternates between c and
+// the work buffer w1, an odd count of steps before the last starts with one
+// one-stage step, and the last step runs one-stage in place (BC fill of c
+// first), so c ends with the state and the ghost layer the one-stage path
+// leaves.
+template <typename T, bool EXACT>
+static int rk4_step2d_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_params* p, int nsteps, hipStream_t st,
+                              int stile, int pure) {
+    const Layout L = make_layout(G);
+    const long long N = L.total();
+    T* w1 = work + N;
+    PairArgs<T> P;
+    P.L = L;
+    P.bx = axis_bc<T>(G, 0);
+    P.by = axis_bc<T>(G, 1);
+    P.bz = axis_bc<T>(G, 2);
+    fill_common<T>(G, p, P.idx2, P.idy2, P.idz2, P.two_rho, P.ca, P.cb, P.nkappa, P.M);
+    const double dt = p->dt, dt6 = dt / 6.0;
+    P.dt6 = (T)dt6;
+    P.dt3 = (T)(2.0 * dt6);
+    P.a1 = (T)(dt * 0.5);
+    P.a2 = (T)dt;
+    P.c = nullptr;
+    P.acc = nullptr;
+    P.zc = 1;
+    const Tile3 tl = kTilesS2[stile];
+    const dim3 grid(cdiv(L.n0, tl.tx), cdiv(L.n1, tl.ty), 1);
+    P.xcd_remap = g_xcd_remap && (grid.x * grid.y) % 8 == 0;
+    auto step = [&](const T* x, T* y) {
+        P.u = x;
+        P.out = y;
+        lt_begin(st);
+        launch_step2d<T, EXACT>(stile, grid, P, st);
+        lt_end(3, st);
+    };
+    int rest = pure ? nsteps : nsteps - 1;
+    if (!pure && rest % 2) {
+        int rc = rk4_steps_t<T, EXACT>(G, c, work, p, 1, st, 1, 0, /*skip_fill=*/1);
+        if (rc) return rc;
+        --rest;
+    }
+    T* x = c;
+    T* y = w1;
+    for (int s = 0; s < rest; ++s) {
+        step(x, y);
+        std::swap(x, y);
+    }
+    if (pure) {
+        PHX_LAUNCH_CHECK("phx_ch_rk4_steps");
+        return PHX_OK;
+    }
+    return rk4_steps_t<T, EXACT>(G, c, work, p, 1, st, 1, 0);
 }
 
 static int check_ch_params(const phx_ch_params* p, const char* who) {
@@ -660,24 +1490,27 @@ size_t phx_ch_rk4_workspace_bytes(const phx_grid* G, int dtype) {
     return 3 * (size_t)make_layout(G).total() * (dtype == PHX_F64 ? 8 : 4);
 }
 
-// z planes per block of the 3D stage kernel: tuned per (shape, dtype) by
-// phx_ch_rk4_autotune, else 16.
+// Launch of the RK4 kernels per (dim, shape, dtype): tile index (3D: one-stage
+// 0-4, two-stage 5-8; 2D: one-stage 0, whole-step 1-4) and, in 3D, the z
+// planes streamed per workgroup. Set by phx_ch_rk4_set_tile/_set_zchunk or the
+// autotuner; the defaults are the one-stage kernels.
 struct Launch3 {
     int zc, tile;
 };
 static std::mutex g_zc_mu;
-static std::map<std::tuple<int, int, int, int>, Launch3> g_zc;
+static std::map<std::tuple<int, int, int, int, int>, Launch3> g_zc;
 
 static Launch3 launch_for(const phx_grid* G, int dtype) {
     std::lock_guard<std::mutex> lk(g_zc_mu);
-    auto it = g_zc.find(std::make_tuple(G->n[0], G->n[1], G->n[2], dtype));
+    auto it = g_zc.find(std::make_tuple(G->dim, G->n[0], G->n[1], G->n[2], dtype));
     if (it != g_zc.end()) return it->second;
+    if (G->dim != 3) return Launch3{1, 0};
     return Launch3{16, dtype == PHX_F32 ? 0 : 1};
 }
 
 static void set_launch(const phx_grid* G, int dtype, Launch3 l) {
     std::lock_guard<std::mutex> lk(g_zc_mu);
-    g_zc[std::make_tuple(G->n[0], G->n[1], G->n[2], dtype)] = l;
+    g_zc[std::make_tuple(G->dim, G->n[0], G->n[1], G->n[2], dtype)] = l;
 }
 
 int phx_ch_set_xcd_remap(int on) {
@@ -707,12 +1540,43 @@ int phx_ch_rk4_get_launch(const phx_grid* G, int dtype, int* zc, int* tile) {
 int phx_ch_rk4_set_tile(const phx_grid* G, int dtype, int tile) {
     int rc = check_grid(G, "phx_ch_rk4_set_tile");
     if (rc) return rc;
-    if (tile < 0 || tile >= kNumTiles3) return fail(PHX_ERR_INVALID, "phx_ch_rk4_set_tile: tile index out of range");
+    const int ntiles = G->dim == 3 ? kNumTiles3 + kNumTilesP : (G->dim == 2 ? 1 + kNumTilesS2 : 1);
+    if (tile < 0 || tile >= ntiles) return fail(PHX_ERR_INVALID, "phx_ch_rk4_set_tile: tile index out of range");
+    if (G->dim == 3 && tile >= kNumTiles3 && !pair_ok(G))
+        return fail(PHX_ERR_UNSUPPORTED, "phx_ch_rk4_set_tile: two-stage tiles need n >= 4 on every axis");
+    if (G->dim == 2 && tile >= 1 && !step2d_ok(G))
+        return fail(PHX_ERR_UNSUPPORTED, "phx_ch_rk4_set_tile: whole-step 2D tiles need n >= 8 on both axes");
     Launch3 l = launch_for(G, dtype);
     l.tile = tile;
     set_launch(G, dtype, l);
     return PHX_OK;
 }
+
+}  // extern "C"
+
+// Schedule by the launch's tile index; `pure` runs nsteps (even) two-stage or
+// whole-step steps only (autotuner timing).
+static int run_rk4(const phx_grid* G, int dtype, void* c, void* work, const phx_ch_params* p, int nsteps,
+                   hipStream_t st, Launch3 l, int pure) {
+    if (G->dim == 2 && l.tile >= 1 && step2d_ok(G))
+        return dtype == PHX_F64 ? rk4_step2d_steps_t<double, true>(G, (double*)c, (double*)work, p, nsteps, st,
+                                                                   l.tile - 1, pure)
+                                : rk4_step2d_steps_t<float, false>(G, (float*)c, (float*)work, p, nsteps, st,
+                                                                   l.tile - 1, pure);
+    if (G->dim != 3) l.tile = 0;
+    if (G->dim == 3 && l.tile >= kNumTiles3) {
+        if (!pair_ok(G)) l.tile = dtype == PHX_F32 ? 0 : 1;
+        else
+            return dtype == PHX_F64 ? rk4_pair_steps_t<double, true>(G, (double*)c, (double*)work, p, nsteps, st,
+                                                                     l.zc, l.tile - kNumTiles3, pure)
+                                    : rk4_pair_steps_t<float, false>(G, (float*)c, (float*)work, p, nsteps, st,
+                                                                     l.zc, l.tile - kNumTiles3, pure);
+    }
+    return dtype == PHX_F64 ? rk4_steps_t<double, true>(G, (double*)c, (double*)work, p, nsteps, st, l.zc, l.tile)
+                            : rk4_steps_t<float, false>(G, (float*)c, (float*)work, p, nsteps, st, l.zc, l.tile);
+}
+
+extern "C" {
 
 int phx_ch_rk4_steps(const phx_grid* G, int dtype, void* c, void* work, const phx_ch_params* p, int nsteps,
                      phx_stream s) {
@@ -723,14 +1587,14 @@ int phx_ch_rk4_steps(const phx_grid* G, int dtype, void* c, void* work, const ph
     if (!c || !work) return fail(PHX_ERR_INVALID, "phx_ch_rk4_steps: NULL buffer");
     if (nsteps < 0) return fail(PHX_ERR_INVALID, "phx_ch_rk4_steps: nsteps < 0");
     if (nsteps == 0) return PHX_OK;
-    const Launch3 l = G->dim == 3 ? launch_for(G, dtype) : Launch3{1, 0};
-    return dtype == PHX_F64
-               ? rk4_steps_t<double, true>(G, (double*)c, (double*)work, p, nsteps, as_stream(s), l.zc, l.tile)
-               : rk4_steps_t<float, false>(G, (float*)c, (float*)work, p, nsteps, as_stream(s), l.zc, l.tile);
+    return run_rk4(G, dtype, c, work, p, nsteps, as_stream(s), launch_for(G, dtype), 0);
 }
 
-// Times the 3D stage kernel at z-chunks {4, 8, 16, 32, 64} on `steps` steps of
-// the caller's state (advancing it) and keeps the fastest for this shape.
+// Times every launch candidate on `steps` steps of the caller's state
+// (advancing it) and keeps the fastest per step for this shape: in 3D the
+// one-stage kernel (5 tiles) and the two-stage kernel (4 tiles) at z-chunks
+// {4, 8, 16, 32, 64}; in 2D the one-stage kernel and the whole-step kernel
+// (4 tiles). Two-stage and whole-step candidates run 2*ceil(steps/2) steps.
 int phx_ch_rk4_autotune(const phx_grid* G, int dtype, void* c, void* work, const phx_ch_params* p, int steps,
                         int* best_zc, double* best_ms_per_step, phx_stream s) {
     int rc = check_grid_fused(G, "phx_ch_rk4_autotune");
@@ -743,20 +1607,28 @@ int phx_ch_rk4_autotune(const phx_grid* G, int dtype, void* c, void* work, const
     hipEvent_t e0, e1;
     PHX_HIP(hipEventCreate(&e0));
     PHX_HIP(hipEventCreate(&e1));
-    for (int tile = 0; tile < (G->dim == 3 ? kNumTiles3 : 1); ++tile)
+    const int ntiles = G->dim == 3 ? kNumTiles3 + (pair_ok(G) ? kNumTilesP : 0)
+                                   : (G->dim == 2 && step2d_ok(G) ? 1 + kNumTilesS2 : 1);
+    const int first_fused = G->dim == 3 ? kNumTiles3 : 1;
+    const int psteps = 2 * ((steps + 1) / 2);
+    for (int tile = 0; tile < ntiles; ++tile)
     for (int zc : cand) {
         if (G->dim == 3 && zc > G->n[2]) continue;
-        set_launch(G, dtype, Launch3{zc, tile});
-        if ((rc = phx_ch_rk4_steps(G, dtype, c, work, p, 1, s))) return rc;
+        if (G->dim != 3 && zc != cand[0]) continue;
+        const Launch3 l{G->dim == 3 ? zc : 1, tile};
+        const int pure = tile >= first_fused;
+        const int n = pure ? psteps : steps;
+        if ((rc = run_rk4(G, dtype, c, work, p, pure ? 2 : 1, st, l, pure))) return rc;
         PHX_HIP(hipEventRecord(e0, st));
-        if ((rc = phx_ch_rk4_steps(G, dtype, c, work, p, steps, s))) return rc;
+        if ((rc = run_rk4(G, dtype, c, work, p, n, st, l, pure))) return rc;
         PHX_HIP(hipEventRecord(e1, st));
         PHX_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
         PHX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ms /= n;
         if (ms < best) {
             best = ms;
-            bz = zc;
+            bz = l.zc;
             bt = tile;
         }
     }
@@ -764,7 +1636,30 @@ int phx_ch_rk4_autotune(const phx_grid* G, int dtype, void* c, void* work, const
     (void)hipEventDestroy(e1);
     set_launch(G, dtype, Launch3{bz, bt});
     if (best_zc) *best_zc = bz;
-    if (best_ms_per_step) *best_ms_per_step = best / steps;
+    if (best_ms_per_step) *best_ms_per_step = best;
+    return PHX_OK;
+}
+
+int phx_ch_timing(int on) {
+    g_lt.on = on ? 1 : 0;
+    g_lt.used = 0;
+    return PHX_OK;
+}
+
+int phx_ch_timing_read(int kind, int* launches, double* total_ms) {
+    if (kind < 0 || kind > 3) return fail(PHX_ERR_INVALID, "phx_ch_timing_read: kind must be in [0, 3]");
+    int n = 0;
+    double ms = 0.0;
+    for (size_t i = 0; i < g_lt.used; ++i) {
+        if (g_lt.kind[i] != kind) continue;
+        PHX_HIP(hipEventSynchronize(g_lt.ev[2 * i + 1]));
+        float t = 0.f;
+        PHX_HIP(hipEventElapsedTime(&t, g_lt.ev[2 * i], g_lt.ev[2 * i + 1]));
+        ms += t;
+        ++n;
+    }
+    if (launches) *launches = n;
+    if (total_ms) *total_ms = ms;
     return PHX_OK;
 }
 
