@@ -4,8 +4,9 @@
  * INFRASTRUCTURE ONLY. The reference has no learned part; this is the naive
  * direct sum from the definition
  *     y[n,o,h,w] = b[o] + sum_{i,dh,dw} w[o,i,dh,dw] * x[n,i,(h+dh-1) mod H,(w+dw-1) mod W]
- * pinned in tests by finite differences, identity/delta kernels and a numpy
- * FFT (circular convolution theorem).
+ * pinned in tests/test_oracle_conv.py by a numpy FFT (circular convolution
+ * theorem), identity and shift kernels, the adjoint identity (data gradient)
+ * and linearity in (w, b) (weight and bias gradients).
  */
 #include <stddef.h>
 
