@@ -164,11 +164,14 @@ size_t phx_ch_rk4_workspace_bytes(const phx_grid* grid, int dtype);
 int phx_ch_rk4_steps(const phx_grid* grid, int dtype, void* c, void* work,
                      const phx_ch_params* p, int nsteps, phx_stream stream);
 
-/* Launch shape of the 3D stage kernel: z planes streamed per workgroup
- * (default 16), per (n, dtype). The autotuner times every tile shape x
- * z-chunk {4,8,16,32,64} on `steps`
- * steps of the given state (advancing it: pass a scratch copy), keeps the
- * fastest and reports it. Both are host-side settings; no device allocation. */
+/* Launch shape of the 3D stage kernels: z-chunk zc (default 16), per
+ * (dim, n, dtype); the planes are split into cdiv(n_z, zc) balanced chunks of
+ * floor or ceil(n_z / chunks) planes, one workgroup column each. The
+ * autotuner times every tile shape x z-chunk in {4,8,16,32,64} plus the
+ * chunk counts that fill 1, 2 or 3 whole waves of resident workgroups (from
+ * the occupancy calculator) on `steps` steps of the given state (advancing
+ * it: pass a scratch copy), keeps the fastest and reports it. Host-side
+ * settings; no device allocation. */
 int phx_ch_rk4_set_zchunk(const phx_grid* grid, int dtype, int zc);
 /* Tile shape of the 3D stage kernel: 0: 64x16, 1: 64x8, 2: 128x8, 3: 32x16,
  * 4: 64x4 (x by y columns per 256-thread workgroup). Tiles 5: 64x8, 6: 32x16,

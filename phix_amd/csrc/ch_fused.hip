@@ -87,27 +87,69 @@ __device__ __forceinline__ T load_halo(const T* __restrict__ u, const CHArgs<T>&
     return u[A.L.at(qx, qy, qz)];
 }
 
+// The six FIXED constants (x lo, x hi, y lo, y hi, z lo, z hi) staged in LDS
+// by each block: reading them there in divergent ghost code keeps the
+// compiler from turning a data-dependent choice between kernel arguments into
+// a vector memory load, whose wait would also drain the prefetched planes.
+template <typename T, typename Args>
+__device__ __forceinline__ void stage_bc_values(T* bcv, const Args& A) {
+    if (threadIdx.x == 0) {
+        bcv[0] = A.bx.vlo;
+        bcv[1] = A.bx.vhi;
+        bcv[2] = A.by.vlo;
+        bcv[3] = A.by.vhi;
+        bcv[4] = A.bz.vlo;
+        bcv[5] = A.bz.vhi;
+    }
+}
+
+// A workgroup-uniform copy of an axis's BC, read once into scalar registers.
+// Per-plane code then chooses between its fields with scalar selects; left
+// to itself the compiler may choose between kernel-argument addresses and
+// issue a vector memory load for a uniform value, whose wait drains the
+// prefetched planes.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float uni(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ double uni(double v) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffLL));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+template <typename T>
+__device__ __forceinline__ AxisBC<T> uni_axis(const AxisBC<T>& b) {
+    AxisBC<T> u;
+    u.n = uni(b.n);
+    u.klo = uni(b.klo);
+    u.khi = uni(b.khi);
+    u.vlo = uni(b.vlo);
+    u.vhi = uni(b.vhi);
+    return u;
+}
+
 // mu at halo position p from the stage state's 7 (5) point star.
-//  c: centre; x/y/z (p,m): the +/- neighbours as loaded.
+//  c: centre; x/y/z (p,m): the +/- neighbours as loaded; bcv: staged FIXED constants.
 template <typename T, int DIM>
-__device__ __forceinline__ T mu_at(const CHArgs<T>& A, int px, int py, int pz, T c, T xp, T xm, T yp, T ym, T zp,
-                                   T zm) {
+__device__ __forceinline__ T mu_at(const CHArgs<T>& A, const T* bcv, int px, int py, int pz, T c, T xp, T xm, T yp,
+                                   T ym, T zp, T zm) {
     const bool gx = px < 0 || px >= A.L.n0;
     const bool gy = DIM >= 2 && (py < 0 || py >= A.L.n1);
     const bool gz = DIM >= 3 && (pz < 0 || pz >= A.L.n2);
     if (gz) {
         int k = pz < 0 ? A.bz.klo : A.bz.khi;
-        if (k == PHX_BC_FIXED) return fixed_value(pz, A.bz);
+        if (k == PHX_BC_FIXED) return bcv[pz < 0 ? 4 : 5];
         if (k == PHX_BC_NOFLUX) { T t = zp; zp = zm; zm = t; }
     }
     if (gy) {
         int k = py < 0 ? A.by.klo : A.by.khi;
-        if (k == PHX_BC_FIXED) return fixed_value(py, A.by);
+        if (k == PHX_BC_FIXED) return bcv[py < 0 ? 2 : 3];
         if (k == PHX_BC_NOFLUX) { T t = yp; yp = ym; ym = t; }
     }
     if (gx) {
         int k = px < 0 ? A.bx.klo : A.bx.khi;
-        if (k == PHX_BC_FIXED) return fixed_value(px, A.bx);
+        if (k == PHX_BC_FIXED) return bcv[px < 0 ? 0 : 1];
         if (k == PHX_BC_NOFLUX) { T t = xp; xp = xm; xm = t; }
     }
     T val = (xp - (T)2 * c + xm) * A.idx2;
@@ -151,30 +193,53 @@ __global__ __launch_bounds__(256) void k_ch_stage_2d(CHArgs<T> A) {
     constexpr int UX = TX + 4, UY = TY + 4, MX = TX + 2, MY = TY + 2;
     __shared__ T U[UY][UX];
     __shared__ T MU[MY][MX];
+    __shared__ T BCV[6];
     const int tid = threadIdx.x;
     const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
     const int kz = 0;
+    stage_bc_values(BCV, A);
 
-    const bool interior = (x0 >= 2) && (x0 + TX + 1 < A.L.n0) && (DIM < 2 || ((y0 >= 2) && (y0 + TY + 1 < A.L.n1)));
-    if (interior) {
-        const T* base = A.u + A.L.at(x0 - 2, y0 - 2, kz);
-        for (int t = tid; t < UX * UY; t += 256) {
-            int ly = t / UX, lx = t - ly * UX;
-            U[ly][lx] = base[(long long)ly * A.L.sx + lx];
+    // Halo load: every slot's offset (BC maps applied) first, then all loads
+    // back to back with no branch between them, so their latencies overlap;
+    // FIXED and surplus slots read a valid cell and are replaced at the store.
+    constexpr int NS = (UX * UY + 255) / 256;
+    const long long zb = (long long)A.L.g * A.L.sxy;
+    const int cell0 = A.L.g * A.L.sx + A.L.g;
+    int off[NS];
+    unsigned fm = 0;
+    T fv[NS];
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        const int t = min(tid + 256 * i, UX * UY - 1);
+        const int ly = t / UX, lx = t - ly * UX;
+        const int px = x0 - 2 + lx, py = DIM >= 2 ? y0 - 2 + ly : 0;
+        const int qy = DIM >= 2 ? map_in(py, A.by) : 0, qx = map_in(px, A.bx);
+        off[i] = cell0;
+        fv[i] = (T)0;
+        if (qy < 0) {
+            fm |= 1u << i;
+            fv[i] = fixed_value(py, A.by);
+        } else if (qx < 0) {
+            fm |= 1u << i;
+            fv[i] = fixed_value(px, A.bx);
+        } else {
+            off[i] = (qy + A.L.g) * A.L.sx + (qx + A.L.g);
         }
-    } else {
-        for (int t = tid; t < UX * UY; t += 256) {
-            int ly = t / UX, lx = t - ly * UX;
-            int py = DIM >= 2 ? y0 - 2 + ly : 0;
-            U[ly][lx] = load_halo<T, DIM>(A.u, A, x0 - 2 + lx, py, kz);
-        }
+    }
+    T r[NS];
+#pragma unroll
+    for (int i = 0; i < NS; ++i) r[i] = A.u[zb + off[i]];
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        const int t = tid + 256 * i;
+        if (t < UX * UY) (&U[0][0])[t] = ((fm >> i) & 1u) ? fv[i] : r[i];
     }
     __syncthreads();
     for (int t = tid; t < MX * MY; t += 256) {
         int my = t / MX, mx = t - my * MX;
         int px = x0 - 1 + mx, py = DIM >= 2 ? y0 - 1 + my : 0;
         T c = U[my + 1][mx + 1];
-        T m = mu_at<T, DIM>(A, px, py, kz, c, U[my + 1][mx + 2], U[my + 1][mx], DIM >= 2 ? U[my + 2][mx + 1] : c,
+        T m = mu_at<T, DIM>(A, BCV, px, py, kz, c, U[my + 1][mx + 2], U[my + 1][mx], DIM >= 2 ? U[my + 2][mx + 1] : c,
                             DIM >= 2 ? U[my][mx + 1] : c, c, c);
         MU[my][mx] = m;
     }
@@ -192,7 +257,8 @@ __global__ __launch_bounds__(256) void k_ch_stage_2d(CHArgs<T> A) {
 }
 
 // ------------------------------------------------------------------- 3D ----
-// One block = TX x TY columns of a z-chunk of A.zc planes, streaming along z
+// One block = TX x TY columns of a z-chunk (the grid's z extent splits the
+// planes into balanced chunks, about A.zc planes each), streaming along z
 // with a ring of 4 halo planes of the stage state and 3 mu planes in LDS.
 // Software pipelined: while plane o is computed, the halo plane o+3 and the
 // next output plane's c/acc values are already in flight to registers, so the
@@ -213,7 +279,9 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
     static_assert((TX * TY) % 256 == 0, "tile must be a multiple of 256 cells");
     __shared__ T U[4][UY * UX];
     __shared__ T MU[3][MY * MX];
+    __shared__ T BCV[6];
     const int tid = threadIdx.x;
+    stage_bc_values(BCV, A);
     // XCD-aware tile order: workgroups are dispatched round-robin over the 8
     // XCDs by linear id, so hand each XCD a contiguous range of tiles (whole
     // xy slabs of consecutive z-chunks) -- tiles sharing halos then share an L2.
@@ -228,35 +296,41 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
         bz = lb / (gx * gy);
     }
     const int x0 = bx * TX, y0 = by * TY;
-    const int z0 = bz * A.zc;
-    const int z1 = min(z0 + A.zc, A.L.n2);
+    // balanced z chunks: gridDim.z chunks of floor or ceil(n2 / gridDim.z) planes
+    const int z0 = (int)(((long long)bz * A.L.n2) / gridDim.z);
+    const int z1 = (int)(((long long)(bz + 1) * A.L.n2) / gridDim.z);
     const int g = A.L.g;
     const long long sxy = A.L.sxy;
 
-    // halo slots: in-plane offset (x/y maps applied) or -1 for a FIXED constant
+    // Halo slots: in-plane offset with the x/y maps applied. Every slot loads
+    // unconditionally from a valid cell (FIXED and surplus slots read the
+    // plane's first cell), so no load sits in a divergent branch and the
+    // prefetched plane stays in flight across the pipeline; the FIXED
+    // constants replace the loaded values when the plane is stored.
     int hoff[NU];
     T hfix[NU];
+    unsigned hfm = 0;   // bit i: slot i holds a FIXED constant
 #pragma unroll
     for (int i = 0; i < NU; ++i) {
         const int t = tid + 256 * i;
-        hoff[i] = 0;
+        hoff[i] = g * A.L.sx + g;
         hfix[i] = (T)0;
         if (t < UX * UY) {
             const int ly = t / UX, lx = t - ly * UX;
             const int px = x0 - 2 + lx, py = y0 - 2 + ly;
             const int qy = map_in(py, A.by), qx = map_in(px, A.bx);
             if (qy < 0) {
-                hoff[i] = -1;
+                hfm |= 1u << i;
                 hfix[i] = fixed_value(py, A.by);
             } else if (qx < 0) {
-                hoff[i] = -1;
+                hfm |= 1u << i;
                 hfix[i] = fixed_value(px, A.bx);
             } else {
                 hoff[i] = (qy + g) * A.L.sx + (qx + g);
             }
         }
     }
-    // output slots
+    // output slots (cells beyond the domain read a valid cell, never store)
     int ooff[NO];
     bool oval[NO];
 #pragma unroll
@@ -265,27 +339,28 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
         const int oy = t / TX, ox = t - oy * TX;
         const int x = x0 + ox, y = y0 + oy;
         oval[j] = x < A.L.n0 && y < A.L.n1;
-        ooff[j] = (y + g) * A.L.sx + (x + g);
+        ooff[j] = oval[j] ? (y + g) * A.L.sx + (x + g) : g * A.L.sx + g;
     }
 
-    auto load_plane = [&](int pz, T (&r)[NU]) {
-        const int qz = map_in(pz, A.bz);
-        if (qz < 0) {
-            const T v = fixed_value(pz, A.bz);
-#pragma unroll
-            for (int i = 0; i < NU; ++i) r[i] = v;
-        } else {
-            const T* base = A.u + (long long)(qz + g) * sxy;
-#pragma unroll
-            for (int i = 0; i < NU; ++i)
-                if (tid + 256 * i < UX * UY) r[i] = hoff[i] >= 0 ? base[hoff[i]] : hfix[i];
-        }
+    // a plane in flight: its registers, and its FIXED z constant if any
+    struct Plane {
+        T r[NU];
+        bool fix;
+        T val;
     };
-    auto store_plane = [&](int pz, const T (&r)[NU]) {
+    auto load_plane = [&](int pz, Plane& P) {
+        const int qz = map_in(pz, A.bz);
+        P.fix = qz < 0;
+        P.val = P.fix ? fixed_value(pz, A.bz) : (T)0;
+        const T* base = A.u + (long long)((P.fix ? 0 : qz) + g) * sxy;
+#pragma unroll
+        for (int i = 0; i < NU; ++i) P.r[i] = base[hoff[i]];
+    };
+    auto store_plane = [&](int pz, const Plane& P) {
         T* dst = U[(pz + 4) & 3];
 #pragma unroll
         for (int i = 0; i < NU; ++i)
-            if (tid + 256 * i < UX * UY) dst[tid + 256 * i] = r[i];
+            if (tid + 256 * i < UX * UY) dst[tid + 256 * i] = P.fix ? P.val : (((hfm >> i) & 1u) ? hfix[i] : P.r[i]);
     };
     // mu slots: LDS centre offset in the halo plane, and halo position
     int mc0[NM], mpx[NM], mpy[NM];
@@ -323,7 +398,7 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
                 const int t = tid + 256 * i;
                 if (t < MX * MY) {
                     const int c0 = mc0[i];
-                    dst[t] = mu_at<T, 3>(A, mpx[i], mpy[i], m, um[c0], um[c0 + 1], um[c0 - 1], um[c0 + UX],
+                    dst[t] = mu_at<T, 3>(A, BCV, mpx[i], mpy[i], m, um[c0], um[c0 + 1], um[c0 - 1], um[c0 + UX],
                                          um[c0 - UX], uh[c0], ul[c0]);
                 }
             }
@@ -331,21 +406,26 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
     };
     // epilogue operands of one output plane
     const int st = A.stage;
+    // Both epilogue operands are loaded for every stage so that each
+    // iteration issues the same loads (the compiler then waits for exactly the
+    // ones it uses); a stage that needs neither c_n nor acc reads the stage
+    // state instead, whose cells are in cache from the halo load.
     const bool need_c = st == 2 || st == 3 || (EXACT && st == 4);
     const bool need_a = st >= 2;
+    const T* csrc = need_c ? A.c : A.u;
+    const T* asrc = need_a ? A.acc : A.u;
     T pc[NO], pa[NO], nc[NO], na[NO];
     auto load_epi = [&](int o, T (&rc)[NO], T (&ra)[NO]) {
         const long long pb = (long long)(o + g) * sxy;
 #pragma unroll
         for (int j = 0; j < NO; ++j) {
-            if (!oval[j]) continue;
-            if (need_c) rc[j] = A.c[pb + ooff[j]];
-            if (need_a) ra[j] = A.acc[pb + ooff[j]];
+            rc[j] = csrc[pb + ooff[j]];
+            ra[j] = asrc[pb + ooff[j]];
         }
     };
 
     // ---- prologue: halo planes z0-2 .. z0+1, mu planes z0-1, z0
-    T pre[NU];
+    Plane pre;
     load_plane(z0 - 2, pre);
     store_plane(z0 - 2, pre);
     load_plane(z0 - 1, pre);
@@ -364,8 +444,9 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
     for (int o = z0; o < z1; ++o) {
         store_plane(o + 2, pre);
         __syncthreads();   // A
-        if (o + 3 <= z1 + 1) load_plane(o + 3, pre);
-        if (o + 1 < z1) load_epi(o + 1, nc, na);
+        // unconditional prefetch (clamped planes): the same loads every iteration
+        load_plane(min(o + 3, z1 + 1), pre);
+        load_epi(min(o + 1, z1 - 1), nc, na);
         compute_mu(o + 1);
         __syncthreads();   // B
         const T* mlo = MU[(o + 2) % 3];
@@ -499,8 +580,8 @@ __device__ __forceinline__ int ghost_code_xy(int px, int py, const PairArgs<T>& 
 }
 
 template <typename T>
-__device__ __forceinline__ T ghost_value_xy(int code, const PairArgs<T>& A) {
-    return (code & GC_FY) ? ((code & GC_HI) ? A.by.vhi : A.by.vlo) : ((code & GC_HI) ? A.bx.vhi : A.bx.vlo);
+__device__ __forceinline__ T ghost_value_xy(int code, const T* bcv) {
+    return bcv[((code & GC_FY) ? 2 : 0) + ((code & GC_HI) ? 1 : 0)];
 }
 
 // (xp - 2c + xm)*idx2 + (yp - 2c + ym)*idy2 + (zp - 2c + zm)*idz2, in this order
@@ -544,7 +625,11 @@ __global__ __launch_bounds__(256) void k_ch_pair_3d(PairArgs<T> A) {
     __shared__ T MB[3][N1];
     __shared__ T KR[3][N0];
     __shared__ T CR[NEED_CR ? 3 : 1][NEED_CR ? N0 : 1];
+    __shared__ T BCV[6];
     const int tid = threadIdx.x;
+    stage_bc_values(BCV, A);
+    const AxisBC<T> bzu = uni_axis(A.bz);
+    const int n2 = uni(A.L.n2);
     int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
     if (A.xcd_remap) {
         const int gx = gridDim.x, gy = gridDim.y;
@@ -556,30 +641,36 @@ __global__ __launch_bounds__(256) void k_ch_pair_3d(PairArgs<T> A) {
         bz = lb / (gx * gy);
     }
     const int x0 = bx * TX, y0 = by * TY;
-    const int z0 = bz * A.zc;
-    const int z1 = min(z0 + A.zc, A.L.n2);
+    // balanced z chunks: gridDim.z chunks of floor or ceil(n2 / gridDim.z) planes
+    const int z0 = (int)(((long long)bz * n2) / gridDim.z);
+    const int z1 = (int)(((long long)(bz + 1) * n2) / gridDim.z);
     const int g = A.L.g;
     const long long sxy = A.L.sxy;
     const bool xy_in = x0 >= 4 && x0 + TX + 4 <= A.L.n0 && y0 >= 4 && y0 + TY + 4 <= A.L.n1;
     auto rs = [](int p) { return (p + 12) % 3; };   // ring slot, planes >= -4
 
-    // UA halo slots: in-plane offset with the x/y images, or -1 for a FIXED constant
+    // UA halo slots: in-plane offset with the x/y images. Every slot loads
+    // unconditionally from a valid cell (FIXED and surplus slots read the
+    // plane's first cell) so loads never sit in divergent branches and stay in
+    // flight across the pipeline; FIXED constants replace them at the store.
+    const int cell0 = g * A.L.sx + g;
     int hoff[S4];
     T hfix[S4];
+    unsigned hfm = 0;   // bit i: slot i holds a FIXED constant
 #pragma unroll
     for (int i = 0; i < S4; ++i) {
         const int t = tid + 256 * i;
-        hoff[i] = 0;
+        hoff[i] = cell0;
         hfix[i] = (T)0;
         if (t < N4) {
             const int ly = t / W4, lx = t - ly * W4;
             const int px = x0 - 4 + lx, py = y0 - 4 + ly;
             const int qy = map_deep(py, A.by), qx = map_deep(px, A.bx);
             if (qy < 0) {
-                hoff[i] = -1;
+                hfm |= 1u << i;
                 hfix[i] = fixed_value(py, A.by);
             } else if (qx < 0) {
-                hoff[i] = -1;
+                hfm |= 1u << i;
                 hfix[i] = fixed_value(px, A.bx);
             } else {
                 hoff[i] = (qy + g) * A.L.sx + (qx + g);
@@ -611,7 +702,7 @@ __global__ __launch_bounds__(256) void k_ch_pair_3d(PairArgs<T> A) {
         for (int i = 0; i < S2; ++i) {
             const int t = tid + 256 * i, ly = t / W2, lx = t - ly * W2;
             const int qy = map_deep(y0 - 2 + ly, A.by), qx = map_deep(x0 - 2 + lx, A.bx);
-            coff[i] = (t < N2 && qy >= 0 && qx >= 0) ? (qy + g) * A.L.sx + (qx + g) : -1;
+            coff[i] = (t < N2 && qy >= 0 && qx >= 0) ? (qy + g) * A.L.sx + (qx + g) : cell0;
             creg[i] = (T)0;
         }
     }
@@ -625,52 +716,52 @@ __global__ __launch_bounds__(256) void k_ch_pair_3d(PairArgs<T> A) {
         const int oy = t / TX, ox = t - oy * TX;
         const int x = x0 + ox, y = y0 + oy;
         oval[j] = x < A.L.n0 && y < A.L.n1;
-        ooff[j] = (y + g) * A.L.sx + (x + g);
+        ooff[j] = oval[j] ? (y + g) * A.L.sx + (x + g) : cell0;
     }
 
-    auto load_ua = [&](int pz, T (&r)[S4]) {
-        const int qz = map_deep(pz, A.bz);
-        if (qz < 0) {
-            const T v = fixed_value(pz, A.bz);
-#pragma unroll
-            for (int i = 0; i < S4; ++i) r[i] = v;
-        } else {
-            const T* base = A.u + (long long)(qz + g) * sxy;
-#pragma unroll
-            for (int i = 0; i < S4; ++i)
-                if (tid + 256 * i < N4) r[i] = hoff[i] >= 0 ? base[hoff[i]] : hfix[i];
-        }
+    // a UA plane in flight: its registers, and its FIXED z constant if any
+    struct Plane {
+        T r[S4];
+        bool fix;
+        T val;
     };
-    auto store_ua = [&](int pz, const T (&r)[S4]) {
+    auto load_ua = [&](int pz, Plane& P) {
+        const int qz = map_deep(pz, bzu);
+        P.fix = qz < 0;
+        P.val = P.fix ? fixed_value(pz, bzu) : (T)0;
+        const T* base = A.u + (long long)((P.fix ? 0 : qz) + g) * sxy;
+#pragma unroll
+        for (int i = 0; i < S4; ++i) P.r[i] = base[hoff[i]];
+    };
+    auto store_ua = [&](int pz, const Plane& P) {
         T* dst = UA[rs(pz)];
 #pragma unroll
         for (int i = 0; i < S4; ++i)
-            if (tid + 256 * i < N4) dst[tid + 256 * i] = r[i];
+            if (tid + 256 * i < N4) dst[tid + 256 * i] = P.fix ? P.val : (((hfm >> i) & 1u) ? hfix[i] : P.r[i]);
     };
+    // c_n of a plane at the UB slots; on a FIXED z plane UB is the constant
+    // and the values are unused
     auto load_c = [&](int pz) {
-        const int qz = map_deep(pz, A.bz);
-        if (qz < 0) return;   // FIXED plane: UB is the constant there
-        const T* base = A.c + (long long)(qz + g) * sxy;
+        const int qz = map_deep(pz, bzu);
+        const T* base = A.c + (long long)((qz < 0 ? 0 : qz) + g) * sxy;
 #pragma unroll
-        for (int i = 0; i < (SECOND ? S2 : 1); ++i)
-            if (coff[i] >= 0) creg[i] = base[coff[i]];
+        for (int i = 0; i < (SECOND ? S2 : 1); ++i) creg[i] = base[coff[i]];
     };
     auto load_acc = [&](int o) {
         const long long pb = (long long)(o + g) * sxy;
 #pragma unroll
-        for (int j = 0; j < (SECOND ? S0 : 1); ++j)
-            if (oval[j]) areg[j] = A.acc[pb + ooff[j]];
+        for (int j = 0; j < (SECOND ? S0 : 1); ++j) areg[j] = A.acc[pb + ooff[j]];
     };
     // ghost rule of plane pz along z (decides before y and x)
     auto zrule = [&](int pz, bool& zfix, bool& zsw, T& zv) {
         zfix = false;
         zsw = false;
         zv = (T)0;
-        if (pz < 0 || pz >= A.L.n2) {
-            const int k = pz < 0 ? A.bz.klo : A.bz.khi;
+        if (pz < 0 || pz >= n2) {
+            const int k = pz < 0 ? bzu.klo : bzu.khi;
             if (k == PHX_BC_FIXED) {
                 zfix = true;
-                zv = fixed_value(pz, A.bz);
+                zv = fixed_value(pz, bzu);
             } else if (k == PHX_BC_NOFLUX) {
                 zsw = true;
             }
@@ -696,7 +787,7 @@ __global__ __launch_bounds__(256) void k_ch_pair_3d(PairArgs<T> A) {
                     const int ci = (ly + 1) * W4 + lx + 1;
                     T v;
                     if (GEN && zfix) v = zv;
-                    else if (GEN && (code3[i] & GC_FIX)) v = ghost_value_xy(code3[i], A);
+                    else if (GEN && (code3[i] & GC_FIX)) v = ghost_value_xy(code3[i], BCV);
                     else {
                         T c;
                         const T l = star_lap<T, W4, GEN>(A, lo, mid, hi, ci, code3[i], zsw, c);
@@ -734,7 +825,7 @@ __global__ __launch_bounds__(256) void k_ch_pair_3d(PairArgs<T> A) {
                     else cc = uc[(ly + 2) * W4 + lx + 2];
                     T v, k = (T)0;
                     if (GEN && zfix) v = zv;
-                    else if (GEN && (code2[i] & GC_FIX)) v = ghost_value_xy(code2[i], A);
+                    else if (GEN && (code2[i] & GC_FIX)) v = ghost_value_xy(code2[i], BCV);
                     else {
                         T m;
                         k = A.M * star_lap<T, W3, GEN>(A, lo, mid, hi, mi, code2[i], zsw, m);
@@ -772,7 +863,7 @@ __global__ __launch_bounds__(256) void k_ch_pair_3d(PairArgs<T> A) {
                     const int ci = (ly + 1) * W2 + lx + 1;
                     T v;
                     if (GEN && zfix) v = zv;
-                    else if (GEN && (code1[i] & GC_FIX)) v = ghost_value_xy(code1[i], A);
+                    else if (GEN && (code1[i] & GC_FIX)) v = ghost_value_xy(code1[i], BCV);
                     else {
                         T c;
                         const T l = star_lap<T, W2, GEN>(A, lo, mid, hi, ci, code1[i], zsw, c);
@@ -821,33 +912,36 @@ __global__ __launch_bounds__(256) void k_ch_pair_3d(PairArgs<T> A) {
     };
 
     // prologue: UA planes z0-4, z0-3 stored, z0-2 in flight
-    T pre[S4];
+    Plane pre;
     load_ua(z0 - 4, pre);
     store_ua(z0 - 4, pre);
     load_ua(z0 - 3, pre);
     store_ua(z0 - 3, pre);
     load_ua(z0 - 2, pre);
-    if (SECOND) {
-        load_c(z0 - 2);
+    if constexpr (SECOND) {
+        // same load sequence as an iteration's (values unused: the loop
+        // reloads both before their first use), so the loop header is
+        // entered with one wait state
+        load_c(z0 - 3);
         load_acc(z0);
     }
+    // Every iteration issues the same loads (planes clamped to the chunk's
+    // range), so the compiler waits for exactly the ones it uses and the
+    // younger prefetches stay in flight: UA(o+5) after B1, c_n(o+3) after P3,
+    // acc(o+1) after P5, each consumed one iteration later.
     for (int o = z0 - 6; o < z1; ++o) {
         store_ua(o + 4, pre);
         __syncthreads();   // B1
-        if (o + 5 <= z1 + 3) load_ua(o + 5, pre);
+        load_ua(min(o + 5, z1 + 3), pre);
         if (o + 3 >= z0 - 3) eval_ma(o + 3);
         __syncthreads();   // B2
-        if (o + 2 >= z0 - 2) {
-            eval_ub(o + 2);
-            if (SECOND && o + 3 <= z1 + 1) load_c(o + 3);
-        }
+        if (o + 2 >= z0 - 2) eval_ub(o + 2);
+        if constexpr (SECOND) load_c(min(o + 3, z1 + 1));
         __syncthreads();   // B3
         if (o + 1 >= z0 - 1) eval_mb(o + 1);
         __syncthreads();   // B4
-        if (o >= z0) {
-            out_plane(o);
-            if (SECOND && o + 1 < z1) load_acc(o + 1);
-        }
+        if (o >= z0) out_plane(o);
+        if constexpr (SECOND) load_acc(max(z0, min(o + 1, z1 - 1)));
     }
 }
 
@@ -884,7 +978,9 @@ __global__ __launch_bounds__(256) void k_ch_step_2d(PairArgs<T> A) {
     __shared__ T SA[NA];
     __shared__ T SB[NB];
     __shared__ T SACC[N0];
+    __shared__ T BCV[6];
     const int tid = threadIdx.x;
+    stage_bc_values(BCV, A);
     int bx = blockIdx.x, by = blockIdx.y;
     if (A.xcd_remap) {
         const int gx = gridDim.x, nb = gx * gridDim.y;
@@ -898,23 +994,41 @@ __global__ __launch_bounds__(256) void k_ch_step_2d(PairArgs<T> A) {
     const long long zb = (long long)g * A.L.sxy;
     const bool xy_in = x0 >= 8 && x0 + TX + 8 <= A.L.n0 && y0 >= 8 && y0 + TY + 8 <= A.L.n1;
 
-    // c_n on the tile and an 8-cell ring, x/y BC images applied
-    if (xy_in) {
-        const T* base = A.u + zb + (long long)(y0 - 8 + g) * A.L.sx + (x0 - 8 + g);
-        for (int t = tid; t < NC; t += 256) {
-            const int ly = t / WC, lx = t - ly * WC;
-            SC[t] = base[(long long)ly * A.L.sx + lx];
-        }
-    } else {
-        for (int t = tid; t < NC; t += 256) {
+    // c_n on the tile and an 8-cell ring, x/y BC images applied: offsets
+    // first, then all loads back to back (no branch between them, so their
+    // latencies overlap); FIXED and surplus slots read a valid cell and are
+    // replaced at the store.
+    constexpr int NSC = (NC + 255) / 256;
+    {
+        const int cell0 = g * A.L.sx + g;
+        int off[NSC];
+        unsigned long long fm = 0;
+#pragma unroll
+        for (int i = 0; i < NSC; ++i) {
+            const int t = min(tid + 256 * i, NC - 1);
             const int ly = t / WC, lx = t - ly * WC;
             const int px = x0 - 8 + lx, py = y0 - 8 + ly;
             const int qy = map_deep<T, 8>(py, A.by), qx = map_deep<T, 8>(px, A.bx);
-            T v;
-            if (qy < 0) v = fixed_value(py, A.by);
-            else if (qx < 0) v = fixed_value(px, A.bx);
-            else v = A.u[zb + (long long)(qy + g) * A.L.sx + (qx + g)];
-            SC[t] = v;
+            off[i] = cell0;
+            if (qy < 0 || qx < 0) fm |= 1ull << i;
+            else off[i] = (qy + g) * A.L.sx + (qx + g);
+        }
+        T r[NSC];
+#pragma unroll
+        for (int i = 0; i < NSC; ++i) r[i] = A.u[zb + off[i]];
+#pragma unroll
+        for (int i = 0; i < NSC; ++i) {
+            const int t = tid + 256 * i;
+            if (t < NC) {
+                T v = r[i];
+                if ((fm >> i) & 1ull) {
+                    const int ly = t / WC, lx = t - ly * WC;
+                    const int px = x0 - 8 + lx, py = y0 - 8 + ly;
+                    // once per block, before the barrier that publishes BCV
+                    v = map_deep<T, 8>(py, A.by) < 0 ? fixed_value(py, A.by) : fixed_value(px, A.bx);
+                }
+                SC[t] = v;
+            }
         }
     }
     __syncthreads();
@@ -929,7 +1043,7 @@ __global__ __launch_bounds__(256) void k_ch_step_2d(PairArgs<T> A) {
             const int code = xy_in ? 0 : ghost_code_xy(x0 - R + lx, y0 - R + ly, A);
             T v;
             if (code & GC_FIX) {
-                v = ghost_value_xy(code, A);
+                v = ghost_value_xy(code, BCV);
             } else {
                 const T c = src[si];
                 T xp = src[si + 1], xm = src[si - 1], yp = src[si + WS], ym = src[si - WS];
@@ -953,7 +1067,7 @@ __global__ __launch_bounds__(256) void k_ch_step_2d(PairArgs<T> A) {
             const T cc = SC[(ly + 8 - R) * WC + lx + 8 - R];
             T v, k = (T)0;
             if (code & GC_FIX) {
-                v = ghost_value_xy(code, A);
+                v = ghost_value_xy(code, BCV);
             } else {
                 const T m = SA[si];
                 T xp = SA[si + 1], xm = SA[si - 1], yp = SA[si + WS], ym = SA[si - WS];
@@ -1240,6 +1354,32 @@ static void lt_end(int kind, hipStream_t st) {
 }
 
 static int g_xcd_remap = 1;
+
+// Resident workgroups per CU of the 3D kernel of launch tile `tile` (one-stage
+// 0-4; two-stage 5-8, sized by its (3,4) kernel), from the occupancy
+// calculator; 0 if unknown.
+template <typename T, bool EXACT>
+static int blocks_per_cu(int tile) {
+    const void* f = nullptr;
+    switch (tile) {
+        case 0: f = reinterpret_cast<const void*>(&k_ch_stage_3d<T, 64, 16, EXACT>); break;
+        case 1: f = reinterpret_cast<const void*>(&k_ch_stage_3d<T, 64, 8, EXACT>); break;
+        case 2: f = reinterpret_cast<const void*>(&k_ch_stage_3d<T, 128, 8, EXACT>); break;
+        case 3: f = reinterpret_cast<const void*>(&k_ch_stage_3d<T, 32, 16, EXACT>); break;
+        case 4: f = reinterpret_cast<const void*>(&k_ch_stage_3d<T, 64, 4, EXACT>); break;
+        case 5: f = reinterpret_cast<const void*>(&k_ch_pair_3d<T, 64, 8, EXACT, true>); break;
+        case 6: f = reinterpret_cast<const void*>(&k_ch_pair_3d<T, 32, 16, EXACT, true>); break;
+        case 7: f = reinterpret_cast<const void*>(&k_ch_pair_3d<T, 32, 8, EXACT, true>); break;
+        case 8: f = reinterpret_cast<const void*>(&k_ch_pair_3d<T, 64, 4, EXACT, true>); break;
+        default: return 0;
+    }
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 256, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return nb;
+}
 
 template <typename T, bool EXACT>
 static int rk4_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_params* p, int nsteps, hipStream_t st,
@@ -1611,8 +1751,24 @@ int phx_ch_rk4_autotune(const phx_grid* G, int dtype, void* c, void* work, const
                                    : (G->dim == 2 && step2d_ok(G) ? 1 + kNumTilesS2 : 1);
     const int first_fused = G->dim == 3 ? kNumTiles3 : 1;
     const int psteps = 2 * ((steps + 1) / 2);
-    for (int tile = 0; tile < ntiles; ++tile)
-    for (int zc : cand) {
+    int dev = 0, ncu = 0;
+    PHX_HIP(hipGetDevice(&dev));
+    PHX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    for (int tile = 0; tile < ntiles; ++tile) {
+    // z-chunks: fixed sizes, plus chunk counts that fill whole waves of the
+    // resident workgroup slots (no partly filled last wave)
+    std::vector<int> zcs(cand, cand + sizeof(cand) / sizeof(cand[0]));
+    if (G->dim == 3) {
+        const Tile3 tl = tile < kNumTiles3 ? kTiles3[tile] : kTilesP[tile - kNumTiles3];
+        const long long ncols = (long long)cdiv(G->n[0], tl.tx) * cdiv(G->n[1], tl.ty);
+        const int bpc = dtype == PHX_F64 ? blocks_per_cu<double, true>(tile) : blocks_per_cu<float, false>(tile);
+        for (int k = 1; k <= 3 && bpc > 0; ++k) {
+            const long long nch = std::max(1LL, (long long)k * bpc * ncu / ncols);
+            const int zc = (int)cdiv(G->n[2], (int)std::min<long long>(nch, G->n[2]));
+            if (std::find(zcs.begin(), zcs.end(), zc) == zcs.end()) zcs.push_back(zc);
+        }
+    }
+    for (int zc : zcs) {
         if (G->dim == 3 && zc > G->n[2]) continue;
         if (G->dim != 3 && zc != cand[0]) continue;
         const Launch3 l{G->dim == 3 ? zc : 1, tile};
@@ -1631,6 +1787,7 @@ int phx_ch_rk4_autotune(const phx_grid* G, int dtype, void* c, void* work, const
             bz = l.zc;
             bt = tile;
         }
+    }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

@@ -61,6 +61,17 @@ class PlainSolver:
             capi.call("phx_ch_rk4_steps", C.byref(self.g), self.dtype, capi.ptr(self.c), capi.ptr(self.work),
                       C.byref(self.p), int(k), capi.stream_handle())
 
+    def autotune(self, steps=20):
+        """Choose the fastest RK4 launch for this shape (kept per shape by the
+        library); advances a scratch copy of the state."""
+        import torch
+        scratch = self.c.clone()
+        zc, ms = C.c_int(), C.c_double()
+        capi.call("phx_ch_rk4_autotune", C.byref(self.g), self.dtype, capi.ptr(scratch), capi.ptr(self.work),
+                  C.byref(self.p), steps, C.byref(zc), C.byref(ms), capi.stream_handle())
+        torch.cuda.synchronize()
+        return ms.value
+
     def mass_energy(self):
         capi.call("phx_mass_energy", C.byref(self.g), self.dtype, capi.ptr(self.c), C.byref(self.p),
                   capi.ptr(self.red), capi.ptr(self.out), capi.stream_handle())
@@ -69,7 +80,17 @@ class PlainSolver:
 
 
 # ---------------------------------------------------------------- data ------
-def generate_pairs(dev, ntraj, nsnap, seed, n=256, kappa=0.5, dt=0.01, every=100, amp=0.05, skip=0):
+def generate_snapshots(dev, ntraj, nsnap, seed, n=256, kappa=0.5, dt=0.01, every=100, amp=0.05, skip=0):
+    """(nsnap+1, ntraj, n, n) FP32 device tensor of snapshots c(t_k),
+    t_k = (skip + k)*every*dt, k = 0..nsnap, of ntraj runs from
+    c0 = 0.5 + U(-amp, amp) (SplitMix64 substream per trajectory), advanced by
+    the fused FP32 RK4 ensemble kernel (hipGraph step loop)."""
+    x, _ = generate_pairs(dev, ntraj, nsnap, seed, n, kappa, dt, every, amp, skip, keep_snaps=True)
+    return x
+
+
+def generate_pairs(dev, ntraj, nsnap, seed, n=256, kappa=0.5, dt=0.01, every=100, amp=0.05, skip=0,
+                   keep_snaps=False):
     """Returns (inputs, targets): (ntraj*nsnap, n, n) FP32 device tensors of
     consecutive snapshots c(t_k), c(t_k + every*dt), t_k = (skip + k)*every*dt,
     from c0 = 0.5 + U(-amp, amp) (SplitMix64 substream per trajectory)."""
@@ -92,6 +113,8 @@ def generate_pairs(dev, ntraj, nsnap, seed, n=256, kappa=0.5, dt=0.01, every=100
         for k in range(nsnap):
             capi.call("phx_ensemble_graph_steps", h, capi.ptr(c), capi.ptr(work), dt, every, 1, sh)
             snaps[k + 1] = c.view(ntraj, n, n)
+        if keep_snaps:
+            return snaps, None
         inputs = snaps[:-1].reshape(-1, n, n).contiguous()
         targets = snaps[1:].reshape(-1, n, n).contiguous()
         return inputs, targets
@@ -177,6 +200,75 @@ def train(dev, inputs, targets, batch=8, epochs=1, lr=1e-3, seed=0, params=None,
     return net, hist
 
 
+def train_rollout(dev, snaps, batch=16, epochs=1, lr=1e-3, seed=0, params=None, log=None, max_seconds=None,
+                  noise=1e-3, pushforward=0.5, warm_epochs=1):
+    """Training against rollout drift on a snapshot tensor (K+1, R, n, n).
+
+    Each batch draws (k, r) pairs. One-step samples take c_k + N(0, noise^2)
+    as input and c_{k+1} as target, so the network learns to remove small
+    perturbations of the kind its own outputs carry. After `warm_epochs`, a
+    fraction `pushforward` of the batches is two-step: the input is the
+    network's own prediction from c_k (no gradient through it) and the target
+    is c_{k+2}, so the network is trained on the distribution it meets in a
+    rollout. Cosine learning-rate decay; returns (Surrogate, mean loss per
+    epoch of the one-step batches)."""
+    import torch
+    K1, R, n, _ = snaps.shape
+    net = Surrogate(dev, batch, n=n, seed=seed, params=params)
+    rng = np.random.default_rng(seed)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    npairs = (K1 - 1) * R
+    nbatch = max(1, npairs // batch)
+    hist = []
+    t0 = time.time()
+    flat = snaps.reshape(K1 * R, n, n)
+    for ep in range(epochs):
+        acc = torch.zeros(1, dtype=torch.float64, device=dev)
+        cnt = 0
+        for b in range(nbatch):
+            lr_now = lr * 0.5 * (1 + math.cos(math.pi * (ep * nbatch + b) / (epochs * nbatch)))
+            two = ep >= warm_epochs and rng.random() < pushforward
+            kmax = K1 - 2 if two else K1 - 1
+            k = rng.integers(0, kmax, batch)
+            r = rng.integers(0, R, batch)
+            src = torch.from_numpy(k * R + r).to(dev)
+            x = flat.index_select(0, src).contiguous()
+            if two:
+                x = net.forward(x).clone()
+                y = flat.index_select(0, src + 2 * R).contiguous()
+                net.train_step(x, y, lr_now)
+            else:
+                if noise > 0:
+                    x = x + noise * torch.randn(x.shape, device=dev, generator=gen)
+                y = flat.index_select(0, src + R).contiguous()
+                acc += net.train_step(x, y, lr_now)
+                cnt += 1
+            if max_seconds and time.time() - t0 > max_seconds:
+                break
+        hist.append(acc.item() / max(1, cnt * batch * n * n))
+        if log:
+            log(f"epoch {ep}: one-step mse {hist[-1]:.3e} ({time.time() - t0:.1f} s)")
+        if max_seconds and time.time() - t0 > max_seconds:
+            break
+    return net, hist
+
+
+def rollout_error(dev, params, snaps, traj, jumps):
+    """Relative L2 of a pure surrogate rollout of `jumps` jumps against the
+    solver snapshots of trajectory `traj`, after each jump."""
+    import torch
+    n = snaps.shape[-1]
+    net = Surrogate(dev, 1, n=n, params=params)
+    cur = snaps[0, traj].reshape(1, n, n).contiguous().clone()
+    errs = []
+    for j in range(jumps):
+        cur = net.forward(cur, reuse_weights=j > 0).clone()
+        ref = snaps[j + 1, traj]
+        errs.append(((cur[0] - ref).norm() / ref.norm()).item())
+    return errs
+
+
 # --------------------------------------------------------------- hybrid -----
 @dataclass
 class HybridReport:
@@ -245,9 +337,125 @@ class HybridSolver:
         return cur[0], HybridReport(time.perf_counter() - t0, blocks, rejected, ssteps, vsteps, log)
 
 
+# ------------------------------------------------------------- parareal -----
+@dataclass
+class PararealReport:
+    seconds: float
+    iterations: int
+    slices: int
+    surrogate_steps: int
+    solver_steps: int
+    increments: list = field(default_factory=list)
+    energy_ok: bool = True
+
+
+class PararealSolver:
+    """Surrogate corrected by the solver in parallel in time (parareal).
+
+    The run [0, T] is cut into `slices` slices of `slice_steps` solver steps.
+    The surrogate is the coarse propagator G (slice_steps/every jumps per
+    slice), the fused FP32 RK4 solver the fine propagator F. F is applied to
+    every slice at once as one ensemble launch (the persistent ensemble kernel
+    with a hipGraph step loop) -- on one GPU a 256^2 run leaves most of the
+    chip idle, so all slices cost about what one does. Each iteration then
+    sweeps the slices in order:
+        U[s+1] <- G(U_new[s]) + F(U_old[s]) - G(U_old[s]),
+    which makes the first k slices exact after k iterations; the surrogate's
+    drift is removed by the solver's corrections, not accepted. The judge:
+    the relative change of the final state between iterations (the parareal
+    increment) and the free energy, which must not rise between slice
+    boundaries; an iteration whose states violate it is not accepted and the
+    solver's F states of the previous iteration are used instead. Every term
+    keeps the mean composition (G projects to its input's mean, F conserves
+    mass), so the corrected states do too."""
+
+    def __init__(self, dev, params, slices, slice_steps, n=256, kappa=0.5, dt=0.01, every=100):
+        import torch
+        assert slice_steps % every == 0
+        self.dev, self.n, self.dt, self.every = dev, n, dt, every
+        self.S, self.steps = slices, slice_steps
+        self.net = Surrogate(dev, 1, n=n, params=params)
+        self.packed = False
+        h = C.c_void_p()
+        sides = (C.c_int * slices)(*([n] * slices))
+        kap = (C.c_double * slices)(*([kappa] * slices))
+        capi.call("phx_ensemble_create", slices, sides, kap, None, C.byref(h))
+        self.h = h
+        cells = capi.lib().phx_ensemble_cells(h)
+        self.arena = torch.empty((slices, n, n), dtype=torch.float32, device=dev)
+        self.work = torch.empty(3 * cells, dtype=torch.float32, device=dev)
+        self.judge = PlainSolver(dev, n=n, kappa=kappa, dt=dt)
+
+    def close(self):
+        if self.h:
+            capi.lib().phx_ensemble_destroy(self.h)
+            self.h = None
+
+    def G(self, u):
+        for _ in range(self.steps // self.every):
+            u = self.net.forward(u, reuse_weights=self.packed).clone()
+            self.packed = True
+        return u
+
+    def F_all(self, U):
+        for s in range(self.S):
+            self.arena[s].copy_(U[s][0])
+        per = min(self.steps, 100)
+        capi.call("phx_ensemble_graph_steps", self.h, capi.ptr(self.arena), capi.ptr(self.work), self.dt, per,
+                  self.steps // per, capi.stream_handle())
+        return self.arena.clone()
+
+    def energies(self, U):
+        es = []
+        for u in U:
+            self.judge.set(u[0])
+            es.append(self.judge.mass_energy()[1])
+        return es
+
+    def run(self, c0, iters):
+        """Advances c0 (n x n FP32 device tensor) by slices*slice_steps solver
+        steps with `iters` parareal iterations; returns (field, report)."""
+        import torch
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        n = self.n
+        U = [c0.reshape(1, n, n).contiguous().clone()]
+        Gp = []
+        for s in range(self.S):
+            g = self.G(U[s])
+            Gp.append(g)
+            U.append(g)
+        incs = []
+        ok = True
+        for _ in range(iters):
+            Fk = self.F_all(U)
+            newU = [U[0]]
+            for s in range(self.S):
+                g = self.G(newU[s])
+                newU.append(g + Fk[s:s + 1] - Gp[s])
+                Gp[s] = g
+            incs.append(((newU[-1] - U[-1]).norm() / newU[-1].norm()).item())
+            es = self.energies(newU[1:])
+            if any(b > a * (1 + 1e-6) for a, b in zip(es, es[1:])):
+                ok = False   # fallback: the solver's states of this iteration
+                newU = [U[0]] + [Fk[s:s + 1].clone() for s in range(self.S)]
+            U = newU
+        torch.cuda.synchronize()
+        jumps = self.S * (self.steps // self.every) * (1 + iters)
+        return U[-1][0], PararealReport(time.perf_counter() - t0, iters, self.S, jumps,
+                                        iters * self.S * self.steps, incs, ok)
+
+
 # ------------------------------------------------------------------ CLI -----
 DATA = __import__("pathlib").Path(__file__).resolve().parent / "data"
 WEIGHTS = DATA / "surrogate_ch256.npz"
+# How the committed weights were made (on one MI355X, ~22 GPU-minutes):
+WEIGHTS_RECIPE = (
+    "python -m phix_amd.surrogate train --ntraj 256 --nsnap 200 --epochs 100 --batch 16 --lr 1e-3 "
+    "--seconds 600 --out stage1.npz",
+    "python -m phix_amd.surrogate train-rollout --init stage1.npz --ntraj 256 --nsnap 200 --epochs 40 "
+    "--batch 16 --lr 3e-4 --seconds 900 --noise 1e-3 --pushforward 0.5 --out phix_amd/data/surrogate_ch256.npz",
+)
 
 
 def fp64_reference(dev, c0_np, nsteps, n=256, kappa=0.5, dt=0.01):
@@ -265,7 +473,7 @@ def main(argv=None):
     import json
     import torch
     ap = argparse.ArgumentParser(description="train or evaluate the CH surrogate")
-    ap.add_argument("mode", choices=["train", "eval"])
+    ap.add_argument("mode", choices=["train", "train-rollout", "eval"])
     ap.add_argument("--ntraj", type=int, default=64)
     ap.add_argument("--nsnap", type=int, default=40)
     ap.add_argument("--epochs", type=int, default=4)
@@ -277,9 +485,31 @@ def main(argv=None):
     ap.add_argument("--out", default=str(WEIGHTS))
     ap.add_argument("--tol", type=float, default=5e-3)
     ap.add_argument("--check-every", type=int, default=10)
+    ap.add_argument("--noise", type=float, default=1e-3)
+    ap.add_argument("--pushforward", type=float, default=0.5)
+    ap.add_argument("--init", default=None, help="start training from these weights")
+    ap.add_argument("--frontier", action="store_true", help="eval at several tolerances")
+    ap.add_argument("--parareal", action="store_true", help="eval the parareal form")
+    ap.add_argument("--slices", type=int, default=40)
     a = ap.parse_args(argv)
     dev = torch.device("cuda:0")
-    if a.mode == "train":
+    if a.mode == "train-rollout":
+        t0 = time.time()
+        snaps = generate_snapshots(dev, a.ntraj, a.nsnap, seed=a.seed)
+        torch.cuda.synchronize()
+        print(f"data: {a.ntraj} trajectories x {a.nsnap + 1} snapshots in {time.time() - t0:.1f} s", flush=True)
+        init_p = np.load(a.init)["params"] if a.init else None
+        net, hist = train_rollout(dev, snaps, batch=a.batch, epochs=a.epochs, lr=a.lr, params=init_p,
+                                  log=lambda s: print(s, flush=True), max_seconds=a.seconds, noise=a.noise,
+                                  pushforward=a.pushforward, warm_epochs=0 if a.init else 1)
+        DATA.mkdir(exist_ok=True)
+        np.savez(a.out, params=net.params_numpy(), mse=np.array(hist), ntraj=a.ntraj, nsnap=a.nsnap,
+                 seed=a.seed, noise=a.noise, pushforward=a.pushforward)
+        test = generate_snapshots(dev, 2, 200, seed=a.seed + 1000)
+        errs = rollout_error(dev, net.params_numpy(), test, 0, 200)
+        print(json.dumps({"saved": a.out, "mse": hist,
+                          "rollout_rel_l2": {j: errs[j - 1] for j in (1, 10, 50, 100, 200)}}), flush=True)
+    elif a.mode == "train":
         t0 = time.time()
         xs, ys = generate_pairs(dev, a.ntraj, a.nsnap, seed=a.seed)
         torch.cuda.synchronize()
@@ -294,13 +524,9 @@ def main(argv=None):
         p = np.load(a.out)["params"]
         c0 = init.uniform(init.substream(1234, 7), 256 * 256, 0.45, 0.55).reshape(256, 256)
         ref = fp64_reference(dev, c0, a.tend_steps)
-        hyb = HybridSolver(dev, p, tol=a.tol, check_every=a.check_every)
-        hyb.run(torch.from_numpy(c0).float().to(dev), 10 * hyb.every)   # warm-up (code paths, weight packing)
-        out, rep = hyb.run(torch.from_numpy(c0).float().to(dev), a.tend_steps)
-        for b in rep.log:
-            print(f"block {b[0]}: one-step residual {b[1]:.3e} energy {b[2]:.6e} accepted {b[3]}", flush=True)
-        err = float(np.linalg.norm(out.cpu().numpy() - ref) / np.linalg.norm(ref))
         plain = PlainSolver(dev)
+        plain.set(torch.from_numpy(c0).float().to(dev))
+        plain.autotune()
         plain.set(torch.from_numpy(c0).float().to(dev))
         plain.steps(100)
         plain.set(torch.from_numpy(c0).float().to(dev))
@@ -310,10 +536,44 @@ def main(argv=None):
         torch.cuda.synchronize()
         tp = time.perf_counter() - t0
         perr = float(np.linalg.norm(plain.phys().cpu().numpy() - ref) / np.linalg.norm(ref))
-        print(json.dumps({"t_end_steps": a.tend_steps, "tol": a.tol, "hybrid_s": rep.seconds, "hybrid_rel_l2": err,
-                          "rejected": rep.rejected, "blocks": rep.blocks, "surrogate_steps": rep.surrogate_steps,
-                          "solver_steps": rep.solver_steps, "plain_s": tp, "plain_rel_l2": perr,
-                          "speedup": tp / rep.seconds}), flush=True)
+        if a.parareal:
+            runs = []
+            for slices in sorted({a.slices, 20, 40}):
+                P = PararealSolver(dev, p, slices, a.tend_steps // slices)
+                x0 = torch.from_numpy(c0).float().to(dev)
+                P.run(x0, 1)   # warm-up: graph capture, weight packing
+                for iters in (1, 2, 3):
+                    out, rep = P.run(x0, iters)
+                    err = float(np.linalg.norm(out.cpu().numpy() - ref) / np.linalg.norm(ref))
+                    r = {"slices": slices, "iters": iters, "seconds": rep.seconds, "rel_l2": err,
+                         "increments": rep.increments, "energy_ok": rep.energy_ok,
+                         "surrogate_jumps": rep.surrogate_steps, "solver_steps": rep.solver_steps,
+                         "speedup": tp / rep.seconds}
+                    print(json.dumps(r), flush=True)
+                    runs.append(r)
+                P.close()
+            print(json.dumps({"t_end_steps": a.tend_steps, "plain_s": tp, "plain_rel_l2": perr, "parareal": runs}),
+                  flush=True)
+            return
+        settings = [(a.tol, a.check_every)]
+        if a.frontier:
+            settings = [(t, ce) for ce in (10, 5) for t in (1e-3, 2e-3, 5e-3, 1e-2, 3e-2)]
+        runs = []
+        for tol, ce in settings:
+            hyb = HybridSolver(dev, p, tol=tol, check_every=ce)
+            hyb.run(torch.from_numpy(c0).float().to(dev), 10 * hyb.every)   # warm-up (code paths, weight packing)
+            out, rep = hyb.run(torch.from_numpy(c0).float().to(dev), a.tend_steps)
+            if not a.frontier:
+                for b in rep.log:
+                    print(f"block {b[0]}: one-step residual {b[1]:.3e} energy {b[2]:.6e} accepted {b[3]}", flush=True)
+            err = float(np.linalg.norm(out.cpu().numpy() - ref) / np.linalg.norm(ref))
+            r = {"tol": tol, "check_every": ce, "hybrid_s": rep.seconds, "hybrid_rel_l2": err,
+                 "rejected": rep.rejected, "blocks": rep.blocks, "surrogate_steps": rep.surrogate_steps,
+                 "solver_steps": rep.solver_steps, "speedup": tp / rep.seconds}
+            print(json.dumps(r), flush=True)
+            runs.append(r)
+        print(json.dumps({"t_end_steps": a.tend_steps, "plain_s": tp, "plain_rel_l2": perr, "runs": runs}),
+              flush=True)
 
 
 if __name__ == "__main__":

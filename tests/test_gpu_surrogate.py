@@ -219,28 +219,56 @@ def test_adam_and_training_reduces_loss(dev):
 
 # ---------------------------------------------------- learned part: the gain
 def test_trained_surrogate_gains_over_untrained(dev):
-    """With the committed trained weights the solver-judged hybrid accepts
-    surrogate blocks (fewer solver steps than the plain solver) and still meets
-    the 1e-2 relative-L2 bound against the FP64 solver; the same hybrid with an
-    untrained (identity) network is rejected on every block and gains nothing."""
+    """What the trained network adds, on a 256^2 periodic CH run of 6000 RK4
+    steps (dt = 0.01) against the FP64 solver:
+      * solver-judged rollout (blocks of 10 jumps of 100 steps, tol 2e-3): the
+        trained network's blocks are accepted, so it needs far fewer solver
+        steps than the plain solver; an untrained (identity) network is
+        rejected on every block and falls back to the solver throughout;
+      * parareal (20 slices, one iteration, the solver as fine propagator):
+        the trained network as coarse propagator ends at least 3x closer to
+        the FP64 solution than the identity map does.
+    Both forms keep the mean composition to the FP32 solver's own round-off
+    (increments below one ulp in the bulk phases are lost in any FP32 step,
+    so the mean drifts by ~1e-6 over thousands of steps)."""
     import torch
     from phix_amd import surrogate as S
     if not S.WEIGHTS.exists():
         pytest.skip("no trained surrogate weights committed yet")
     params = np.load(S.WEIGHTS)["params"]
+    untrained = S.Surrogate(dev, 1).init_params(0)
     nsteps = 6000
     c0 = S.init.uniform(S.init.substream(4321, 7), 256 * 256, 0.45, 0.55).reshape(256, 256)
     ref = S.fp64_reference(dev, c0, nsteps)
     x0 = torch.from_numpy(c0).float().to(dev)
-    hyb = S.HybridSolver(dev, params)
+
+    def rel(out):
+        return float(np.linalg.norm(out.cpu().numpy() - ref) / np.linalg.norm(ref))
+
+    hyb = S.HybridSolver(dev, params, tol=2e-3)
     out, rep = hyb.run(x0, nsteps)
-    err = float(np.linalg.norm(out.cpu().numpy() - ref) / np.linalg.norm(ref))
-    untrained = S.Surrogate(dev, 1).init_params(0)
-    hyb0 = S.HybridSolver(dev, untrained)
+    hyb0 = S.HybridSolver(dev, untrained, tol=2e-3)
     out0, rep0 = hyb0.run(x0, nsteps)
-    err0 = float(np.linalg.norm(out0.cpu().numpy() - ref) / np.linalg.norm(ref))
-    print(f"trained: {rep.solver_steps} solver steps, {rep.rejected}/{rep.blocks} blocks rejected, rel-L2 {err:.3e}; "
-          f"untrained: {rep0.solver_steps} solver steps, {rep0.rejected}/{rep0.blocks} rejected, rel-L2 {err0:.3e}")
-    assert err <= 1e-2 and err0 <= 1e-2
+    print(f"judged rollout -- trained: {rep.solver_steps} solver steps, {rep.rejected}/{rep.blocks} rejected, "
+          f"rel-L2 {rel(out):.3e}; untrained: {rep0.solver_steps} solver steps, {rep0.rejected}/{rep0.blocks} "
+          f"rejected, rel-L2 {rel(out0):.3e}")
     assert rep0.rejected == rep0.blocks
-    assert rep.solver_steps < rep0.solver_steps
+    assert rep.rejected < rep.blocks
+    assert rep.solver_steps <= nsteps // 2 < rep0.solver_steps
+    assert abs(out.double().mean().item() - c0.mean()) < 1e-6
+
+    P = S.PararealSolver(dev, params, 20, nsteps // 20)
+    outp, repp = P.run(x0, 1)
+    P.close()
+    P0 = S.PararealSolver(dev, untrained, 20, nsteps // 20)
+    outp0, repp0 = P0.run(x0, 1)
+    P0.close()
+    print(f"parareal, 1 iteration -- trained rel-L2 {rel(outp):.3e}, untrained {rel(outp0):.3e}")
+    assert rel(outp) * 3 <= rel(outp0)
+    plain = S.PlainSolver(dev)
+    plain.set(x0)
+    plain.steps(nsteps)
+    drift_plain = abs(plain.phys().double().mean().item() - c0.mean())
+    drift_pr = abs(outp.double().mean().item() - c0.mean())
+    print(f"mean drift vs FP64 c0 -- plain FP32 solver {drift_plain:.2e}, parareal {drift_pr:.2e}")
+    assert drift_pr <= 5 * max(drift_plain, 1e-7)
