@@ -49,65 +49,138 @@ struct EnsArgs {
 // Periodic wrap for |p - n| < n.
 __device__ __forceinline__ int wrapp(int p, int n) { return p < 0 ? p + n : (p >= n ? p - n : p); }
 
+// One persistent workgroup walks tiles w, w + G, w + 2G, ... (G = grid size)
+// and is software pipelined across them: the next tile's halo is in flight
+// (registers) while the current tile's mu is computed, its epilogue operands
+// while the current tile's outputs are written, and tile descriptors two tiles
+// ahead. Every load is unconditional (wrapped, clamped offsets; clamped tile
+// indices past the end re-read a valid tile), so each iteration issues the
+// same loads and the compiler waits only for the ones it uses.
 template <int TX, int TY>
 __global__ __launch_bounds__(256) void k_ens_stage(EnsArgs A) {
     constexpr int UX = TX + 4, UY = TY + 4, MX = TX + 2, MY = TY + 2;
-    __shared__ float U[UY][UX];
-    __shared__ float MU[MY][MX];
+    constexpr int NS = (UX * UY + 255) / 256;
+    constexpr int NO = (TX * TY) / 256;
+    static_assert((TX * TY) % 256 == 0, "tile must be a multiple of 256 cells");
+    __shared__ float U[UY * UX];
+    __shared__ float MU[MY * MX];
     const int tid = threadIdx.x;
-    for (int w = blockIdx.x; w < A.ntiles; w += gridDim.x) {
-        const EnsTile T = A.tiles[w];
-        const EnsRun R = A.runs[T.run];
+    const int G = gridDim.x, last = A.ntiles - 1;
+    int w = blockIdx.x;
+    if (w > last) return;
+    // epilogue sources: an operand the stage does not need is read from the
+    // stage state instead (in cache from the halo load)
+    const int st = A.stage;
+    const float* csrc = (st == 2 || st == 3) ? A.c : A.u;
+    const float* asrc = st >= 2 ? A.acc : A.u;
+
+    float h[NS], pc[NO], pa[NO];
+    auto load_halo = [&](const EnsTile& T, const EnsRun& R) {
+        const int n = R.n;
+        const float* u = A.u + R.off;
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+            const int t = min(tid + 256 * i, UX * UY - 1);
+            const int ly = t / UX, lx = t - ly * UX;
+            const int px = min(max(T.x0 - 2 + lx, -2), n + 1), py = min(max(T.y0 - 2 + ly, -2), n + 1);
+            h[i] = u[(long long)wrapp(py, n) * n + wrapp(px, n)];
+        }
+    };
+    auto load_epi = [&](const EnsTile& T, const EnsRun& R) {
+        const int n = R.n;
+#pragma unroll
+        for (int j = 0; j < NO; ++j) {
+            const int t = tid + 256 * j;
+            const int oy = t / TX, ox = t - oy * TX;
+            const long long gi = R.off + (long long)min(T.y0 + oy, n - 1) * n + min(T.x0 + ox, n - 1);
+            pc[j] = csrc[gi];
+            pa[j] = asrc[gi];
+        }
+    };
+
+    // tile and run indices are workgroup-uniform: in scalar registers the
+    // descriptor loads are scalar loads, whose waits do not drain the vector
+    // loads in flight
+    auto uidx = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
+    // the tables are read-only during the launch: read them through the
+    // constant address space, which the compiler serves with scalar loads
+    using CTile = const __attribute__((address_space(4))) EnsTile;
+    using CRun = const __attribute__((address_space(4))) EnsRun;
+    CTile* ctiles = (CTile*)A.tiles;
+    CRun* cruns = (CRun*)A.runs;
+    auto tile_at = [&](int i) {
+        EnsTile t;
+        t.run = ctiles[i].run;
+        t.x0 = ctiles[i].x0;
+        t.y0 = ctiles[i].y0;
+        return t;
+    };
+    auto run_at = [&](int i) {
+        EnsRun r;
+        r.off = cruns[i].off;
+        r.n = cruns[i].n;
+        r.kappa = cruns[i].kappa;
+        r.M = cruns[i].M;
+        return r;
+    };
+    EnsTile T = tile_at(uidx(w));
+    EnsRun R = run_at(uidx(T.run));
+    EnsTile Tn = tile_at(uidx(min(w + G, last)));
+    EnsRun Rn = run_at(uidx(Tn.run));
+    load_halo(T, R);
+    load_epi(T, R);
+    for (;;) {
+        // descriptors two tiles ahead
+        const EnsTile T2 = tile_at(uidx(min(w + 2 * G, last)));
+        const EnsRun R2 = run_at(uidx(T2.run));
+#pragma unroll
+        for (int i = 0; i < NS; ++i)
+            if (tid + 256 * i < UX * UY) U[tid + 256 * i] = h[i];
+        __syncthreads();
+        load_halo(Tn, Rn);   // next tile's halo, in flight during mu and the outputs
         const int n = R.n;
         const long long off = R.off;
-        const bool interior = T.x0 >= 2 && T.x0 + TX + 1 < n && T.y0 >= 2 && T.y0 + TY + 1 < n;
-        const float* u = A.u + off;
-        if (interior) {
-            const float* base = u + (long long)(T.y0 - 2) * n + (T.x0 - 2);
-            for (int t = tid; t < UX * UY; t += 256) {
-                int ly = t / UX, lx = t - ly * UX;
-                U[ly][lx] = base[(long long)ly * n + lx];
-            }
-        } else {
-            for (int t = tid; t < UX * UY; t += 256) {
-                int ly = t / UX, lx = t - ly * UX;
-                int px = T.x0 - 2 + lx, py = T.y0 - 2 + ly;
-                px = min(max(px, -2), n + 1);
-                py = min(max(py, -2), n + 1);
-                U[ly][lx] = u[(long long)wrapp(py, n) * n + wrapp(px, n)];
-            }
-        }
-        __syncthreads();
         const float nk = -R.kappa;
         for (int t = tid; t < MX * MY; t += 256) {
             int my = t / MX, mx = t - my * MX;
-            float cc = U[my + 1][mx + 1];
-            float lap = (U[my + 1][mx + 2] - 2.f * cc + U[my + 1][mx]) + (U[my + 2][mx + 1] - 2.f * cc + U[my][mx + 1]);
+            const int c0 = (my + 1) * UX + mx + 1;
+            float cc = U[c0];
+            float lap = (U[c0 + 1] - 2.f * cc + U[c0 - 1]) + (U[c0 + UX] - 2.f * cc + U[c0 - UX]);
             // f'(c) = 2c(c-1)(2c-1) for f = c^2(1-c)^2
             float fp = 2.f * cc * (cc - 1.f) * (2.f * cc - 1.f);
-            MU[my][mx] = fp + nk * lap;
+            MU[t] = fp + nk * lap;
         }
         __syncthreads();
-        for (int t = tid; t < TX * TY; t += 256) {
-            int oy = t / TX, ox = t - oy * TX;
-            int x = T.x0 + ox, y = T.y0 + oy;
+#pragma unroll
+        for (int j = 0; j < NO; ++j) {
+            const int t = tid + 256 * j;
+            const int oy = t / TX, ox = t - oy * TX;
+            const int x = T.x0 + ox, y = T.y0 + oy;
             if (x < n && y < n) {
-                float m = MU[oy + 1][ox + 1];
-                float lm = (MU[oy + 1][ox + 2] - 2.f * m + MU[oy + 1][ox]) + (MU[oy + 2][ox + 1] - 2.f * m + MU[oy][ox + 1]);
+                const int m0 = (oy + 1) * MX + ox + 1;
+                float m = MU[m0];
+                float lm = (MU[m0 + 1] - 2.f * m + MU[m0 - 1]) + (MU[m0 + MX] - 2.f * m + MU[m0 - MX]);
                 float k = R.M * lm;
                 long long gi = off + (long long)y * n + x;
-                float uc = U[oy + 2][ox + 2];
-                if (A.stage == 1) {
+                float uc = U[(oy + 2) * UX + ox + 2];
+                if (st == 1) {
                     A.acc[gi] = uc + A.dt6 * k;
                     A.out[gi] = uc + A.a * k;
-                } else if (A.stage < 4) {
-                    A.acc[gi] = A.acc[gi] + A.dt3 * k;
-                    A.out[gi] = A.c[gi] + A.a * k;
+                } else if (st < 4) {
+                    A.acc[gi] = pa[j] + A.dt3 * k;
+                    A.out[gi] = pc[j] + A.a * k;
                 } else {
-                    A.out[gi] = A.acc[gi] + A.dt6 * k;
+                    A.out[gi] = pa[j] + A.dt6 * k;
                 }
             }
         }
+        w += G;
+        if (w > last) break;
+        load_epi(Tn, Rn);   // next tile's epilogue operands
+        T = Tn;
+        R = Rn;
+        Tn = T2;
+        Rn = R2;
         __syncthreads();   // LDS is reused by the next tile of this workgroup
     }
 }
