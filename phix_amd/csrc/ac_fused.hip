@@ -35,18 +35,6 @@ struct ACArgs {
     int stage;
 };
 
-template <typename T, int DIM>
-__device__ __forceinline__ T ac_load(const T* __restrict__ u, const ACArgs<T>& A, int px, int py) {
-    int qy = 0;
-    if (DIM >= 2) {
-        qy = map_in(py, A.by);
-        if (qy < 0) return fixed_value(py, A.by);
-    }
-    int qx = map_in(px, A.bx);
-    if (qx < 0) return fixed_value(px, A.bx);
-    return u[A.L.at(qx, qy, 0)];
-}
-
 template <typename T, int DIM, int NF, int TX, int TY, bool EXACT>
 __global__ __launch_bounds__(256) void k_ac_stage_2d(ACArgs<T> A) {
     constexpr int UX = TX + 2, UY = TY + 2;
@@ -55,33 +43,71 @@ __global__ __launch_bounds__(256) void k_ac_stage_2d(ACArgs<T> A) {
     __shared__ T U[NF][UY * UX];
     const int tid = threadIdx.x;
     const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
-    const bool interior = (x0 >= 1) && (x0 + TX < A.L.n0) && (DIM < 2 || ((y0 >= 1) && (y0 + TY < A.L.n1)));
 
-    // every field's halo tile is requested before the single barrier, so the
-    // NF global latencies overlap
-    if (interior) {
-        const long long b0 = A.L.at(x0 - 1, y0 - 1, 0);
+    // Every load of the block -- all fields' halo tiles (offsets with the BC
+    // maps computed first) and the stage epilogue's c_n and accumulator
+    // values -- is issued unconditionally, from valid cells, before the single
+    // barrier, so one global latency covers them all. FIXED halo slots and
+    // cells beyond the domain are replaced or skipped afterwards.
+    const int g = A.L.g;
+    const long long zb = (long long)g * A.L.sxy;
+    const int cell0 = g * A.L.sx + g;
+    int off[NS];
+    unsigned fm = 0;
+    T fv[NS];
 #pragma unroll
-        for (int i = 0; i < NS; ++i) {
-            const int t = tid + 256 * i;
-            if (t < UX * UY) {
-                const int ly = t / UX, lx = t - ly * UX;
-                const long long off = b0 + (long long)ly * A.L.sx + lx;
-#pragma unroll
-                for (int f = 0; f < NF; ++f) U[f][t] = A.u[f][off];
-            }
-        }
-    } else {
-#pragma unroll
-        for (int i = 0; i < NS; ++i) {
-            const int t = tid + 256 * i;
-            if (t < UX * UY) {
-                const int ly = t / UX, lx = t - ly * UX;
-#pragma unroll
-                for (int f = 0; f < NF; ++f) U[f][t] = ac_load<T, DIM>(A.u[f], A, x0 - 1 + lx, DIM >= 2 ? y0 - 1 + ly : 0);
-            }
+    for (int i = 0; i < NS; ++i) {
+        const int t = min(tid + 256 * i, UX * UY - 1);
+        const int ly = t / UX, lx = t - ly * UX;
+        const int px = x0 - 1 + lx, py = DIM >= 2 ? y0 - 1 + ly : 0;
+        const int qy = DIM >= 2 ? map_in(py, A.by) : 0, qx = map_in(px, A.bx);
+        off[i] = cell0;
+        fv[i] = (T)0;
+        if (qy < 0) {
+            fm |= 1u << i;
+            fv[i] = fixed_value(py, A.by);
+        } else if (qx < 0) {
+            fm |= 1u << i;
+            fv[i] = fixed_value(px, A.bx);
+        } else {
+            off[i] = (qy + g) * A.L.sx + (qx + g);
         }
     }
+    T hr[NF][NS];
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int i = 0; i < NS; ++i) hr[f][i] = A.u[f][zb + off[i]];
+    // epilogue operands; a stage that does not need one reads the stage state
+    const int st = A.stage;
+    const bool need_c = st == 2 || st == 3 || (EXACT && st == 4);
+    const bool need_a = st >= 2;
+    T pc[NF][R], pa[NF][R];
+    long long ogi[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int t = tid + r * 256;
+        const int oy = t / TX, ox = t - oy * TX;
+        const int x = min(x0 + ox, A.L.n0 - 1), y = min(y0 + oy, A.L.n1 - 1);
+        ogi[r] = zb + (long long)(y + g) * A.L.sx + (x + g);
+    }
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        const T* cs = need_c ? A.c[f] : A.u[f];
+        const T* as = need_a ? A.acc[f] : A.u[f];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            pc[f][r] = cs[ogi[r]];
+            pa[f][r] = as[ogi[r]];
+        }
+    }
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+            const int t = tid + 256 * i;
+            if (t < UX * UY) U[f][t] = ((fm >> i) & 1u) ? fv[i] : hr[f][i];
+        }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -116,20 +142,20 @@ __global__ __launch_bounds__(256) void k_ac_stage_2d(ACArgs<T> A) {
                     A.acc[f][gi] = k;
                     A.out[f][gi] = ef + A.a * k;
                 } else if (s < 4) {
-                    A.acc[f][gi] = A.acc[f][gi] + (T)2 * k;
-                    A.out[f][gi] = A.c[f][gi] + A.a * k;
+                    A.acc[f][gi] = pa[f][r] + (T)2 * k;
+                    A.out[f][gi] = pc[f][r] + A.a * k;
                 } else {
-                    A.out[f][gi] = A.c[f][gi] + A.dt6 * (A.acc[f][gi] + k);
+                    A.out[f][gi] = pc[f][r] + A.dt6 * (pa[f][r] + k);
                 }
             } else {
                 if (s == 1) {
                     A.acc[f][gi] = ef + A.dt6 * k;
                     A.out[f][gi] = ef + A.a * k;
                 } else if (s < 4) {
-                    A.acc[f][gi] = A.acc[f][gi] + A.dt3 * k;
-                    A.out[f][gi] = A.c[f][gi] + A.a * k;
+                    A.acc[f][gi] = pa[f][r] + A.dt3 * k;
+                    A.out[f][gi] = pc[f][r] + A.a * k;
                 } else {
-                    A.out[f][gi] = A.acc[f][gi] + A.dt6 * k;
+                    A.out[f][gi] = pa[f][r] + A.dt6 * k;
                 }
             }
         }
