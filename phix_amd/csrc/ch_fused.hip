@@ -129,6 +129,16 @@ __device__ __forceinline__ AxisBC<T> uni_axis(const AxisBC<T>& b) {
     return u;
 }
 
+// In-plane ghost rule of a position (y decides before x):
+// FIX -> the FIXED constant of axis FY ? y : x, side HI ? high : low;
+// SWX/SWY -> exchange the +/- neighbours of that axis (NOFLUX image).
+enum : int { GC_FIX = 1, GC_SWX = 2, GC_SWY = 4, GC_FY = 8, GC_HI = 16 };
+
+template <typename T>
+__device__ __forceinline__ T ghost_value_xy(int code, const T* bcv) {
+    return bcv[((code & GC_FY) ? 2 : 0) + ((code & GC_HI) ? 1 : 0)];
+}
+
 // mu at halo position p from the stage state's 7 (5) point star.
 //  c: centre; x/y/z (p,m): the +/- neighbours as loaded; bcv: staged FIXED constants.
 template <typename T, int DIM>
@@ -158,29 +168,30 @@ __device__ __forceinline__ T mu_at(const CHArgs<T>& A, const T* bcv, int px, int
     return dwell(c, A.two_rho, A.ca, A.cb) + A.nkappa * val;
 }
 
-// Stage epilogue for one physical output cell.
+// Stage epilogue for one physical output cell; cn, an: c_n and the
+// accumulator at the cell (loaded beforehand, used by stages 2-4).
 template <typename T, bool EXACT>
-__device__ __forceinline__ void epilogue(const CHArgs<T>& A, long long gi, T uc, T k) {
+__device__ __forceinline__ void epilogue(const CHArgs<T>& A, long long gi, T uc, T k, T cn, T an) {
     const int s = A.stage;
     if (EXACT) {
         if (s == 1) {
             A.acc[gi] = k;
             A.out[gi] = uc + A.a * k;
         } else if (s < 4) {
-            A.acc[gi] = A.acc[gi] + (T)2 * k;
-            A.out[gi] = A.c[gi] + A.a * k;
+            A.acc[gi] = an + (T)2 * k;
+            A.out[gi] = cn + A.a * k;
         } else {
-            A.out[gi] = A.c[gi] + A.dt6 * (A.acc[gi] + k);
+            A.out[gi] = cn + A.dt6 * (an + k);
         }
     } else {
         if (s == 1) {
             A.acc[gi] = uc + A.dt6 * k;
             A.out[gi] = uc + A.a * k;
         } else if (s < 4) {
-            A.acc[gi] = A.acc[gi] + A.dt3 * k;
-            A.out[gi] = A.c[gi] + A.a * k;
+            A.acc[gi] = an + A.dt3 * k;
+            A.out[gi] = cn + A.a * k;
         } else {
-            A.out[gi] = A.acc[gi] + A.dt6 * k;
+            A.out[gi] = an + A.dt6 * k;
         }
     }
 }
@@ -229,6 +240,22 @@ __global__ __launch_bounds__(256) void k_ch_stage_2d(CHArgs<T> A) {
     T r[NS];
 #pragma unroll
     for (int i = 0; i < NS; ++i) r[i] = A.u[zb + off[i]];
+    // epilogue operands of the tile cells, in flight with the halo; a stage
+    // that does not need c_n or acc reads the stage state instead
+    constexpr int NO = (TX * TY) / 256;
+    static_assert((TX * TY) % 256 == 0, "tile must be a multiple of 256 cells");
+    const int st = A.stage;
+    const T* csrc = (st == 2 || st == 3 || (EXACT && st == 4)) ? A.c : A.u;
+    const T* asrc = st >= 2 ? A.acc : A.u;
+    T pc[NO], pa[NO];
+#pragma unroll
+    for (int j = 0; j < NO; ++j) {
+        const int t = tid + 256 * j;
+        const int oy = t / TX, ox = t - oy * TX;
+        const long long gi = A.L.at(min(x0 + ox, A.L.n0 - 1), DIM >= 2 ? min(y0 + oy, A.L.n1 - 1) : 0, kz);
+        pc[j] = csrc[gi];
+        pa[j] = asrc[gi];
+    }
 #pragma unroll
     for (int i = 0; i < NS; ++i) {
         const int t = tid + 256 * i;
@@ -244,7 +271,9 @@ __global__ __launch_bounds__(256) void k_ch_stage_2d(CHArgs<T> A) {
         MU[my][mx] = m;
     }
     __syncthreads();
-    for (int t = tid; t < TX * TY; t += 256) {
+#pragma unroll
+    for (int j = 0; j < NO; ++j) {
+        const int t = tid + 256 * j;
         int oy = t / TX, ox = t - oy * TX;
         int x = x0 + ox, y = y0 + oy;
         if (x >= A.L.n0 || y >= A.L.n1) continue;
@@ -252,7 +281,7 @@ __global__ __launch_bounds__(256) void k_ch_stage_2d(CHArgs<T> A) {
         T val = (MU[oy + 1][ox + 2] - (T)2 * m + MU[oy + 1][ox]) * A.idx2;
         if (DIM >= 2) val += (MU[oy + 2][ox + 1] - (T)2 * m + MU[oy][ox + 1]) * A.idy2;
         T k = A.M * val;
-        epilogue<T, EXACT>(A, A.L.at(x, y, kz), U[oy + 2][ox + 2], k);
+        epilogue<T, EXACT>(A, A.L.at(x, y, kz), U[oy + 2][ox + 2], k, pc[j], pa[j]);
     }
 }
 
@@ -269,7 +298,9 @@ __global__ __launch_bounds__(256) void k_ch_stage_2d(CHArgs<T> A) {
 //   halo plane o+2 is stored (before A) into the slot of plane o-2, whose last
 //   readers (mu(o-1) and output o-2) ran before iteration o-1's barriers;
 //   mu(o+1) is written (between A and B) into the slot of mu(o-2), last read
-//   by output o-1 before barrier A of iteration o.
+//   by output o-1 before barrier A of iteration o. mu of a NOFLUX ghost plane
+//   o+1 = n_z reads the image plane o and its neighbours o-1..o+1, all held by
+//   the ring in iteration o (plane o-1's slot is rewritten in iteration o+1).
 template <typename T, int TX, int TY, bool EXACT>
 __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
     constexpr int UX = TX + 4, UY = TY + 4, MX = TX + 2, MY = TY + 2;
@@ -362,45 +393,64 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
         for (int i = 0; i < NU; ++i)
             if (tid + 256 * i < UX * UY) dst[tid + 256 * i] = P.fix ? P.val : (((hfm >> i) & 1u) ? hfix[i] : P.r[i]);
     };
-    // mu slots: LDS centre offset in the halo plane, and halo position
-    int mc0[NM], mpx[NM], mpy[NM];
+    // mu slots. mu at a ghost position is the BC fill of mu: the value at the
+    // mirror image for NOFLUX, the value computed from the wrapped halo (its
+    // own position) for PERIODIC, the constant for FIXED. Each slot therefore
+    // evaluates the stencil at the LDS centre of its image position (x and y
+    // images precomputed here, z by taking the image plane's ring slots),
+    // which reproduces the image's own evaluation bit for bit; a FIXED slot
+    // takes its constant. The plane loop has no per-point BC logic.
+    int mc0[NM], mcode[NM];
 #pragma unroll
     for (int i = 0; i < NM; ++i) {
-        const int t = tid + 256 * i;
+        const int t = min(tid + 256 * i, MX * MY - 1);
         const int my = t / MX, mx = t - my * MX;
-        mc0[i] = (my + 1) * UX + mx + 1;
-        mpx[i] = x0 - 1 + mx;
-        mpy[i] = y0 - 1 + my;
+        const int px = x0 - 1 + mx, py = y0 - 1 + my;
+        int ix = px, iy = py, code = 0;
+        if (py < 0 || py >= A.L.n1) {   // y decides before x
+            const int k = py < 0 ? A.by.klo : A.by.khi;
+            if (k == PHX_BC_FIXED) code = GC_FIX | GC_FY | (py < 0 ? 0 : GC_HI);
+            else if (k == PHX_BC_NOFLUX) iy = py < 0 ? -1 - py : 2 * A.L.n1 - 1 - py;
+        }
+        if (!code && (px < 0 || px >= A.L.n0)) {
+            const int k = px < 0 ? A.bx.klo : A.bx.khi;
+            if (k == PHX_BC_FIXED) code = GC_FIX | (px < 0 ? 0 : GC_HI);
+            else if (k == PHX_BC_NOFLUX) ix = px < 0 ? -1 - px : 2 * A.L.n0 - 1 - px;
+        }
+        // positions past the first ghost layer (partial tiles) feed no
+        // output; keep their stencil inside the halo
+        ix = min(max(ix, x0 - 1), x0 + TX);
+        iy = min(max(iy, y0 - 1), y0 + TY);
+        mc0[i] = (iy - y0 + 2) * UX + (ix - x0 + 2);
+        mcode[i] = code;
     }
-    // tiles whose mu ring lies inside the domain in x and y need no BC logic
-    const bool xy_in = x0 >= 1 && x0 + TX < A.L.n0 && y0 >= 1 && y0 + TY < A.L.n1;
     auto compute_mu = [&](int m) {
-        const T* ul = U[(m + 3) & 3];
-        const T* um = U[(m + 4) & 3];
-        const T* uh = U[(m + 5) & 3];
+        // z: NOFLUX ghost planes evaluate at the image plane, FIXED ones are
+        // the constant, PERIODIC ones use their own (wrapped) planes
+        int mz = m;
+        bool zfix = false;
+        if (m < 0 || m >= A.L.n2) {
+            const int k = m < 0 ? A.bz.klo : A.bz.khi;
+            if (k == PHX_BC_FIXED) zfix = true;
+            else if (k == PHX_BC_NOFLUX) mz = m < 0 ? -1 - m : 2 * A.L.n2 - 1 - m;
+        }
+        const T zv = BCV[m < 0 ? 4 : 5];
+        const T* ul = U[(mz + 3) & 3];
+        const T* um = U[(mz + 4) & 3];
+        const T* uh = U[(mz + 5) & 3];
         T* dst = MU[(m + 3) % 3];
-        if (xy_in && m >= 0 && m < A.L.n2) {
 #pragma unroll
-            for (int i = 0; i < NM; ++i) {
-                const int t = tid + 256 * i;
-                if (t < MX * MY) {
-                    const int c0 = mc0[i];
-                    const T c = um[c0];
-                    T val = (um[c0 + 1] - (T)2 * c + um[c0 - 1]) * A.idx2;
-                    val += (um[c0 + UX] - (T)2 * c + um[c0 - UX]) * A.idy2;
-                    val += (uh[c0] - (T)2 * c + ul[c0]) * A.idz2;
-                    dst[t] = dwell(c, A.two_rho, A.ca, A.cb) + A.nkappa * val;
-                }
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < NM; ++i) {
-                const int t = tid + 256 * i;
-                if (t < MX * MY) {
-                    const int c0 = mc0[i];
-                    dst[t] = mu_at<T, 3>(A, BCV, mpx[i], mpy[i], m, um[c0], um[c0 + 1], um[c0 - 1], um[c0 + UX],
-                                         um[c0 - UX], uh[c0], ul[c0]);
-                }
+        for (int i = 0; i < NM; ++i) {
+            const int t = tid + 256 * i;
+            if (t < MX * MY) {
+                const int c0 = mc0[i];
+                const T c = um[c0];
+                T val = (um[c0 + 1] - (T)2 * c + um[c0 - 1]) * A.idx2;
+                val += (um[c0 + UX] - (T)2 * c + um[c0 - UX]) * A.idy2;
+                val += (uh[c0] - (T)2 * c + ul[c0]) * A.idz2;
+                T v = dwell(c, A.two_rho, A.ca, A.cb) + A.nkappa * val;
+                if (mcode[i] & GC_FIX) v = ghost_value_xy(mcode[i], BCV);
+                dst[t] = zfix ? zv : v;
             }
         }
     };
@@ -424,18 +474,19 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
         }
     };
 
-    // ---- prologue: halo planes z0-2 .. z0+1, mu planes z0-1, z0
-    Plane pre;
+    // ---- prologue: halo planes z0-2 .. z0+1, mu planes z0-1, z0; the five
+    // planes are loaded in two batches (each batch's loads in flight together)
+    Plane pre, p1, p2;
     load_plane(z0 - 2, pre);
+    load_plane(z0 - 1, p1);
+    load_plane(z0, p2);
     store_plane(z0 - 2, pre);
-    load_plane(z0 - 1, pre);
-    store_plane(z0 - 1, pre);
-    load_plane(z0, pre);
-    store_plane(z0, pre);
-    load_plane(z0 + 1, pre);
-    store_plane(z0 + 1, pre);
+    store_plane(z0 - 1, p1);
+    store_plane(z0, p2);
+    load_plane(z0 + 1, p1);
     load_plane(z0 + 2, pre);
     load_epi(z0, pc, pa);
+    store_plane(z0 + 1, p1);
     __syncthreads();
     compute_mu(z0 - 1);
     compute_mu(z0);
@@ -558,11 +609,6 @@ __device__ __forceinline__ int map_deep(int p, const AxisBC<T>& b) {
     return 2 * b.n - 1 - p;
 }
 
-// In-plane ghost rule of a position (y decides before x):
-// FIX -> the FIXED constant of axis FY ? y : x, side HI ? high : low;
-// SWX/SWY -> exchange the +/- neighbours of that axis (NOFLUX image).
-enum : int { GC_FIX = 1, GC_SWX = 2, GC_SWY = 4, GC_FY = 8, GC_HI = 16 };
-
 template <typename T>
 __device__ __forceinline__ int ghost_code_xy(int px, int py, const PairArgs<T>& A) {
     int code = 0;
@@ -579,10 +625,6 @@ __device__ __forceinline__ int ghost_code_xy(int px, int py, const PairArgs<T>& 
     return code;
 }
 
-template <typename T>
-__device__ __forceinline__ T ghost_value_xy(int code, const T* bcv) {
-    return bcv[((code & GC_FY) ? 2 : 0) + ((code & GC_HI) ? 1 : 0)];
-}
 
 // (xp - 2c + xm)*idx2 + (yp - 2c + ym)*idy2 + (zp - 2c + zm)*idz2, in this order
 template <typename T>
@@ -913,10 +955,11 @@ __global__ __launch_bounds__(256) void k_ch_pair_3d(PairArgs<T> A) {
 
     // prologue: UA planes z0-4, z0-3 stored, z0-2 in flight
     Plane pre;
+    Plane p1;
     load_ua(z0 - 4, pre);
+    load_ua(z0 - 3, p1);
     store_ua(z0 - 4, pre);
-    load_ua(z0 - 3, pre);
-    store_ua(z0 - 3, pre);
+    store_ua(z0 - 3, p1);
     load_ua(z0 - 2, pre);
     if constexpr (SECOND) {
         // same load sequence as an iteration's (values unused: the loop
