@@ -21,9 +21,11 @@
 //     (Equation.cu:571-572 applies the same bcs to the materialised scratch).
 //     PERIODIC: mu at ghost p equals mu at p mod n; computing it from the
 //     wrapped halo gives the same operands in the same order, bit for bit.
-//     NOFLUX: mu at ghost -1 equals mu(0); with the mirrored halo the same
-//     operands appear with the +/- neighbours of the ghost axis exchanged, so
-//     the kernel exchanges them back -- again bit for bit.
+//     NOFLUX: mu at ghost -1 equals mu(0); the one-stage kernels evaluate
+//     the stencil at the LDS centre of the image cell (the same operands as
+//     the image's own evaluation), the blocked kernels exchange the +/-
+//     neighbours of the ghost axis of the mirrored halo -- bit for bit either
+//     way.
 //     FIXED: mu at the ghost is the BC constant.
 //
 // FP64 keeps the reference's operation order and no FMA contraction, so the
@@ -66,25 +68,6 @@ template <typename T>
 __device__ __forceinline__ T dwell(T v, T two_rho, T ca, T cb) {
     // 2.0*rho*(c-ca)*(c-cb)*(2.0*c-ca-cb), evaluated left to right
     return two_rho * (v - ca) * (v - cb) * ((T)2 * v - ca - cb);
-}
-
-// Halo value of the stage state at position (px,py,pz) in [-2, n+1]^dim, with
-// the BC maps. FIXED faces return their constant; z, then y, then x decide
-// first, the reverse of the X, Y, Z order in which the reference fills faces.
-template <typename T, int DIM>
-__device__ __forceinline__ T load_halo(const T* __restrict__ u, const CHArgs<T>& A, int px, int py, int pz) {
-    int qz = 0, qy = 0, qx;
-    if (DIM >= 3) {
-        qz = map_in(pz, A.bz);
-        if (qz < 0) return fixed_value(pz, A.bz);
-    }
-    if (DIM >= 2) {
-        qy = map_in(py, A.by);
-        if (qy < 0) return fixed_value(py, A.by);
-    }
-    qx = map_in(px, A.bx);
-    if (qx < 0) return fixed_value(px, A.bx);
-    return u[A.L.at(qx, qy, qz)];
 }
 
 // The six FIXED constants (x lo, x hi, y lo, y hi, z lo, z hi) staged in LDS
@@ -137,35 +120,6 @@ enum : int { GC_FIX = 1, GC_SWX = 2, GC_SWY = 4, GC_FY = 8, GC_HI = 16 };
 template <typename T>
 __device__ __forceinline__ T ghost_value_xy(int code, const T* bcv) {
     return bcv[((code & GC_FY) ? 2 : 0) + ((code & GC_HI) ? 1 : 0)];
-}
-
-// mu at halo position p from the stage state's 7 (5) point star.
-//  c: centre; x/y/z (p,m): the +/- neighbours as loaded; bcv: staged FIXED constants.
-template <typename T, int DIM>
-__device__ __forceinline__ T mu_at(const CHArgs<T>& A, const T* bcv, int px, int py, int pz, T c, T xp, T xm, T yp,
-                                   T ym, T zp, T zm) {
-    const bool gx = px < 0 || px >= A.L.n0;
-    const bool gy = DIM >= 2 && (py < 0 || py >= A.L.n1);
-    const bool gz = DIM >= 3 && (pz < 0 || pz >= A.L.n2);
-    if (gz) {
-        int k = pz < 0 ? A.bz.klo : A.bz.khi;
-        if (k == PHX_BC_FIXED) return bcv[pz < 0 ? 4 : 5];
-        if (k == PHX_BC_NOFLUX) { T t = zp; zp = zm; zm = t; }
-    }
-    if (gy) {
-        int k = py < 0 ? A.by.klo : A.by.khi;
-        if (k == PHX_BC_FIXED) return bcv[py < 0 ? 2 : 3];
-        if (k == PHX_BC_NOFLUX) { T t = yp; yp = ym; ym = t; }
-    }
-    if (gx) {
-        int k = px < 0 ? A.bx.klo : A.bx.khi;
-        if (k == PHX_BC_FIXED) return bcv[px < 0 ? 0 : 1];
-        if (k == PHX_BC_NOFLUX) { T t = xp; xp = xm; xm = t; }
-    }
-    T val = (xp - (T)2 * c + xm) * A.idx2;
-    if (DIM >= 2) val += (yp - (T)2 * c + ym) * A.idy2;
-    if (DIM >= 3) val += (zp - (T)2 * c + zm) * A.idz2;
-    return dwell(c, A.two_rho, A.ca, A.cb) + A.nkappa * val;
 }
 
 // Stage epilogue for one physical output cell; cn, an: c_n and the
@@ -262,12 +216,32 @@ __global__ __launch_bounds__(256) void k_ch_stage_2d(CHArgs<T> A) {
         if (t < UX * UY) (&U[0][0])[t] = ((fm >> i) & 1u) ? fv[i] : r[i];
     }
     __syncthreads();
+    // mu at a ghost position is the BC fill of mu: evaluated at the LDS
+    // centre of its BC image (mirror for NOFLUX, its own wrapped position for
+    // PERIODIC), which reproduces the image's own evaluation bit for bit, or
+    // the FIXED constant (y decides before x).
     for (int t = tid; t < MX * MY; t += 256) {
-        int my = t / MX, mx = t - my * MX;
-        int px = x0 - 1 + mx, py = DIM >= 2 ? y0 - 1 + my : 0;
-        T c = U[my + 1][mx + 1];
-        T m = mu_at<T, DIM>(A, BCV, px, py, kz, c, U[my + 1][mx + 2], U[my + 1][mx], DIM >= 2 ? U[my + 2][mx + 1] : c,
-                            DIM >= 2 ? U[my][mx + 1] : c, c, c);
+        const int my = t / MX, mx = t - my * MX;
+        const int px = x0 - 1 + mx, py = DIM >= 2 ? y0 - 1 + my : 0;
+        int ix = px, iy = py, code = 0;
+        if (DIM >= 2 && (py < 0 || py >= A.L.n1)) {
+            const int k = py < 0 ? A.by.klo : A.by.khi;
+            if (k == PHX_BC_FIXED) code = GC_FIX | GC_FY | (py < 0 ? 0 : GC_HI);
+            else if (k == PHX_BC_NOFLUX) iy = py < 0 ? -1 - py : 2 * A.L.n1 - 1 - py;
+        }
+        if (!code && (px < 0 || px >= A.L.n0)) {
+            const int k = px < 0 ? A.bx.klo : A.bx.khi;
+            if (k == PHX_BC_FIXED) code = GC_FIX | (px < 0 ? 0 : GC_HI);
+            else if (k == PHX_BC_NOFLUX) ix = px < 0 ? -1 - px : 2 * A.L.n0 - 1 - px;
+        }
+        ix = min(max(ix, x0 - 1), x0 + TX);   // only partial tiles' unused cells clamp
+        const int lyc = DIM >= 2 ? min(max(iy, y0 - 1), y0 + TY) - y0 + 2 : 2;
+        const int lxc = ix - x0 + 2;
+        const T c = U[lyc][lxc];
+        T val = (U[lyc][lxc + 1] - (T)2 * c + U[lyc][lxc - 1]) * A.idx2;
+        if (DIM >= 2) val += (U[lyc + 1][lxc] - (T)2 * c + U[lyc - 1][lxc]) * A.idy2;
+        T m = dwell(c, A.two_rho, A.ca, A.cb) + A.nkappa * val;
+        if (code & GC_FIX) m = ghost_value_xy(code, BCV);
         MU[my][mx] = m;
     }
     __syncthreads();
