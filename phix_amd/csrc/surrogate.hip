@@ -347,15 +347,27 @@ __global__ void k_reduce_edge(const float* __restrict__ part, int nblk, int nout
 }
 
 // --------------------------------------------- projection, loss, optimiser --
-// per-sample FP64 compensated sums of v over H*W pixels (one block per sample)
-__global__ __launch_bounds__(256) void k_sample_sum(const float* __restrict__ v, int HW, double* __restrict__ out) {
-    const float* s0 = v + (long long)blockIdx.x * HW;
+// Per-segment FP64 compensated sums of v: segment s is v[s*len, (s+1)*len).
+// Two levels with a fixed reduction tree (bitwise deterministic): SUM_BLOCKS
+// workgroups per segment each reduce one contiguous share (Neumaier sums per
+// thread, then a compensated tree), and one thread per segment combines the
+// SUM_BLOCKS partial (sum, compensation) pairs in order.
+constexpr int SUM_BLOCKS = 64;
+
+__device__ __forceinline__ void neu_add(double& s, double& cmp, double x) {
+    const double t = s + x;
+    cmp += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+    s = t;
+}
+
+__global__ __launch_bounds__(256) void k_sum_partial(const float* __restrict__ v, long long len,
+                                                     double* __restrict__ part) {
+    const long long share = (len + SUM_BLOCKS - 1) / SUM_BLOCKS;
+    const long long q0 = (long long)blockIdx.x * share;
+    const long long q1 = min(q0 + share, len);
+    const float* s0 = v + (long long)blockIdx.y * len;
     double s = 0.0, cmp = 0.0;
-    for (int q = threadIdx.x; q < HW; q += 256) {
-        double x = s0[q], t = s + x;
-        cmp += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
-        s = t;
-    }
+    for (long long q = q0 + threadIdx.x; q < q1; q += 256) neu_add(s, cmp, (double)s0[q]);
     __shared__ double S[256], Cc[256];
     S[threadIdx.x] = s;
     Cc[threadIdx.x] = cmp;
@@ -369,7 +381,28 @@ __global__ __launch_bounds__(256) void k_sample_sum(const float* __restrict__ v,
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[blockIdx.x] = S[0] + Cc[0];
+    if (threadIdx.x == 0) {
+        double* pp = part + 2 * ((long long)blockIdx.y * SUM_BLOCKS + blockIdx.x);
+        pp[0] = S[0];
+        pp[1] = Cc[0];
+    }
+}
+
+__global__ void k_sum_final(const double* __restrict__ part, int nseg, double* __restrict__ out) {
+    const int sg = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sg >= nseg) return;
+    const double* pp = part + 2LL * sg * SUM_BLOCKS;
+    double s = 0.0, cmp = 0.0;
+    for (int k = 0; k < SUM_BLOCKS; ++k) {
+        neu_add(s, cmp, pp[2 * k]);
+        cmp += pp[2 * k + 1];
+    }
+    out[sg] = s + cmp;
+}
+
+static void seg_sums(const float* v, long long len, int nseg, double* part, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_sum_partial, dim3(SUM_BLOCKS, nseg), dim3(256), 0, st, v, len, part);
+    hipLaunchKernelGGL(k_sum_final, dim3((nseg + 63) / 64), dim3(64), 0, st, part, nseg, out);
 }
 
 __global__ void k_project(const float* __restrict__ c, const float* __restrict__ d, const double* __restrict__ dsum,
@@ -420,6 +453,7 @@ struct SurWS {
     float* e2;
     double* sums;     // per-sample sums (B)
     double* loss;     // 1
+    double* spart;    // SUM_BLOCKS * 2 * B partial sums
     float* part;      // WG_BLOCKS * PART
     __bf16* wfwd;     // 4 * MID_W
     __bf16* wbwd;     // 4 * MID_W
@@ -443,6 +477,7 @@ static size_t ws_layout(int B, int H, int W, SurWS* w, char* base) {
     s.e2 = (float*)take(npix * 4);
     s.sums = (double*)take((size_t)B * 8 + 8);
     s.loss = (double*)take(16);
+    s.spart = (double*)take((size_t)SUM_BLOCKS * 2 * B * 8);
     s.part = (float*)take((size_t)WG_BLOCKS * PART * 4);
     s.wfwd = (__bf16*)take(4 * MID_W * 2);
     s.wbwd = (__bf16*)take(4 * MID_W * 2);
@@ -469,7 +504,7 @@ static int sur_forward(const float* P, SurWS& S, const float* c, float* out, int
         hipLaunchKernelGGL(k_conv_mid, dim3(W / 16, H / 16, B), dim3(256), 0, st, S.act[l], S.wfwd + l * MID_W,
                            P + P_MID + l * (MID_W + CH) + MID_W, (const __bf16*)nullptr, S.act[l + 1], H, W, 0);
     hipLaunchKernelGGL(k_conv_out_fwd, dim3(nb), dim3(256), 0, st, S.act[4], P, S.d, B, H, W);
-    hipLaunchKernelGGL(k_sample_sum, dim3(B), dim3(256), 0, st, S.d, H * W, S.sums);
+    seg_sums(S.d, (long long)H * W, B, S.spart, S.sums, st);
     hipLaunchKernelGGL(k_project, dim3(nb), dim3(256), 0, st, c, S.d, S.sums, H * W, n, out);
     PHX_LAUNCH_CHECK("phx_sur_forward");
     return PHX_OK;
@@ -512,9 +547,9 @@ int phx_sur_backward(const float* params, void* ws, const float* c_in, const flo
     hipLaunchKernelGGL(k_loss_grad, dim3(nb), dim3(256), 0, st, c_out, c_target, n, S.g, S.e2);
     if (loss_dev) {
         // loss = mean of e2 = (sum over samples of per-sample sums) / n
-        hipLaunchKernelGGL(k_sample_sum, dim3(1), dim3(256), 0, st, S.e2, (int)n, S.loss);
+        seg_sums(S.e2, n, 1, S.spart, S.loss, st);
     }
-    hipLaunchKernelGGL(k_sample_sum, dim3(B), dim3(256), 0, st, S.g, H * W, S.sums);
+    seg_sums(S.g, (long long)H * W, B, S.spart, S.sums, st);
     hipLaunchKernelGGL(k_unproject, dim3(nb), dim3(256), 0, st, S.g, S.sums, H * W, n, S.d);
     // last layer: weight grads, then dz5
     hipLaunchKernelGGL(k_edge_wgrad, dim3(WG_BLOCKS), dim3(320), 0, st, (const float*)nullptr, S.act[4], S.d, S.part,

@@ -156,6 +156,24 @@ class Surrogate:
                   self.n, self.n, 1 if reuse_weights else 0, capi.stream_handle())
         return self.out
 
+    def forward_to(self, c, out, reuse_weights=True):
+        """Forward into a caller buffer (B x n x n FP32, may be c itself: the
+        projection, the last kernel, is elementwise)."""
+        assert c.shape == out.shape == (self.B, self.n, self.n) and out.is_contiguous()
+        capi.call("phx_sur_forward", capi.ptr(self.p), capi.ptr(self.ws), capi.ptr(c), capi.ptr(out), self.B,
+                  self.n, self.n, 1 if reuse_weights else 0, capi.stream_handle())
+        return out
+
+    def rollout(self, c, jumps, bufs):
+        """`jumps` forwards from c, alternating between the two buffers of
+        `bufs`; returns the buffer holding the result."""
+        cur = c
+        for j in range(jumps):
+            nxt = bufs[j % 2]
+            self.forward_to(cur, nxt)
+            cur = nxt
+        return cur
+
     def train_step(self, c, tgt, lr):
         self.forward(c)
         capi.call("phx_sur_backward", capi.ptr(self.p), capi.ptr(self.ws), capi.ptr(c), capi.ptr(self.out),
@@ -284,11 +302,13 @@ class HybridSolver:
     """Surrogate rollout judged by the solver, with solver fallback."""
 
     def __init__(self, dev, params, n=256, kappa=0.5, dt=0.01, every=100, check_every=10, tol=2e-3, etol=1e-6):
+        import torch
         self.net = Surrogate(dev, 1, n=n, params=params)
         self.solver = PlainSolver(dev, n=n, kappa=kappa, dt=dt)
         self.every, self.check_every, self.tol, self.etol = every, check_every, tol, etol
         self.n = n
         self.packed = False
+        self.bufs = [torch.empty((1, n, n), dtype=torch.float32, device=dev) for _ in range(3)]
 
     def run(self, c0, nsteps):
         """Advances c0 (n x n FP32 device tensor) by nsteps solver steps (a
@@ -307,11 +327,12 @@ class HybridSolver:
         while remaining > 0:
             k = min(self.check_every, remaining)
             start = cur.clone()
-            prev = cur
-            for _ in range(k):
-                prev = cur
-                cur = self.net.forward(cur, reuse_weights=self.packed).clone()
+            if not self.packed:
+                self.net.forward(cur)   # packs the BF16 weights once
                 self.packed = True
+            # prev: input of the block's last jump (kept for the judge)
+            prev = self.net.rollout(cur, k - 1, self.bufs).clone() if k > 1 else cur.clone()
+            cur = self.net.forward_to(prev, self.bufs[2])
             ssteps += k
             # judge: one surrogate step against `every` solver steps from its input
             S.set(prev[0])
@@ -385,6 +406,7 @@ class PararealSolver:
         self.arena = torch.empty((slices, n, n), dtype=torch.float32, device=dev)
         self.work = torch.empty(3 * cells, dtype=torch.float32, device=dev)
         self.judge = PlainSolver(dev, n=n, kappa=kappa, dt=dt)
+        self.bufs = [torch.empty((1, n, n), dtype=torch.float32, device=dev) for _ in range(2)]
 
     def close(self):
         if self.h:
@@ -392,10 +414,10 @@ class PararealSolver:
             self.h = None
 
     def G(self, u):
-        for _ in range(self.steps // self.every):
-            u = self.net.forward(u, reuse_weights=self.packed).clone()
+        if not self.packed:
+            self.net.forward(u)   # packs the BF16 weights once
             self.packed = True
-        return u
+        return self.net.rollout(u, self.steps // self.every, self.bufs).clone()
 
     def F_all(self, U):
         for s in range(self.S):
