@@ -164,7 +164,9 @@ size_t phx_ch_rk4_workspace_bytes(const phx_grid* grid, int dtype);
 int phx_ch_rk4_steps(const phx_grid* grid, int dtype, void* c, void* work,
                      const phx_ch_params* p, int nsteps, phx_stream stream);
 
-/* Launch shape of the 3D stage kernels: z-chunk zc (default 16), per
+/* Launch shape of the 3D stage kernels: z-chunk zc (default 16 FP32, 32
+ * FP64; default tiles: 3D FP32 0, 3D FP64 3, 2D FP32 the whole-step tile 4
+ * when n >= 8, else 0), per
  * (dim, n, dtype); the planes are split into cdiv(n_z, zc) balanced chunks of
  * floor or ceil(n_z / chunks) planes, one workgroup column each. The
  * autotuner times every tile shape x z-chunk in {4,8,16,32,64} plus the
@@ -177,7 +179,10 @@ int phx_ch_rk4_set_zchunk(const phx_grid* grid, int dtype, int zc);
  * 4: 64x4 (x by y columns per 256-thread workgroup). Tiles 5: 64x8, 6: 32x16,
  * 7: 32x8, 8: 64x4 select the two-stage kernel (stages 1+2 and 3+4 each in one
  * launch: 7 field passes per step instead of 16/17; same results bit for bit;
- * needs n >= 4 on every axis, else PHX_ERR_UNSUPPORTED). */
+ * needs n >= 4 on every axis, else PHX_ERR_UNSUPPORTED). In 2D: 0 is the
+ * one-stage kernel (64x16), 1: 64x16, 2: 64x32, 3: 128x16, 4: 32x32 the
+ * whole-step kernel (a full RK4 step per launch, 2 field passes; same results
+ * bit for bit; needs n >= 8 on both axes). */
 int phx_ch_rk4_set_tile(const phx_grid* grid, int dtype, int tile);
 int phx_ch_rk4_get_launch(const phx_grid* grid, int dtype, int* zc, int* tile);
 /* XCD-aware workgroup order of the 3D stage kernel (on by default): each of

@@ -1650,7 +1650,7 @@ size_t phx_ch_rk4_workspace_bytes(const phx_grid* G, int dtype) {
 // Launch of the RK4 kernels per (dim, shape, dtype): tile index (3D: one-stage
 // 0-4, two-stage 5-8; 2D: one-stage 0, whole-step 1-4) and, in 3D, the z
 // planes streamed per workgroup. Set by phx_ch_rk4_set_tile/_set_zchunk or the
-// autotuner; the defaults are the one-stage kernels.
+// autotuner; defaults below.
 struct Launch3 {
     int zc, tile;
 };
@@ -1661,8 +1661,12 @@ static Launch3 launch_for(const phx_grid* G, int dtype) {
     std::lock_guard<std::mutex> lk(g_zc_mu);
     auto it = g_zc.find(std::make_tuple(G->dim, G->n[0], G->n[1], G->n[2], dtype));
     if (it != g_zc.end()) return it->second;
+    // defaults: the launches measured fastest at 1024^2 and 256^3 on MI355X
+    // (FP32 2D: whole-step 32x32; FP32 3D: 64x16, z-chunk 16; FP64 3D:
+    // 32x16, z-chunk 32); every launch returns the same bits
+    if (G->dim == 2) return Launch3{1, dtype == PHX_F32 && step2d_ok(G) ? 4 : 0};
     if (G->dim != 3) return Launch3{1, 0};
-    return Launch3{16, dtype == PHX_F32 ? 0 : 1};
+    return dtype == PHX_F32 ? Launch3{16, 0} : Launch3{32, 3};
 }
 
 static void set_launch(const phx_grid* G, int dtype, Launch3 l) {
