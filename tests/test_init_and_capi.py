@@ -66,3 +66,59 @@ def test_argument_errors_are_reported():
     assert capi.lib().phx_stored_size(C.byref(capi.make_grid((128, 128)))) == 130 * 130 * 3
     with pytest.raises(capi.PhxError):
         capi.call("phx_grad_accumulate", C.byref(capi.make_grid((8, 8))), C.c_void_p(1), C.c_void_p(1), 2, 1.0, None)
+
+
+def _launch(g, dtype):
+    zc, tile = C.c_int(), C.c_int()
+    capi.call("phx_ch_rk4_get_launch", C.byref(g), dtype, C.byref(zc), C.byref(tile))
+    return zc.value, tile.value
+
+
+def test_launch_table_defaults_and_ranges():
+    """The RK4 launch table is host state (no GPU needed): defaults per
+    (dim, dtype), the tile ranges of 2D and 3D, the size limits of the
+    temporally blocked kernels and their error codes."""
+    lib = capi.lib()
+    g3 = capi.make_grid((40, 36, 34), bc=capi.NOFLUX)
+    assert _launch(g3, capi.F32) == (16, 0)
+    assert _launch(g3, capi.F64) == (32, 3)
+    g2 = capi.make_grid((1024, 1024), bc=capi.PERIODIC)
+    assert _launch(g2, capi.F32) == (1, 4)      # whole-step 32x32
+    assert _launch(g2, capi.F64) == (1, 0)      # one-stage
+    g2s = capi.make_grid((7, 64), bc=capi.PERIODIC)
+    assert _launch(g2s, capi.F32) == (1, 0)     # too small for the whole-step kernel
+    g1 = capi.make_grid((64,), bc=capi.PERIODIC)
+    assert _launch(g1, capi.F32)[1] == 0
+    # ranges: 3D 0-8, 2D 0-4, 1D 0
+    for t in range(9):
+        assert lib.phx_ch_rk4_set_tile(C.byref(g3), capi.F32, t) == 0
+    assert lib.phx_ch_rk4_set_tile(C.byref(g3), capi.F32, 9) == capi.PHX_ERR_INVALID
+    assert lib.phx_ch_rk4_set_tile(C.byref(g3), capi.F32, -1) == capi.PHX_ERR_INVALID
+    assert lib.phx_ch_rk4_set_tile(C.byref(g2), capi.F32, 5) == capi.PHX_ERR_INVALID
+    assert lib.phx_ch_rk4_set_tile(C.byref(g1), capi.F32, 1) == capi.PHX_ERR_INVALID
+    # the two-stage tiles need n >= 4 per axis, the whole-step tiles n >= 8
+    small3 = capi.make_grid((3, 16, 16), bc=capi.NOFLUX)
+    assert lib.phx_ch_rk4_set_tile(C.byref(small3), capi.F64, 5) == capi.PHX_ERR_UNSUPPORTED
+    assert lib.phx_ch_rk4_set_tile(C.byref(small3), capi.F64, 2) == 0
+    assert lib.phx_ch_rk4_set_tile(C.byref(g2s), capi.F32, 3) == capi.PHX_ERR_UNSUPPORTED
+    # settings are kept per (dim, shape, dtype)
+    assert lib.phx_ch_rk4_set_zchunk(C.byref(g3), capi.F32, 22) == 0
+    assert _launch(g3, capi.F32) == (22, 8)
+    assert _launch(g3, capi.F64) == (32, 3)
+    assert lib.phx_ch_rk4_set_zchunk(C.byref(g3), capi.F32, 0) == capi.PHX_ERR_INVALID
+    capi.call("phx_ch_rk4_set_tile", C.byref(g3), capi.F32, 0)
+    capi.call("phx_ch_rk4_set_zchunk", C.byref(g3), capi.F32, 16)
+
+
+def test_timing_record_without_launches():
+    """The per-launch timing record: enabling clears it, an empty record
+    reads zero launches, and kinds outside 0-3 are rejected."""
+    lib = capi.lib()
+    capi.call("phx_ch_timing", 1)
+    for kind in range(4):
+        n, ms = C.c_int(-1), C.c_double(-1.0)
+        capi.call("phx_ch_timing_read", kind, C.byref(n), C.byref(ms))
+        assert n.value == 0 and ms.value == 0.0
+    assert lib.phx_ch_timing_read(4, None, None) == capi.PHX_ERR_INVALID
+    assert lib.phx_ch_timing_read(-1, None, None) == capi.PHX_ERR_INVALID
+    capi.call("phx_ch_timing", 0)
