@@ -1,6 +1,6 @@
 """Benchmark of the fused phase-field hot path on one MI355X.
 
-Default workload (the north-star configuration that fits one GPU): 3D
+Default workload (the headline configuration that fits one GPU): 3D
 Cahn-Hilliard, 256^3 cells, dx=1, zero-flux BCs on all faces,
 f = c^2(1-c)^2, kappa=1, M=1, dt=0.005, classic RK4, FP32 stencil bulk;
 c0 = 0.4 + U(-0.02, 0.02) from SplitMix64 seed 2. One step = one RK4 time
@@ -400,7 +400,7 @@ def main(argv=None) -> int:
     ap.add_argument("--zc", type=int, default=None, help="3D stage z-chunk (skips the autotuner)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
+        print(f"bench.py: this benchmark runs on one GPU only (asked for --gpus {a.gpus}); "
               "the stencil path shards by slab with halo exchange but that is not built here.", file=sys.stderr)
         return 2
     out = WORKLOADS[a.workload](a, not a.no_cpu_baseline)

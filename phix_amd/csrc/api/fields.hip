@@ -350,7 +350,7 @@ void ScalarField::write(const std::string& path, FieldFormat fmt) const {
             os << mesh.coord(0, i) << " " << mesh.coord(1, j) << " " << mesh.coord(2, k) << " " << curr[q] << "\n";
         });
     } else {
-        throw std::runtime_error("IO::writeField: VTS output is not part of this build");
+        throw std::runtime_error("IO::writeField: VTS output is not supported by this library");
     }
 }
 
@@ -443,7 +443,7 @@ void VectorField::downloadPrevFromDevice() { for (auto& f : components_) f.downl
 void VectorField::downloadAllFromDevice() { for (auto& f : components_) f.downloadAllFromDevice(); }
 
 void VectorField::write(const std::string& path, FieldFormat fmt) const {
-    if (fmt != FieldFormat::BINARY) throw std::runtime_error("VectorField::write: only BINARY is part of this build");
+    if (fmt != FieldFormat::BINARY) throw std::runtime_error("VectorField::write: only BINARY is supported by this library");
     std::ofstream os(path, std::ios::binary);
     if (!os) throw std::runtime_error("IO::writeField: cannot open file: " + path);
     os << "# PhiX VectorField\nname         " << name << "\nnComponents  " << nComponents() << "\nnx " << mesh.n[0]

@@ -1,7 +1,7 @@
 // Known-answer tests of pw() and the field operators through the drop-in C++
 // API -- the cases and expected values of the reference's
 // develop/fieldOperation/test_fieldops.cu:118-291 (4x4 mesh, ghost 1, f1=2,
-// f2=3, f3=5, tolerance 1e-12), written against this build's headers.
+// f2=3, f3=5, tolerance 1e-12), written against the drop-in headers.
 // Usage: fieldops cpu|gpu   (gpu also checks the device path)
 #include <cmath>
 #include <cstdio>
