@@ -299,6 +299,7 @@ __global__ void k_reduce_mid(const float* __restrict__ part, int nblk, float* __
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= PART) return;
     float s = 0.f;
+#pragma unroll 16
     for (int b = 0; b < nblk; ++b) s += part[(long long)b * PART + t];
     if (t < MID_W) {
         int tap = t / (CH * CH), co = (t / CH) % CH, ci = t % CH;
@@ -311,37 +312,94 @@ __global__ void k_reduce_mid(const float* __restrict__ part, int nblk, float* __
 // ------------------------------------- first/last layer weight gradients --
 // mode 0 (first layer): part[blk][co*9+tap] = sum_p x[p+off(tap)] * dz1[p][co]; part[blk][288+co] = sum_p dz1[p][co]
 // mode 1 (last layer):  part[blk][ci*9+tap] = sum_p a5[p+off(tap)][ci] * dd[p]; part[blk][288] = sum_p dd[p]
+// Persistent grid over 16x16 pixel tiles (the tile order of k_conv_mid_wgrad).
+// The tile's halo and its per-pixel factor are staged in LDS. Thread t owns
+// output t and sums the tile's 256 pixels in row order, so each block's
+// partial is a fixed-order sum.
+template <int MODE>
 __global__ __launch_bounds__(320) void k_edge_wgrad(const float* __restrict__ c, const __bf16* __restrict__ act,
                                                     const float* __restrict__ dd, float* __restrict__ part, int B,
-                                                    int H, int W, int mode) {
+                                                    int H, int W) {
+    constexpr int XP = 18 * 18 + 4;   // row pitch of the transposed a5 halo tile (mode 1)
+    constexpr int ZP = 256 + 8;       // row pitch of the transposed dz1 tile (mode 0)
+    __shared__ __attribute__((aligned(16))) __bf16 Ct[CH * (MODE == 0 ? ZP : XP)];
+    __shared__ __attribute__((aligned(16))) float Fs[18 * 18];   // mode 0: x halo; mode 1: dd tile (256)
     const int t = threadIdx.x;   // 0..319: (channel, tap) pairs 0..287, bias 288..319
-    const long long npix = (long long)B * H * W;
-    const long long chunk = (npix + gridDim.x - 1) / gridDim.x;
-    const long long p0 = blockIdx.x * chunk, p1 = min(npix, p0 + chunk);
-    float s = 0.f;
+    const int tx = W / 16, ty = H / 16, ntiles = B * tx * ty;
     const int ch = t < 288 ? t / 9 : t - 288, tap = t < 288 ? t % 9 : 0;
-    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-    if (mode == 0 || t <= 288) {
-        for (long long p = p0; p < p1; ++p) {
-            const int x = (int)(p % W), y = (int)((p / W) % H);
-            const long long base = p - (long long)y * W - x;
-            const long long q = base + (long long)wrap(y + dy, H) * W + wrap(x + dx, W);
-            if (mode == 0) {
-                const float g = bf2f(act[p * CH + ch]);
-                s += t < 288 ? (c[q] - 0.5f) * g : g;
+    const int off = (tap / 3) * 18 + tap % 3;   // halo index of pixel (0, 0) shifted by the tap
+    float s = 0.f;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int n = tile / (tx * ty), rem = tile % (tx * ty);
+        const int y0 = (rem / tx) * 16, x0 = (rem % tx) * 16;
+        const long long img = (long long)n * H * W;
+        if (MODE == 0) {
+            for (int i = t; i < 18 * 18; i += 320) {
+                const int yy = wrap(y0 - 1 + i / 18, H), xx = wrap(x0 - 1 + i % 18, W);
+                Fs[i] = c[img + (long long)yy * W + xx] - 0.5f;
+            }
+            for (int i = t; i < 256 * 4; i += 320) {
+                const int pi = i >> 2, q = i & 3;
+                const long long pix = img + (long long)(y0 + pi / 16) * W + x0 + pi % 16;
+                uint4 v = *reinterpret_cast<const uint4*>(act + pix * CH + q * 8);
+                const __bf16* e = reinterpret_cast<const __bf16*>(&v);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) Ct[(q * 8 + j) * ZP + pi] = e[j];
+            }
+        } else {
+            for (int i = t; i < 18 * 18 * 4; i += 320) {
+                const int pi = i >> 2, q = i & 3;
+                const int yy = wrap(y0 - 1 + pi / 18, H), xx = wrap(x0 - 1 + pi % 18, W);
+                uint4 v = *reinterpret_cast<const uint4*>(act + (img + (long long)yy * W + xx) * CH + q * 8);
+                const __bf16* e = reinterpret_cast<const __bf16*>(&v);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) Ct[(q * 8 + j) * XP + pi] = e[j];
+            }
+            if (t < 256) Fs[t] = dd[img + (long long)(y0 + t / 16) * W + x0 + t % 16];
+        }
+        __syncthreads();
+        if (MODE == 0) {
+            const __bf16* z = &Ct[ch * ZP];
+            if (t < 288) {
+                for (int p8 = 0; p8 < 32; ++p8) {
+                    const bf16x8 zv = *reinterpret_cast<const bf16x8*>(z + p8 * 8);
+                    const float* xr = &Fs[(p8 >> 1) * 18 + (p8 & 1) * 8 + off];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) s += xr[j] * bf2f(zv[j]);
+                }
             } else {
-                s += t < 288 ? bf2f(act[q * CH + ch]) * dd[p] : dd[p];
+                for (int p8 = 0; p8 < 32; ++p8) {
+                    const bf16x8 zv = *reinterpret_cast<const bf16x8*>(z + p8 * 8);
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) s += bf2f(zv[j]);
+                }
+            }
+        } else {
+            if (t < 288) {
+                const __bf16* a = &Ct[ch * XP + off];
+                for (int p8 = 0; p8 < 32; ++p8) {
+                    const __bf16* ar = a + (p8 >> 1) * 18 + (p8 & 1) * 8;
+                    const float* dr = &Fs[p8 * 8];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) s += bf2f(ar[j]) * dr[j];
+                }
+            } else if (t == 288) {
+                for (int p = 0; p < 256; ++p) s += Fs[p];
             }
         }
+        __syncthreads();
     }
-    const int nout = mode == 0 ? 320 : 289;
+    const int nout = MODE == 0 ? 320 : 289;
     if (t < nout) part[(long long)blockIdx.x * 320 + t] = s;
 }
 
+// Fixed-order sums of the per-block partials (block 0 first); the unrolled
+// loop keeps many loads in flight while the adds stay in order.
 __global__ void k_reduce_edge(const float* __restrict__ part, int nblk, int nout, float* __restrict__ g) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nout) return;
     float s = 0.f;
+#pragma unroll 16
     for (int b = 0; b < nblk; ++b) s += part[(long long)b * 320 + t];
     g[t] = s;
 }
@@ -552,8 +610,8 @@ int phx_sur_backward(const float* params, void* ws, const float* c_in, const flo
     seg_sums(S.g, (long long)H * W, B, S.spart, S.sums, st);
     hipLaunchKernelGGL(k_unproject, dim3(nb), dim3(256), 0, st, S.g, S.sums, H * W, n, S.d);
     // last layer: weight grads, then dz5
-    hipLaunchKernelGGL(k_edge_wgrad, dim3(WG_BLOCKS), dim3(320), 0, st, (const float*)nullptr, S.act[4], S.d, S.part,
-                       B, H, W, 1);
+    hipLaunchKernelGGL(k_edge_wgrad<1>, dim3(WG_BLOCKS), dim3(320), 0, st, (const float*)nullptr, S.act[4], S.d,
+                       S.part, B, H, W);
     hipLaunchKernelGGL(k_reduce_edge, dim3(2), dim3(256), 0, st, S.part, WG_BLOCKS, 289, grad + P_W6);
     hipLaunchKernelGGL(k_conv_out_bwd, dim3(nb), dim3(256), 0, st, S.d, S.act[4], params, S.dz[0], B, H, W);
     int cur = 0;
@@ -567,8 +625,8 @@ int phx_sur_backward(const float* params, void* ws, const float* c_in, const flo
         cur ^= 1;
     }
     // first layer: weight grads from x and dz1
-    hipLaunchKernelGGL(k_edge_wgrad, dim3(WG_BLOCKS), dim3(320), 0, st, c_in, S.dz[cur], (const float*)nullptr,
-                       S.part, B, H, W, 0);
+    hipLaunchKernelGGL(k_edge_wgrad<0>, dim3(WG_BLOCKS), dim3(320), 0, st, c_in, S.dz[cur], (const float*)nullptr,
+                       S.part, B, H, W);
     hipLaunchKernelGGL(k_reduce_edge, dim3(2), dim3(256), 0, st, S.part, WG_BLOCKS, 320, grad + P_W1);
     if (loss_dev) hipMemcpyAsync(loss_dev, S.loss, sizeof(double), hipMemcpyDeviceToDevice, st);
     PHX_LAUNCH_CHECK("phx_sur_backward");
