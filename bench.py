@@ -37,9 +37,11 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-# launch tile indices of the 3D CH kernels: one-stage 0-4, two-stage 5-8
+# launch tile indices of the 3D CH kernels: one-stage 0-4, two-stage 5-8,
+# one-stage with z-neighbours in registers 9-14 (columns, workgroup threads)
 TILE_NAMES = ["64x16", "64x8", "128x8", "32x16", "64x4", "two-stage 64x8", "two-stage 32x16", "two-stage 32x8",
-              "two-stage 64x4"]
+              "two-stage 64x4", "zreg 64x16/256", "zreg 128x8/256", "zreg 128x16/512", "zreg 256x8/512",
+              "zreg 256x16/1024", "zreg 256x4/256"]
 
 
 def cpu_model() -> str:
@@ -52,16 +54,22 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
-def traffic_from_profiles(name: str):
+def traffic_from_profiles(name: str, launch: str | None = None):
     """HBM bytes per launch of the dominant kernel measured by rocprofv3 --pmc
-    (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if committed."""
+    (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if committed for this
+    launch (files with a "per_launch" table hold one entry per launch shape;
+    a launch that was not profiled reads None)."""
     f = ROOT / "profiles" / f"{name}_traffic.json"
-    if f.exists():
-        try:
-            return json.loads(f.read_text()).get("bytes_per_launch")
-        except (OSError, ValueError):
-            return None
-    return None
+    if not f.exists():
+        return None
+    try:
+        d = json.loads(f.read_text())
+    except (OSError, ValueError):
+        return None
+    if "per_launch" in d:
+        e = d["per_launch"].get(launch) if launch is not None else None
+        return e.get("bytes_per_launch") if e else None
+    return d.get("bytes_per_launch")
 
 
 def timed(fn, stream):
@@ -94,10 +102,11 @@ def kernel_times(run, steps):
     return kinds
 
 
-def roofline(bytes_per_launch, launch_s, kernel, traffic_name):
+def roofline(bytes_per_launch, launch_s, kernel, traffic_name, launch=None):
     ach = bytes_per_launch / launch_s / 1e9
     return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(traffic_name), "kernel": kernel,
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(traffic_name, launch),
+            "kernel": kernel,
             "alg_bytes_per_launch": bytes_per_launch, "avg_launch_us": round(launch_s * 1e6, 2)}
 
 
@@ -148,11 +157,12 @@ def bench_ch3d(a, with_cpu):
     # (1,2): read c_n, write acc and the stage-3 state; (3,4): read the stage-3
     # state, c_n and acc, write c_{n+1}
     passes = {0: (16 if dtype == capi.F32 else 17) / 4, 1: 3, 2: 4}
-    names = {0: "k_ch_stage_3d", 1: "k_ch_pair_3d (stages 1+2)", 2: "k_ch_pair_3d (stages 3+4)"}
+    names = {0: "k_ch_stage_3dz" if tile.value >= 9 else "k_ch_stage_3d", 1: "k_ch_pair_3d (stages 1+2)",
+             2: "k_ch_pair_3d (stages 3+4)"}
     tags = {0: "ch3d_rk4", 1: "ch3d_pair12", 2: "ch3d_pair34"}
     dom = max(kinds, key=lambda k: kinds[k][1])
     roof = roofline(passes[dom] * esz * cells, kinds[dom][1] / 1e3 / kinds[dom][0], names[dom],
-                    f"{tags[dom]}_{a.dtype}_256")
+                    f"{tags[dom]}_{a.dtype}_256", TILE_NAMES[tile.value])
     roof["launches"] = {names[k]: {"count": v[0], "avg_us": round(v[1] * 1e3 / v[0], 2),
                                    "alg_bytes": passes[k] * esz * cells} for k, v in kinds.items()}
     cpu = None
@@ -210,8 +220,10 @@ def bench_ch2d(a, with_cpu):
     passes = {0: 4, 3: 2}
     names = {0: "k_ch_stage_2d", 3: "k_ch_step_2d"}
     dom = max(kinds, key=lambda k: kinds[k][1])
+    tile_name = ["64x16 one-stage", "whole-step 64x16", "whole-step 64x32", "whole-step 128x16",
+                 "whole-step 32x32"][tile.value]
     roof = roofline(passes[dom] * 4 * cells, kinds[dom][1] / 1e3 / kinds[dom][0], names[dom],
-                    f"{'ch2d_rk4' if dom == 0 else 'ch2d_step'}_f32_1024")
+                    f"{'ch2d_rk4' if dom == 0 else 'ch2d_step'}_f32_1024", tile_name)
     roof["launches"] = {names[k]: {"count": v[0], "avg_us": round(v[1] * 1e3 / v[0], 2),
                                    "alg_bytes": passes[k] * 4 * cells} for k, v in kinds.items()}
     cpu = None
@@ -225,9 +237,7 @@ def bench_ch2d(a, with_cpu):
         cpu = {"value": cells * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
                "sample": f"{steps} RK4 steps of the same 1024^2 problem, FP64 oracle, {dt:.1f} s on {cpu_model()}"}
     cfg = {"workload": "ch2d_rk4_1024sq_periodic", "grid": [NX, NX], "bc": "periodic", "scheme": "rk4", "dt": 0.005,
-           "kappa": 1.0, "M": 1.0, "bulk": "f32", "parallelism": "single-gpu",
-           "tile": ["64x16 one-stage", "whole-step 64x16", "whole-step 64x32", "whole-step 128x16",
-                    "whole-step 32x32"][tile.value]}
+           "kappa": 1.0, "M": 1.0, "bulk": "f32", "parallelism": "single-gpu", "tile": tile_name}
     return line("cell-updates/s, Cahn-Hilliard RK4 2D (fused stencil)", cells * a.steps / (ms / 1e3),
                 "cell-updates/s", a.steps, a.warmup, ms, "f32",
                 "synthetic (c0 = 0.4 + U(-0.02, 0.02), SplitMix64 seed 2)", cfg, roof, cpu)

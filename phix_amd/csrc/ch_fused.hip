@@ -47,6 +47,15 @@
 namespace phx {
 
 // ------------------------------------------------------------- parameters --
+// Coefficients of the FP32 arithmetic form (below), rounded once from double
+// on the host; unused by FP64.
+template <typename T>
+struct FastCoef {
+    T icc;                // -2 (idx2 + idy2 + idz2)
+    T mx, my, mz, mcc;    // M*idx2, M*idy2, M*idz2, -2 M (idx2 + idy2 + idz2)
+    T ntrca, ncs;         // -2 rho ca, -(ca + cb)
+};
+
 template <typename T>
 struct CHArgs {
     const T* u;     // stage state (stencil input)
@@ -59,6 +68,7 @@ struct CHArgs {
     T two_rho, ca, cb, nkappa, M;
     T a;            // stage shift dt/2, dt/2, dt
     T dt6, dt3;
+    FastCoef<T> F;  // FP32 arithmetic form
     int stage;      // 1..4
     int zc;         // z planes per block (3D)
     int xcd_remap;  // 1: XCD-aware tile order (grid size a multiple of 8)
@@ -68,6 +78,64 @@ template <typename T>
 __device__ __forceinline__ T dwell(T v, T two_rho, T ca, T cb) {
     // 2.0*rho*(c-ca)*(c-cb)*(2.0*c-ca-cb), evaluated left to right
     return two_rho * (v - ca) * (v - cb) * ((T)2 * v - ca - cb);
+}
+
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// The arithmetic of one cell, shared by every kernel so that their results
+// agree bit for bit. FP64 (EXACT) keeps the reference's operation order:
+//   lap = (xp - 2c + xm)*idx2, then + (yp - 2c + ym)*idy2, then + z
+//         (Equation.cu:164-180);
+//   mu  = 2rho*(c-ca)*(c-cb)*(2c-ca-cb) + (-kappa)*lap
+//         (Cahn-Hillard_double-well.cu:87-90, Equation.cu:578-591);
+//   k   = M*lap(mu);  stage state c + a*k.
+// FP32 is the bandwidth path and is held to a stated error against the FP64
+// oracle instead; it uses explicit FMAs (contraction stays off file-wide):
+//   lap = fma(zp+zm, idz2, fma(yp+ym, idy2, fma(xp+xm, idx2, c*icc)));
+//   f'  = fma(2rho, c, -2rho*ca) * (c-cb) * fma(2, c, -(ca+cb)), exactly 0
+//         at c = cb (and at c = ca whenever 2rho*ca is exact);
+//   mu  = fma(-kappa, lap, f');
+//   k   = the lap form with M folded into the coefficients;
+//   stage state fma(a, k, c).
+// The neighbour pairs are sums, so exchanging the +/- neighbours of an axis
+// (NOFLUX images in the blocked kernels) leaves the FP32 result unchanged too.
+template <typename T, bool EXACT, int DIM, class Args>
+__device__ __forceinline__ T mu_at(const Args& A, T c, T xp, T xm, T yp, T ym, T zp, T zm) {
+    if constexpr (EXACT) {
+        T val = (xp - (T)2 * c + xm) * A.idx2;
+        if constexpr (DIM >= 2) val += (yp - (T)2 * c + ym) * A.idy2;
+        if constexpr (DIM >= 3) val += (zp - (T)2 * c + zm) * A.idz2;
+        return dwell(c, A.two_rho, A.ca, A.cb) + A.nkappa * val;
+    } else {
+        T val = fma_t(xp + xm, A.idx2, c * A.F.icc);
+        if constexpr (DIM >= 2) val = fma_t(yp + ym, A.idy2, val);
+        if constexpr (DIM >= 3) val = fma_t(zp + zm, A.idz2, val);
+        const T fp = fma_t(A.two_rho, c, A.F.ntrca) * (c - A.cb) * fma_t((T)2, c, A.F.ncs);
+        return fma_t(A.nkappa, val, fp);
+    }
+}
+
+template <typename T, bool EXACT, int DIM, class Args>
+__device__ __forceinline__ T k_at(const Args& A, T m, T xp, T xm, T yp, T ym, T zp, T zm) {
+    if constexpr (EXACT) {
+        T val = (xp - (T)2 * m + xm) * A.idx2;
+        if constexpr (DIM >= 2) val += (yp - (T)2 * m + ym) * A.idy2;
+        if constexpr (DIM >= 3) val += (zp - (T)2 * m + zm) * A.idz2;
+        return A.M * val;
+    } else {
+        T val = fma_t(xp + xm, A.F.mx, m * A.F.mcc);
+        if constexpr (DIM >= 2) val = fma_t(yp + ym, A.F.my, val);
+        if constexpr (DIM >= 3) val = fma_t(zp + zm, A.F.mz, val);
+        return val;
+    }
+}
+
+// c + a*k
+template <typename T, bool EXACT>
+__device__ __forceinline__ T axpy(T a, T k, T c) {
+    if constexpr (EXACT) return c + a * k;
+    else return fma_t(a, k, c);
 }
 
 // The six FIXED constants (x lo, x hi, y lo, y hi, z lo, z hi) staged in LDS
@@ -139,13 +207,13 @@ __device__ __forceinline__ void epilogue(const CHArgs<T>& A, long long gi, T uc,
         }
     } else {
         if (s == 1) {
-            A.acc[gi] = uc + A.dt6 * k;
-            A.out[gi] = uc + A.a * k;
+            A.acc[gi] = axpy<T, false>(A.dt6, k, uc);
+            A.out[gi] = axpy<T, false>(A.a, k, uc);
         } else if (s < 4) {
-            A.acc[gi] = an + A.dt3 * k;
-            A.out[gi] = cn + A.a * k;
+            A.acc[gi] = axpy<T, false>(A.dt3, k, an);
+            A.out[gi] = axpy<T, false>(A.a, k, cn);
         } else {
-            A.out[gi] = an + A.dt6 * k;
+            A.out[gi] = axpy<T, false>(A.dt6, k, an);
         }
     }
 }
@@ -238,9 +306,9 @@ __global__ __launch_bounds__(256) void k_ch_stage_2d(CHArgs<T> A) {
         const int lyc = DIM >= 2 ? min(max(iy, y0 - 1), y0 + TY) - y0 + 2 : 2;
         const int lxc = ix - x0 + 2;
         const T c = U[lyc][lxc];
-        T val = (U[lyc][lxc + 1] - (T)2 * c + U[lyc][lxc - 1]) * A.idx2;
-        if (DIM >= 2) val += (U[lyc + 1][lxc] - (T)2 * c + U[lyc - 1][lxc]) * A.idy2;
-        T m = dwell(c, A.two_rho, A.ca, A.cb) + A.nkappa * val;
+        T m = DIM >= 2 ? mu_at<T, EXACT, 2>(A, c, U[lyc][lxc + 1], U[lyc][lxc - 1], U[lyc + 1][lxc], U[lyc - 1][lxc],
+                                             (T)0, (T)0)
+                       : mu_at<T, EXACT, 1>(A, c, U[lyc][lxc + 1], U[lyc][lxc - 1], (T)0, (T)0, (T)0, (T)0);
         if (code & GC_FIX) m = ghost_value_xy(code, BCV);
         MU[my][mx] = m;
     }
@@ -252,9 +320,9 @@ __global__ __launch_bounds__(256) void k_ch_stage_2d(CHArgs<T> A) {
         int x = x0 + ox, y = y0 + oy;
         if (x >= A.L.n0 || y >= A.L.n1) continue;
         T m = MU[oy + 1][ox + 1];
-        T val = (MU[oy + 1][ox + 2] - (T)2 * m + MU[oy + 1][ox]) * A.idx2;
-        if (DIM >= 2) val += (MU[oy + 2][ox + 1] - (T)2 * m + MU[oy][ox + 1]) * A.idy2;
-        T k = A.M * val;
+        T k = DIM >= 2 ? k_at<T, EXACT, 2>(A, m, MU[oy + 1][ox + 2], MU[oy + 1][ox], MU[oy + 2][ox + 1], MU[oy][ox + 1],
+                                            (T)0, (T)0)
+                       : k_at<T, EXACT, 1>(A, m, MU[oy + 1][ox + 2], MU[oy + 1][ox], (T)0, (T)0, (T)0, (T)0);
         epilogue<T, EXACT>(A, A.L.at(x, y, kz), U[oy + 2][ox + 2], k, pc[j], pa[j]);
     }
 }
@@ -419,10 +487,7 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
             if (t < MX * MY) {
                 const int c0 = mc0[i];
                 const T c = um[c0];
-                T val = (um[c0 + 1] - (T)2 * c + um[c0 - 1]) * A.idx2;
-                val += (um[c0 + UX] - (T)2 * c + um[c0 - UX]) * A.idy2;
-                val += (uh[c0] - (T)2 * c + ul[c0]) * A.idz2;
-                T v = dwell(c, A.two_rho, A.ca, A.cb) + A.nkappa * val;
+                T v = mu_at<T, EXACT, 3>(A, c, um[c0 + 1], um[c0 - 1], um[c0 + UX], um[c0 - UX], uh[c0], ul[c0]);
                 if (mcode[i] & GC_FIX) v = ghost_value_xy(mcode[i], BCV);
                 dst[t] = zfix ? zv : v;
             }
@@ -449,28 +514,34 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
     };
 
     // ---- prologue: halo planes z0-2 .. z0+1, mu planes z0-1, z0; the five
-    // planes are loaded in two batches (each batch's loads in flight together)
-    Plane pre, p1, p2;
-    load_plane(z0 - 2, pre);
-    load_plane(z0 - 1, p1);
+    // planes are loaded in two batches (each batch's loads in flight together),
+    // then halo planes z0+2 and z0+3 go in flight for the loop
+    Plane pa0, pa1, p2;
+    load_plane(z0 - 2, pa0);
+    load_plane(z0 - 1, pa1);
     load_plane(z0, p2);
-    store_plane(z0 - 2, pre);
-    store_plane(z0 - 1, p1);
+    store_plane(z0 - 2, pa0);
+    store_plane(z0 - 1, pa1);
     store_plane(z0, p2);
-    load_plane(z0 + 1, p1);
-    load_plane(z0 + 2, pre);
+    load_plane(z0 + 1, p2);
+    load_plane(z0 + 2, pa0);
     load_epi(z0, pc, pa);
-    store_plane(z0 + 1, p1);
+    load_plane(min(z0 + 3, z1 + 1), pa1);
+    store_plane(z0 + 1, p2);
     __syncthreads();
     compute_mu(z0 - 1);
     compute_mu(z0);
     __syncthreads();
 
-    for (int o = z0; o < z1; ++o) {
-        store_plane(o + 2, pre);
+    // Iteration o stores halo plane o+2 from the register set P and refills P
+    // with plane o+4: each plane is in flight for two iterations. The loop runs
+    // pairs of iterations on the two register sets (registers cannot be
+    // indexed at run time), with one single iteration for an odd count.
+    auto iteration = [&](int o, Plane& P) {
+        store_plane(o + 2, P);
         __syncthreads();   // A
         // unconditional prefetch (clamped planes): the same loads every iteration
-        load_plane(min(o + 3, z1 + 1), pre);
+        load_plane(min(o + 4, z1 + 1), P);
         load_epi(min(o + 1, z1 - 1), nc, na);
         compute_mu(o + 1);
         __syncthreads();   // B
@@ -486,10 +557,8 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
             const int oy = t / TX, ox = t - oy * TX;
             const int m0 = (oy + 1) * MX + ox + 1;
             const T mc = mmid[m0];
-            T val = (mmid[m0 + 1] - (T)2 * mc + mmid[m0 - 1]) * A.idx2;
-            val += (mmid[m0 + MX] - (T)2 * mc + mmid[m0 - MX]) * A.idy2;
-            val += (mhi[m0] - (T)2 * mc + mlo[m0]) * A.idz2;
-            const T k = A.M * val;
+            const T k = k_at<T, EXACT, 3>(A, mc, mmid[m0 + 1], mmid[m0 - 1], mmid[m0 + MX], mmid[m0 - MX], mhi[m0],
+                                          mlo[m0]);
             const long long gi = pb + ooff[j];
             const T uc = uo[(oy + 2) * UX + ox + 2];
             if (EXACT) {
@@ -504,18 +573,381 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
                 }
             } else {
                 if (st == 1) {
-                    A.acc[gi] = uc + A.dt6 * k;
-                    A.out[gi] = uc + A.a * k;
+                    A.acc[gi] = axpy<T, false>(A.dt6, k, uc);
+                    A.out[gi] = axpy<T, false>(A.a, k, uc);
                 } else if (st < 4) {
-                    A.acc[gi] = pa[j] + A.dt3 * k;
-                    A.out[gi] = pc[j] + A.a * k;
+                    A.acc[gi] = axpy<T, false>(A.dt3, k, pa[j]);
+                    A.out[gi] = axpy<T, false>(A.a, k, pc[j]);
                 } else {
-                    A.out[gi] = pa[j] + A.dt6 * k;
+                    A.out[gi] = axpy<T, false>(A.dt6, k, pa[j]);
                 }
             }
         }
 #pragma unroll
         for (int j = 0; j < NO; ++j) {
+            pc[j] = nc[j];
+            pa[j] = na[j];
+        }
+    };
+    int o = z0;
+    for (; o + 1 < z1; o += 2) {
+        iteration(o, pa0);
+        iteration(o + 1, pa1);
+    }
+    if (o < z1) iteration(o, pa0);
+}
+
+// ------------------------------------------ 3D, z-neighbours in registers --
+// One-stage kernel in the 2.5D form. A workgroup of NT threads owns TX x TY
+// columns of a z-chunk and streams along z like k_ch_stage_3d, but each
+// thread keeps the z-neighbours its cells need in registers: the stage state
+// at planes o, o+1, o+2 and mu at planes o-1, o, o+1 of its own columns. LDS
+// holds one plane of the stage state (2-cell ring) and one plane of mu
+// (1-cell ring), from which only in-plane neighbours are read. That is
+// ~10 KB per 1024 cells instead of ~36 KB, so tiles 128-256 cells wide keep
+// their occupancy; their rows are long contiguous HBM bursts, which stream
+// faster on MI355X than 64-cell row pieces. Every value is formed from the
+// same operands by the same operations as in k_ch_stage_3d, so the results
+// are equal bit for bit.
+//
+// Slots of a thread (index q = tid + NT * i within each class):
+//   interior: q < TX*TY              output cells (mu kept for its z-stencil)
+//   ring:     q < 2*(TX+2) + 2*TY    the 1-cell ring of the mu plane
+//   outer:    q < 2*(TX+4) + 2*(TY+2) the 2-cell ring of the state plane
+// Interior and ring slots (the mu slots) keep the state at three planes in
+// registers; outer slots only feed the state plane in LDS. Ghost mu slots
+// follow the rules of k_ch_stage_3d: FIXED takes the constant, NOFLUX and
+// PERIODIC evaluate the stencil at the LDS centre of their BC image, whose
+// z-neighbours equal the slot's own (the column loads apply the x/y maps).
+//
+// Iteration o (output plane o), barriers B1, B2:
+//   state plane o+1 -> LDS                                          | B1
+//   mu(o+1) on every mu slot; output plane o from LDS mu(o) in-plane and the
+//   registers mu(o-1), mu(o+1)                                      | B2
+//   mu(o+1) -> LDS
+// The state plane written in iteration o was last read by the mu pass of
+// iteration o-1 (before its B2); mu(o+1) is written after B2 of iteration o,
+// which follows the last reads of mu(o), and is read after B1 of o+1.
+template <typename T, int TX, int TY, int NT, bool EXACT>
+__global__ __launch_bounds__(NT) void k_ch_stage_3dz(CHArgs<T> A) {
+    constexpr int UX = TX + 4, UY = TY + 4, MX = TX + 2, MY = TY + 2;
+    constexpr int NI = TX * TY, NRING = 2 * MX + 2 * TY, NOUT = 2 * UX + 2 * MY;
+    static_assert(NI % NT == 0, "tile must be a multiple of the workgroup size");
+    constexpr int NO = NI / NT;
+    constexpr int NR = (NRING + NT - 1) / NT;
+    constexpr int NQ = (NOUT + NT - 1) / NT;
+    constexpr int NS = NO + NR;
+    __shared__ T SU[UY * UX];
+    __shared__ T SM[MY * MX];
+    __shared__ T BCV[6];
+    const int tid = threadIdx.x;
+    stage_bc_values(BCV, A);
+    int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+    if (A.xcd_remap) {
+        const int gx = gridDim.x, gy = gridDim.y;
+        const int nb = gx * gy * gridDim.z;
+        const int h = bx + gx * (by + gy * bz);
+        const int lb = (h & 7) * (nb >> 3) + (h >> 3);
+        bx = lb % gx;
+        by = (lb / gx) % gy;
+        bz = lb / (gx * gy);
+    }
+    const int x0 = bx * TX, y0 = by * TY;
+    const int z0 = (int)(((long long)bz * A.L.n2) / gridDim.z);
+    const int z1 = (int)(((long long)(bz + 1) * A.L.n2) / gridDim.z);
+    const int g = A.L.g;
+    const long long sxy = A.L.sxy;
+    const int cell0 = g * A.L.sx + g;
+
+    // source offset of a state position (x/y maps applied; FIXED -> flag)
+    auto src_of = [&](int px, int py, int& off, bool& fix, T& fv) {
+        const int qy = map_in(py, A.by), qx = map_in(px, A.bx);
+        off = cell0;
+        fix = false;
+        fv = (T)0;
+        if (qy < 0) {
+            fix = true;
+            fv = fixed_value(py, A.by);
+        } else if (qx < 0) {
+            fix = true;
+            fv = fixed_value(px, A.bx);
+        } else {
+            off = (qy + g) * A.L.sx + (qx + g);
+        }
+    };
+    // mu slots: interior then ring. Column offsets are unsigned byte
+    // offsets within a plane: a uniform plane base plus a 32-bit offset
+    // addresses without per-lane 64-bit arithmetic.
+    unsigned soff[NS];
+    int sl[NS], mi[NS], mc[NS], mcode[NS];
+    T sfv[NS];
+    unsigned sfm = 0, sact = 0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        int mx, my;
+        bool act = true;
+        if (s < NO) {
+            const int q = tid + NT * s;
+            my = q / TX + 1;
+            mx = q - (my - 1) * TX + 1;
+        } else {
+            int q = tid + NT * (s - NO);
+            act = q < NRING;
+            q = min(q, NRING - 1);
+            if (q < MX) { mx = q; my = 0; }
+            else if (q < 2 * MX) { mx = q - MX; my = MY - 1; }
+            else if (q < 2 * MX + TY) { mx = 0; my = 1 + q - 2 * MX; }
+            else { mx = MX - 1; my = 1 + q - 2 * MX - TY; }
+        }
+        const int px = x0 - 1 + mx, py = y0 - 1 + my;
+        bool fix;
+        int off;
+        src_of(px, py, off, fix, sfv[s]);
+        soff[s] = (unsigned)off * (unsigned)sizeof(T);
+        if (fix) sfm |= 1u << s;
+        if (act) sact |= 1u << s;
+        sl[s] = (my + 1) * UX + (mx + 1);
+        mi[s] = my * MX + mx;
+        // mu image centre: y decides before x (as in k_ch_stage_3d)
+        int ix = px, iy = py, code = 0;
+        if (py < 0 || py >= A.L.n1) {
+            const int k = py < 0 ? A.by.klo : A.by.khi;
+            if (k == PHX_BC_FIXED) code = GC_FIX | GC_FY | (py < 0 ? 0 : GC_HI);
+            else if (k == PHX_BC_NOFLUX) iy = py < 0 ? -1 - py : 2 * A.L.n1 - 1 - py;
+        }
+        if (!code && (px < 0 || px >= A.L.n0)) {
+            const int k = px < 0 ? A.bx.klo : A.bx.khi;
+            if (k == PHX_BC_FIXED) code = GC_FIX | (px < 0 ? 0 : GC_HI);
+            else if (k == PHX_BC_NOFLUX) ix = px < 0 ? -1 - px : 2 * A.L.n0 - 1 - px;
+        }
+        // positions past the first ghost layer (partial tiles) feed no
+        // output; keep their stencil inside the state plane
+        ix = min(max(ix, x0 - 1), x0 + TX);
+        iy = min(max(iy, y0 - 1), y0 + TY);
+        mc[s] = (iy - y0 + 2) * UX + (ix - x0 + 2);
+        mcode[s] = code;
+    }
+    // outer slots
+    unsigned qoff[NQ];
+    int ql[NQ];
+    T qfv[NQ];
+    unsigned qfm = 0, qact = 0;
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+        int q = tid + NT * k;
+        if (q < NOUT) qact |= 1u << k;
+        q = min(q, NOUT - 1);
+        int ux, uy;
+        if (q < UX) { ux = q; uy = 0; }
+        else if (q < 2 * UX) { ux = q - UX; uy = UY - 1; }
+        else if (q < 2 * UX + MY) { ux = 0; uy = 1 + q - 2 * UX; }
+        else { ux = UX - 1; uy = 1 + q - 2 * UX - MY; }
+        bool fix;
+        int off;
+        src_of(x0 - 2 + ux, y0 - 2 + uy, off, fix, qfv[k]);
+        qoff[k] = (unsigned)off * (unsigned)sizeof(T);
+        if (fix) qfm |= 1u << k;
+        ql[k] = uy * UX + ux;
+    }
+    // output cells (cells beyond the domain read a valid cell, never store)
+    unsigned ooff[NO];
+    bool oval[NO];
+#pragma unroll
+    for (int j = 0; j < NO; ++j) {
+        const int q = tid + NT * j;
+        const int oy = q / TX, ox = q - oy * TX;
+        const int x = x0 + ox, y = y0 + oy;
+        oval[j] = x < A.L.n0 && y < A.L.n1;
+        ooff[j] = (unsigned)(oval[j] ? (y + g) * A.L.sx + (x + g) : cell0) * (unsigned)sizeof(T);
+    }
+
+    // column loads of plane pz: raw values plus the plane's FIXED z state
+    // The empty asm re-defines the 32-bit offset at each use: the compiler
+    // would otherwise hoist its zero extension out of the plane loop and add
+    // 64-bit VGPR addresses per load instead of using the scalar-base form.
+    auto at = [](const T* base, unsigned off) {
+        asm volatile("" : "+v"(off));
+        return *(const T*)((const char*)base + off);
+    };
+    auto zplane = [&](int pz, bool& fix, T& val) -> const T* {
+        const int qz = map_in(pz, A.bz);
+        fix = qz < 0;
+        val = fix ? fixed_value(pz, A.bz) : (T)0;
+        return A.u + (long long)((fix ? 0 : qz) + g) * sxy;
+    };
+    auto load_cols = [&](int pz, T (&r)[NS], bool& fix, T& val) {
+        const T* base = zplane(pz, fix, val);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) r[s] = at(base, soff[s]);
+    };
+    auto load_outer = [&](int pz, T (&r)[NQ], bool& fix, T& val) {
+        const T* base = zplane(pz, fix, val);
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) r[k] = at(base, qoff[k]);
+    };
+    auto resolve_cols = [&](const T (&r)[NS], bool fix, T val, T (&u)[NS]) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) u[s] = fix ? val : (((sfm >> s) & 1u) ? sfv[s] : r[s]);
+    };
+    auto store_state = [&](const T (&u)[NS], const T (&r)[NQ], bool qfix, T qval) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+            if ((sact >> s) & 1u) SU[sl[s]] = u[s];
+#pragma unroll
+        for (int k = 0; k < NQ; ++k)
+            if ((qact >> k) & 1u) SU[ql[k]] = qfix ? qval : (((qfm >> k) & 1u) ? qfv[k] : r[k]);
+    };
+    // z rule of mu plane m: FIXED ghost -> constant; NOFLUX ghost -> the
+    // image plane's mu (copied by the caller); else evaluate
+    auto zrule = [&](int m, bool& zfix, bool& znf, T& zv) {
+        zfix = znf = false;
+        zv = (T)0;
+        if (m < 0 || m >= A.L.n2) {
+            const int k = m < 0 ? A.bz.klo : A.bz.khi;
+            if (k == PHX_BC_FIXED) {
+                zfix = true;
+                zv = BCV[m < 0 ? 4 : 5];
+            } else if (k == PHX_BC_NOFLUX) {
+                znf = true;
+            }
+        }
+    };
+    auto mu_slot = [&](int s, T zp, T zm) {
+        const int c0 = mc[s];
+        T v = mu_at<T, EXACT, 3>(A, SU[c0], SU[c0 + 1], SU[c0 - 1], SU[c0 + UX], SU[c0 - UX], zp, zm);
+        if (mcode[s] & GC_FIX) v = ghost_value_xy(mcode[s], BCV);
+        return v;
+    };
+    // epilogue operands (both loaded for every stage: the same loads each
+    // iteration; a stage that needs neither reads the stage state)
+    const int st = A.stage;
+    const bool need_c = st == 2 || st == 3 || (EXACT && st == 4);
+    const bool need_a = st >= 2;
+    const T* csrc = need_c ? A.c : A.u;
+    const T* asrc = need_a ? A.acc : A.u;
+    auto load_epi = [&](int o, T (&rc)[NO], T (&ra)[NO]) {
+        const long long pb = (long long)(o + g) * sxy;
+#pragma unroll
+        for (int j = 0; j < NO; ++j) {
+            rc[j] = at(csrc + pb, ooff[j]);
+            ra[j] = at(asrc + pb, ooff[j]);
+        }
+    };
+
+    T u0[NS], u1[NS], u2[NS], upf[NS], qv[NQ];
+    T mum[NO], mu0[NO], mup[NS];
+    T pc[NO], pa[NO], nc[NO], na[NO];
+    bool ufix, qfix;
+    T uval, qval;
+    // ---- prologue: state planes z0-2 .. z0+1 at the mu slots, z0-1 and z0
+    // on the outer ring; mu(z0-1) on the interior, mu(z0) everywhere
+    {
+        T r0[NS], r1[NS], r2[NS], r3[NS], o0[NQ], o1[NQ];
+        bool f0, f1, f2, f3, g0, g1;
+        T v0, v1, v2, v3, w0, w1;
+        load_cols(z0 - 2, r0, f0, v0);
+        load_cols(z0 - 1, r1, f1, v1);
+        load_cols(z0, r2, f2, v2);
+        load_cols(z0 + 1, r3, f3, v3);
+        load_outer(z0 - 1, o0, g0, w0);
+        load_outer(z0, o1, g1, w1);
+        load_epi(z0, pc, pa);
+        T a0[NS], a1[NS];
+        resolve_cols(r0, f0, v0, a0);
+        resolve_cols(r1, f1, v1, a1);
+        resolve_cols(r2, f2, v2, u0);
+        resolve_cols(r3, f3, v3, u1);
+        store_state(a1, o0, g0, w0);
+        __syncthreads();
+        bool zfix, znf;
+        T zv;
+        zrule(z0 - 1, zfix, znf, zv);
+#pragma unroll
+        for (int j = 0; j < NO; ++j) mum[j] = zfix ? zv : mu_slot(j, u0[j], a0[j]);
+        __syncthreads();
+        store_state(u0, o1, g1, w1);
+        load_cols(min(z0 + 2, z1 + 1), upf, ufix, uval);
+        load_outer(min(z0 + 1, z1), qv, qfix, qval);
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const T v = mu_slot(s, u1[s], a1[s]);
+            if ((sact >> s) & 1u) SM[mi[s]] = v;
+            if (s < NO) mu0[s] = v;
+        }
+        // NOFLUX: mu of the ghost plane z0-1 = -1 is mu of plane 0
+        if (znf) {
+#pragma unroll
+            for (int j = 0; j < NO; ++j) mum[j] = mu0[j];
+        }
+        __syncthreads();
+    }
+
+    for (int o = z0; o < z1; ++o) {
+        resolve_cols(upf, ufix, uval, u2);
+        store_state(u1, qv, qfix, qval);
+        // unconditional prefetch (clamped planes): the same loads every iteration
+        load_cols(min(o + 3, z1 + 1), upf, ufix, uval);
+        load_outer(min(o + 2, z1), qv, qfix, qval);
+        load_epi(min(o + 1, z1 - 1), nc, na);
+        __syncthreads();   // B1
+        bool zfix, znf;
+        T zv;
+        zrule(o + 1, zfix, znf, zv);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            T v = mu_slot(s, u2[s], u0[s]);
+            if (s < NO && znf) v = mu0[s];
+            mup[s] = zfix ? zv : v;
+        }
+        const long long pb = (long long)(o + g) * sxy;
+        T* const accp = A.acc + pb;
+        T* const outp = A.out + pb;
+#pragma unroll
+        for (int j = 0; j < NO; ++j) {
+            if (!oval[j]) continue;
+            const int m0 = mi[j];
+            const T k = k_at<T, EXACT, 3>(A, SM[m0], SM[m0 + 1], SM[m0 - 1], SM[m0 + MX], SM[m0 - MX], mup[j], mum[j]);
+            unsigned ob = ooff[j];
+            asm volatile("" : "+v"(ob));
+            T& acc = *(T*)((char*)accp + ob);
+            T& out = *(T*)((char*)outp + ob);
+            const T uc = u0[j];
+            if (EXACT) {
+                if (st == 1) {
+                    acc = k;
+                    out = uc + A.a * k;
+                } else if (st < 4) {
+                    acc = pa[j] + (T)2 * k;
+                    out = pc[j] + A.a * k;
+                } else {
+                    out = pc[j] + A.dt6 * (pa[j] + k);
+                }
+            } else {
+                if (st == 1) {
+                    acc = axpy<T, false>(A.dt6, k, uc);
+                    out = axpy<T, false>(A.a, k, uc);
+                } else if (st < 4) {
+                    acc = axpy<T, false>(A.dt3, k, pa[j]);
+                    out = axpy<T, false>(A.a, k, pc[j]);
+                } else {
+                    out = axpy<T, false>(A.dt6, k, pa[j]);
+                }
+            }
+        }
+        __syncthreads();   // B2
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+            if ((sact >> s) & 1u) SM[mi[s]] = mup[s];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            u0[s] = u1[s];
+            u1[s] = u2[s];
+        }
+#pragma unroll
+        for (int j = 0; j < NO; ++j) {
+            mum[j] = mu0[j];
+            mu0[j] = mup[j];
             pc[j] = nc[j];
             pa[j] = na[j];
         }
@@ -562,6 +994,7 @@ struct PairArgs {
     T two_rho, ca, cb, nkappa, M;
     T a1, a2;      // stage shifts: (dt/2, dt/2) for (1,2); (dt, unused) for (3,4)
     T dt6, dt3;
+    FastCoef<T> F; // FP32 arithmetic form
     int zc;
     int xcd_remap;
 };
@@ -600,28 +1033,31 @@ __device__ __forceinline__ int ghost_code_xy(int px, int py, const PairArgs<T>& 
 }
 
 
-// (xp - 2c + xm)*idx2 + (yp - 2c + ym)*idy2 + (zp - 2c + zm)*idz2, in this order
-template <typename T>
-__device__ __forceinline__ T lap7(const PairArgs<T>& A, T c, T xp, T xm, T yp, T ym, T zp, T zm) {
-    T val = (xp - (T)2 * c + xm) * A.idx2;
-    val += (yp - (T)2 * c + ym) * A.idy2;
-    val += (zp - (T)2 * c + zm) * A.idz2;
-    return val;
+// Seven-point star around LDS index i of a level with row width W, with the
+// neighbours of NOFLUX ghost axes exchanged: s = {c, xp, xm, yp, ym, zp, zm}.
+template <typename T, int W, bool GEN>
+__device__ __forceinline__ void star(const T* lo, const T* mid, const T* hi, int i, int code, bool zsw, T (&s)[7]) {
+    s[0] = mid[i];
+    s[1] = mid[i + 1];
+    s[2] = mid[i - 1];
+    s[3] = mid[i + W];
+    s[4] = mid[i - W];
+    s[5] = hi[i];
+    s[6] = lo[i];
+    if (GEN) {
+        if (code & GC_SWX) { const T t = s[1]; s[1] = s[2]; s[2] = t; }
+        if (code & GC_SWY) { const T t = s[3]; s[3] = s[4]; s[4] = t; }
+        if (zsw) { const T t = s[5]; s[5] = s[6]; s[6] = t; }
+    }
 }
 
-// Seven-point star around LDS index i of a level with row width W, with the
-// neighbours of NOFLUX ghost axes exchanged.
-template <typename T, int W, bool GEN>
-__device__ __forceinline__ T star_lap(const PairArgs<T>& A, const T* lo, const T* mid, const T* hi, int i, int code,
-                                      bool zsw, T& c) {
-    c = mid[i];
-    T xp = mid[i + 1], xm = mid[i - 1], yp = mid[i + W], ym = mid[i - W], zp = hi[i], zm = lo[i];
-    if (GEN) {
-        if (code & GC_SWX) { const T t = xp; xp = xm; xm = t; }
-        if (code & GC_SWY) { const T t = yp; yp = ym; ym = t; }
-        if (zsw) { const T t = zp; zp = zm; zm = t; }
-    }
-    return lap7(A, c, xp, xm, yp, ym, zp, zm);
+template <typename T, bool EXACT>
+__device__ __forceinline__ T mu_star(const PairArgs<T>& A, const T (&s)[7]) {
+    return mu_at<T, EXACT, 3>(A, s[0], s[1], s[2], s[3], s[4], s[5], s[6]);
+}
+template <typename T, bool EXACT>
+__device__ __forceinline__ T k_star(const PairArgs<T>& A, const T (&s)[7]) {
+    return k_at<T, EXACT, 3>(A, s[0], s[1], s[2], s[3], s[4], s[5], s[6]);
 }
 
 template <typename T, int TX, int TY, bool EXACT, bool SECOND>
@@ -805,9 +1241,9 @@ __global__ __launch_bounds__(256) void k_ch_pair_3d(PairArgs<T> A) {
                     if (GEN && zfix) v = zv;
                     else if (GEN && (code3[i] & GC_FIX)) v = ghost_value_xy(code3[i], BCV);
                     else {
-                        T c;
-                        const T l = star_lap<T, W4, GEN>(A, lo, mid, hi, ci, code3[i], zsw, c);
-                        v = dwell(c, A.two_rho, A.ca, A.cb) + A.nkappa * l;
+                        T s7[7];
+                        star<T, W4, GEN>(lo, mid, hi, ci, code3[i], zsw, s7);
+                        v = mu_star<T, EXACT>(A, s7);
                     }
                     dst[t] = v;
                 }
@@ -843,15 +1279,16 @@ __global__ __launch_bounds__(256) void k_ch_pair_3d(PairArgs<T> A) {
                     if (GEN && zfix) v = zv;
                     else if (GEN && (code2[i] & GC_FIX)) v = ghost_value_xy(code2[i], BCV);
                     else {
-                        T m;
-                        k = A.M * star_lap<T, W3, GEN>(A, lo, mid, hi, mi, code2[i], zsw, m);
-                        v = cc + A.a1 * k;
+                        T s7[7];
+                        star<T, W3, GEN>(lo, mid, hi, mi, code2[i], zsw, s7);
+                        k = k_star<T, EXACT>(A, s7);
+                        v = axpy<T, EXACT>(A.a1, k, cc);
                     }
                     dst[t] = v;
                     const int ox = lx - 2, oy = ly - 2;
                     if (ox >= 0 && ox < TX && oy >= 0 && oy < TY) {
                         // stage 1 (FP32 form) keeps c + dt/6*k1, the others k
-                        kr[oy * TX + ox] = (!SECOND && !EXACT) ? cc + A.dt6 * k : k;
+                        kr[oy * TX + ox] = (!SECOND && !EXACT) ? axpy<T, EXACT>(A.dt6, k, cc) : k;
                         if constexpr (NEED_CR) cr[oy * TX + ox] = cc;
                     }
                 }
@@ -881,9 +1318,9 @@ __global__ __launch_bounds__(256) void k_ch_pair_3d(PairArgs<T> A) {
                     if (GEN && zfix) v = zv;
                     else if (GEN && (code1[i] & GC_FIX)) v = ghost_value_xy(code1[i], BCV);
                     else {
-                        T c;
-                        const T l = star_lap<T, W2, GEN>(A, lo, mid, hi, ci, code1[i], zsw, c);
-                        v = dwell(c, A.two_rho, A.ca, A.cb) + A.nkappa * l;
+                        T s7[7];
+                        star<T, W2, GEN>(lo, mid, hi, ci, code1[i], zsw, s7);
+                        v = mu_star<T, EXACT>(A, s7);
                     }
                     dst[t] = v;
                 }
@@ -906,22 +1343,23 @@ __global__ __launch_bounds__(256) void k_ch_pair_3d(PairArgs<T> A) {
             const int t = tid + 256 * j;
             const int oy = t / TX, ox = t - oy * TX;
             const int mi = (oy + 1) * W1 + ox + 1;
-            T m;
-            const T k = A.M * star_lap<T, W1, false>(A, lo, mid, hi, mi, 0, false, m);
+            T s7[7];
+            star<T, W1, false>(lo, mid, hi, mi, 0, false, s7);
+            const T k = k_star<T, EXACT>(A, s7);
             const long long gi = pb + ooff[j];
             if constexpr (!SECOND) {
                 // stage 2: acc = k1 + 2*k2 (FP64) or (c + dt/6*k1) + dt/3*k2; state c + dt/2*k2
                 if (EXACT) A.acc[gi] = kr[t] + (T)2 * k;
-                else A.acc[gi] = kr[t] + A.dt3 * k;
-                A.out[gi] = cr[t] + A.a2 * k;
+                else A.acc[gi] = axpy<T, EXACT>(A.dt3, k, kr[t]);
+                A.out[gi] = axpy<T, EXACT>(A.a2, k, cr[t]);
             } else {
                 // stage 3 accumulates k3, stage 4 finishes the step
                 if (EXACT) {
                     const T ac = areg[j] + (T)2 * kr[t];
                     A.out[gi] = cr[t] + A.dt6 * (ac + k);
                 } else {
-                    const T ac = areg[j] + A.dt3 * kr[t];
-                    A.out[gi] = ac + A.dt6 * k;
+                    const T ac = axpy<T, EXACT>(A.dt3, kr[t], areg[j]);
+                    A.out[gi] = axpy<T, EXACT>(A.dt6, k, ac);
                 }
             }
         }
@@ -978,13 +1416,6 @@ __global__ __launch_bounds__(256) void k_ch_pair_3d(PairArgs<T> A) {
 // SB = stage states 2..4, SACC = accumulator. A level reads the other buffer
 // and is separated by a barrier from the level that last read the buffer it
 // writes.
-template <typename T>
-__device__ __forceinline__ T lap5(const PairArgs<T>& A, T c, T xp, T xm, T yp, T ym) {
-    T val = (xp - (T)2 * c + xm) * A.idx2;
-    val += (yp - (T)2 * c + ym) * A.idy2;
-    return val;
-}
-
 template <typename T, int TX, int TY, bool EXACT>
 __global__ __launch_bounds__(256) void k_ch_step_2d(PairArgs<T> A) {
     constexpr int WC = TX + 16, NC = WC * (TY + 16);
@@ -1066,7 +1497,7 @@ __global__ __launch_bounds__(256) void k_ch_step_2d(PairArgs<T> A) {
                 T xp = src[si + 1], xm = src[si - 1], yp = src[si + WS], ym = src[si - WS];
                 if (code & GC_SWX) { const T q = xp; xp = xm; xm = q; }
                 if (code & GC_SWY) { const T q = yp; yp = ym; ym = q; }
-                v = dwell(c, A.two_rho, A.ca, A.cb) + A.nkappa * lap5(A, c, xp, xm, yp, ym);
+                v = mu_at<T, EXACT, 2>(A, c, xp, xm, yp, ym, (T)0, (T)0);
             }
             SA[t] = v;
         }
@@ -1090,15 +1521,15 @@ __global__ __launch_bounds__(256) void k_ch_step_2d(PairArgs<T> A) {
                 T xp = SA[si + 1], xm = SA[si - 1], yp = SA[si + WS], ym = SA[si - WS];
                 if (code & GC_SWX) { const T q = xp; xp = xm; xm = q; }
                 if (code & GC_SWY) { const T q = yp; yp = ym; ym = q; }
-                k = A.M * lap5(A, m, xp, xm, yp, ym);
-                v = cc + a * k;
+                k = k_at<T, EXACT, 2>(A, m, xp, xm, yp, ym, (T)0, (T)0);
+                v = axpy<T, EXACT>(a, k, cc);
             }
             SB[t] = v;
             const int ox = lx - R, oy = ly - R;
             if (ox >= 0 && ox < TX && oy >= 0 && oy < TY) {
                 T& ac = SACC[oy * TX + ox];
-                if (S == 1) ac = EXACT ? k : cc + A.dt6 * k;
-                else ac = EXACT ? ac + (T)2 * k : ac + A.dt3 * k;
+                if (S == 1) ac = EXACT ? k : axpy<T, EXACT>(A.dt6, k, cc);
+                else ac = EXACT ? ac + (T)2 * k : axpy<T, EXACT>(A.dt3, k, ac);
             }
         }
     };
@@ -1128,10 +1559,10 @@ __global__ __launch_bounds__(256) void k_ch_step_2d(PairArgs<T> A) {
         if (x >= A.L.n0 || y >= A.L.n1) continue;
         const int si = (oy + 1) * W1 + ox + 1;
         const T m = SA[si];
-        const T k = A.M * lap5(A, m, SA[si + 1], SA[si - 1], SA[si + W1], SA[si - W1]);
+        const T k = k_at<T, EXACT, 2>(A, m, SA[si + 1], SA[si - 1], SA[si + W1], SA[si - W1], (T)0, (T)0);
         const long long gi = zb + (long long)(y + g) * A.L.sx + (x + g);
         if (EXACT) A.out[gi] = SC[(oy + 8) * WC + ox + 8] + A.dt6 * (SACC[t] + k);
-        else A.out[gi] = SACC[t] + A.dt6 * k;
+        else A.out[gi] = axpy<T, EXACT>(A.dt6, k, SACC[t]);
     }
 }
 
@@ -1255,6 +1686,25 @@ static void fill_common(const phx_grid* G, const phx_ch_params* p, T& idx2, T& i
     M = (T)p->M;
 }
 
+// FP32 arithmetic-form coefficients, from the same double values as the
+// FP64 path's, each rounded once.
+template <typename T>
+static FastCoef<T> fast_coef(const phx_grid* G, const phx_ch_params* p) {
+    const double ix = 1.0 / (G->d[0] * G->d[0]);
+    const double iy = G->dim >= 2 ? 1.0 / (G->d[1] * G->d[1]) : 0.0;
+    const double iz = G->dim >= 3 ? 1.0 / (G->d[2] * G->d[2]) : 0.0;
+    const double sum = ix + iy + iz;
+    FastCoef<T> F;
+    F.icc = (T)(-2.0 * sum);
+    F.mx = (T)(p->M * ix);
+    F.my = (T)(p->M * iy);
+    F.mz = (T)(p->M * iz);
+    F.mcc = (T)(-2.0 * p->M * sum);
+    F.ntrca = (T)(-2.0 * p->rho * p->ca);
+    F.ncs = (T)(-(p->ca + p->cb));
+    return F;
+}
+
 template <typename T>
 static int euler_steps_t(const phx_grid* G, T* c, T* mu, const phx_ch_params* p, int nsteps, hipStream_t st) {
     EulerArgs<T> A;
@@ -1287,6 +1737,29 @@ struct Tile3 {
 static const Tile3 kTiles3[] = {{64, 16}, {64, 8}, {128, 8}, {32, 16}, {64, 4}};
 constexpr int kNumTiles3 = sizeof(kTiles3) / sizeof(kTiles3[0]);
 
+// Two-stage kernel tiles; launch tile indices kNumTiles3.. select them.
+static const Tile3 kTilesP[] = {{64, 8}, {32, 16}, {32, 8}, {64, 4}};
+constexpr int kNumTilesP = sizeof(kTilesP) / sizeof(kTilesP[0]);
+
+// One-stage kernel with z-neighbours in registers (TX x TY columns, NT
+// threads); launch tile indices kFirstZ.. select them.
+struct TileZ {
+    int tx, ty, nt;
+};
+static const TileZ kTilesZ[] = {{64, 16, 256}, {128, 8, 256}, {128, 16, 512},
+                                {256, 8, 512}, {256, 16, 1024}, {256, 4, 256}};
+constexpr int kNumTilesZ = sizeof(kTilesZ) / sizeof(kTilesZ[0]);
+constexpr int kFirstZ = kNumTiles3 + kNumTilesP;
+
+// Columns of a 3D launch tile (any kind) and its workgroup size.
+static Tile3 shape3(int tile) {
+    if (tile < kNumTiles3) return kTiles3[tile];
+    if (tile < kFirstZ) return kTilesP[tile - kNumTiles3];
+    return Tile3{kTilesZ[tile - kFirstZ].tx, kTilesZ[tile - kFirstZ].ty};
+}
+static int threads3(int tile) { return tile >= kFirstZ ? kTilesZ[tile - kFirstZ].nt : 256; }
+static bool is_pair_tile(int tile) { return tile >= kNumTiles3 && tile < kFirstZ; }
+
 template <typename T, bool EXACT>
 static void launch3(int tile, dim3 grid, const CHArgs<T>& A, hipStream_t st) {
     switch (tile) {
@@ -1294,13 +1767,17 @@ static void launch3(int tile, dim3 grid, const CHArgs<T>& A, hipStream_t st) {
         case 1: hipLaunchKernelGGL((k_ch_stage_3d<T, 64, 8, EXACT>), grid, dim3(256), 0, st, A); break;
         case 2: hipLaunchKernelGGL((k_ch_stage_3d<T, 128, 8, EXACT>), grid, dim3(256), 0, st, A); break;
         case 3: hipLaunchKernelGGL((k_ch_stage_3d<T, 32, 16, EXACT>), grid, dim3(256), 0, st, A); break;
-        default: hipLaunchKernelGGL((k_ch_stage_3d<T, 64, 4, EXACT>), grid, dim3(256), 0, st, A); break;
+        case 4: hipLaunchKernelGGL((k_ch_stage_3d<T, 64, 4, EXACT>), grid, dim3(256), 0, st, A); break;
+        case kFirstZ + 0: hipLaunchKernelGGL((k_ch_stage_3dz<T, 64, 16, 256, EXACT>), grid, dim3(256), 0, st, A); break;
+        case kFirstZ + 1: hipLaunchKernelGGL((k_ch_stage_3dz<T, 128, 8, 256, EXACT>), grid, dim3(256), 0, st, A); break;
+        case kFirstZ + 2: hipLaunchKernelGGL((k_ch_stage_3dz<T, 128, 16, 512, EXACT>), grid, dim3(512), 0, st, A); break;
+        case kFirstZ + 3: hipLaunchKernelGGL((k_ch_stage_3dz<T, 256, 8, 512, EXACT>), grid, dim3(512), 0, st, A); break;
+        case kFirstZ + 4:
+            hipLaunchKernelGGL((k_ch_stage_3dz<T, 256, 16, 1024, EXACT>), grid, dim3(1024), 0, st, A);
+            break;
+        default: hipLaunchKernelGGL((k_ch_stage_3dz<T, 256, 4, 256, EXACT>), grid, dim3(256), 0, st, A); break;
     }
 }
-
-// Two-stage kernel tiles; launch tile indices kNumTiles3.. select them.
-static const Tile3 kTilesP[] = {{64, 8}, {32, 16}, {32, 8}, {64, 4}};
-constexpr int kNumTilesP = sizeof(kTilesP) / sizeof(kTilesP[0]);
 
 template <typename T, bool EXACT, bool SECOND>
 static void launch_pair(int tile, dim3 grid, const PairArgs<T>& P, hipStream_t st) {
@@ -1373,8 +1850,8 @@ static void lt_end(int kind, hipStream_t st) {
 static int g_xcd_remap = 1;
 
 // Resident workgroups per CU of the 3D kernel of launch tile `tile` (one-stage
-// 0-4; two-stage 5-8, sized by its (3,4) kernel), from the occupancy
-// calculator; 0 if unknown.
+// 0-4; two-stage 5-8, sized by its (3,4) kernel; z-register 9-14), from the
+// occupancy calculator; 0 if unknown.
 template <typename T, bool EXACT>
 static int blocks_per_cu(int tile) {
     const void* f = nullptr;
@@ -1388,10 +1865,16 @@ static int blocks_per_cu(int tile) {
         case 6: f = reinterpret_cast<const void*>(&k_ch_pair_3d<T, 32, 16, EXACT, true>); break;
         case 7: f = reinterpret_cast<const void*>(&k_ch_pair_3d<T, 32, 8, EXACT, true>); break;
         case 8: f = reinterpret_cast<const void*>(&k_ch_pair_3d<T, 64, 4, EXACT, true>); break;
+        case kFirstZ + 0: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 64, 16, 256, EXACT>); break;
+        case kFirstZ + 1: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 128, 8, 256, EXACT>); break;
+        case kFirstZ + 2: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 128, 16, 512, EXACT>); break;
+        case kFirstZ + 3: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 256, 8, 512, EXACT>); break;
+        case kFirstZ + 4: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 256, 16, 1024, EXACT>); break;
+        case kFirstZ + 5: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 256, 4, 256, EXACT>); break;
         default: return 0;
     }
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 256, 0) != hipSuccess) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, threads3(tile), 0) != hipSuccess) {
         (void)hipGetLastError();
         return 0;
     }
@@ -1411,6 +1894,7 @@ static int rk4_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_params* p,
     A.by = axis_bc<T>(G, 1);
     A.bz = axis_bc<T>(G, 2);
     fill_common<T>(G, p, A.idx2, A.idy2, A.idz2, A.two_rho, A.ca, A.cb, A.nkappa, A.M);
+    A.F = fast_coef<T>(G, p);
     const double dt = p->dt, dt6 = dt / 6.0;
     A.dt6 = (T)dt6;
     A.dt3 = (T)(2.0 * dt6);
@@ -1423,7 +1907,8 @@ static int rk4_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_params* p,
     T* uout[4] = {w0, w1, w0, c};
     dim3 grid;
     if (G->dim == 3) {
-        grid = dim3(cdiv(A.L.n0, kTiles3[tile].tx), cdiv(A.L.n1, kTiles3[tile].ty), cdiv(A.L.n2, zc));
+        const Tile3 tl = shape3(tile);
+        grid = dim3(cdiv(A.L.n0, tl.tx), cdiv(A.L.n1, tl.ty), cdiv(A.L.n2, zc));
         A.xcd_remap = g_xcd_remap && (grid.x * grid.y * grid.z) % 8 == 0;
     }
     else grid = dim3(cdiv(A.L.n0, T2X), cdiv(A.L.n1, T2Y), 1);
@@ -1469,6 +1954,7 @@ static int rk4_pair_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_param
     P.by = axis_bc<T>(G, 1);
     P.bz = axis_bc<T>(G, 2);
     fill_common<T>(G, p, P.idx2, P.idy2, P.idz2, P.two_rho, P.ca, P.cb, P.nkappa, P.M);
+    P.F = fast_coef<T>(G, p);
     const double dt = p->dt, dt6 = dt / 6.0;
     P.dt6 = (T)dt6;
     P.dt3 = (T)(2.0 * dt6);
@@ -1533,6 +2019,7 @@ static int rk4_pair_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_param
     A.by = P.by;
     A.bz = P.bz;
     fill_common<T>(G, p, A.idx2, A.idy2, A.idz2, A.two_rho, A.ca, A.cb, A.nkappa, A.M);
+    A.F = fast_coef<T>(G, p);
     A.dt6 = P.dt6;
     A.dt3 = P.dt3;
     A.c = c;
@@ -1577,6 +2064,7 @@ static int rk4_step2d_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_par
     P.by = axis_bc<T>(G, 1);
     P.bz = axis_bc<T>(G, 2);
     fill_common<T>(G, p, P.idx2, P.idy2, P.idz2, P.two_rho, P.ca, P.cb, P.nkappa, P.M);
+    P.F = fast_coef<T>(G, p);
     const double dt = p->dt, dt6 = dt / 6.0;
     P.dt6 = (T)dt6;
     P.dt3 = (T)(2.0 * dt6);
@@ -1648,7 +2136,8 @@ size_t phx_ch_rk4_workspace_bytes(const phx_grid* G, int dtype) {
 }
 
 // Launch of the RK4 kernels per (dim, shape, dtype): tile index (3D: one-stage
-// 0-4, two-stage 5-8; 2D: one-stage 0, whole-step 1-4) and, in 3D, the z
+// 0-4, two-stage 5-8, one-stage with z-neighbours in registers 9-14; 2D:
+// one-stage 0, whole-step 1-4) and, in 3D, the z
 // planes streamed per workgroup. Set by phx_ch_rk4_set_tile/_set_zchunk or the
 // autotuner; defaults below.
 struct Launch3 {
@@ -1662,11 +2151,12 @@ static Launch3 launch_for(const phx_grid* G, int dtype) {
     auto it = g_zc.find(std::make_tuple(G->dim, G->n[0], G->n[1], G->n[2], dtype));
     if (it != g_zc.end()) return it->second;
     // defaults: the launches measured fastest at 1024^2 and 256^3 on MI355X
-    // (FP32 2D: whole-step 32x32; FP32 3D: 64x16, z-chunk 16; FP64 3D:
-    // 32x16, z-chunk 32); every launch returns the same bits
+    // (FP32 2D: whole-step 32x32; 3D: z-register kernel with 128x16 columns
+    // per 512-thread workgroup, z-chunk 16 in FP32 and 32 in FP64); every
+    // launch returns the same bits
     if (G->dim == 2) return Launch3{1, dtype == PHX_F32 && step2d_ok(G) ? 4 : 0};
     if (G->dim != 3) return Launch3{1, 0};
-    return dtype == PHX_F32 ? Launch3{16, 0} : Launch3{32, 3};
+    return dtype == PHX_F32 ? Launch3{16, kFirstZ + 2} : Launch3{32, kFirstZ + 2};
 }
 
 static void set_launch(const phx_grid* G, int dtype, Launch3 l) {
@@ -1701,9 +2191,9 @@ int phx_ch_rk4_get_launch(const phx_grid* G, int dtype, int* zc, int* tile) {
 int phx_ch_rk4_set_tile(const phx_grid* G, int dtype, int tile) {
     int rc = check_grid(G, "phx_ch_rk4_set_tile");
     if (rc) return rc;
-    const int ntiles = G->dim == 3 ? kNumTiles3 + kNumTilesP : (G->dim == 2 ? 1 + kNumTilesS2 : 1);
+    const int ntiles = G->dim == 3 ? kFirstZ + kNumTilesZ : (G->dim == 2 ? 1 + kNumTilesS2 : 1);
     if (tile < 0 || tile >= ntiles) return fail(PHX_ERR_INVALID, "phx_ch_rk4_set_tile: tile index out of range");
-    if (G->dim == 3 && tile >= kNumTiles3 && !pair_ok(G))
+    if (G->dim == 3 && is_pair_tile(tile) && !pair_ok(G))
         return fail(PHX_ERR_UNSUPPORTED, "phx_ch_rk4_set_tile: two-stage tiles need n >= 4 on every axis");
     if (G->dim == 2 && tile >= 1 && !step2d_ok(G))
         return fail(PHX_ERR_UNSUPPORTED, "phx_ch_rk4_set_tile: whole-step 2D tiles need n >= 8 on both axes");
@@ -1725,7 +2215,7 @@ static int run_rk4(const phx_grid* G, int dtype, void* c, void* work, const phx_
                                 : rk4_step2d_steps_t<float, false>(G, (float*)c, (float*)work, p, nsteps, st,
                                                                    l.tile - 1, pure);
     if (G->dim != 3) l.tile = 0;
-    if (G->dim == 3 && l.tile >= kNumTiles3) {
+    if (G->dim == 3 && is_pair_tile(l.tile)) {
         if (!pair_ok(G)) l.tile = dtype == PHX_F32 ? 0 : 1;
         else
             return dtype == PHX_F64 ? rk4_pair_steps_t<double, true>(G, (double*)c, (double*)work, p, nsteps, st,
@@ -1753,9 +2243,11 @@ int phx_ch_rk4_steps(const phx_grid* G, int dtype, void* c, void* work, const ph
 
 // Times every launch candidate on `steps` steps of the caller's state
 // (advancing it) and keeps the fastest per step for this shape: in 3D the
-// one-stage kernel (5 tiles) and the two-stage kernel (4 tiles) at z-chunks
-// {4, 8, 16, 32, 64}; in 2D the one-stage kernel and the whole-step kernel
-// (4 tiles). Two-stage and whole-step candidates run 2*ceil(steps/2) steps.
+// one-stage kernel (5 tiles), the two-stage kernel (4 tiles) and the
+// z-register kernel (6 tiles) at z-chunks {4, 8, 16, 32, 64} plus the chunk
+// counts that fill whole waves of resident workgroups; in 2D the one-stage
+// kernel and the whole-step kernel (4 tiles). Two-stage and whole-step
+// candidates run 2*ceil(steps/2) steps.
 int phx_ch_rk4_autotune(const phx_grid* G, int dtype, void* c, void* work, const phx_ch_params* p, int steps,
                         int* best_zc, double* best_ms_per_step, phx_stream s) {
     int rc = check_grid_fused(G, "phx_ch_rk4_autotune");
@@ -1768,19 +2260,18 @@ int phx_ch_rk4_autotune(const phx_grid* G, int dtype, void* c, void* work, const
     hipEvent_t e0, e1;
     PHX_HIP(hipEventCreate(&e0));
     PHX_HIP(hipEventCreate(&e1));
-    const int ntiles = G->dim == 3 ? kNumTiles3 + (pair_ok(G) ? kNumTilesP : 0)
-                                   : (G->dim == 2 && step2d_ok(G) ? 1 + kNumTilesS2 : 1);
-    const int first_fused = G->dim == 3 ? kNumTiles3 : 1;
+    const int ntiles = G->dim == 3 ? kFirstZ + kNumTilesZ : (G->dim == 2 && step2d_ok(G) ? 1 + kNumTilesS2 : 1);
     const int psteps = 2 * ((steps + 1) / 2);
     int dev = 0, ncu = 0;
     PHX_HIP(hipGetDevice(&dev));
     PHX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     for (int tile = 0; tile < ntiles; ++tile) {
+    if (G->dim == 3 && is_pair_tile(tile) && !pair_ok(G)) continue;
     // z-chunks: fixed sizes, plus chunk counts that fill whole waves of the
     // resident workgroup slots (no partly filled last wave)
     std::vector<int> zcs(cand, cand + sizeof(cand) / sizeof(cand[0]));
     if (G->dim == 3) {
-        const Tile3 tl = tile < kNumTiles3 ? kTiles3[tile] : kTilesP[tile - kNumTiles3];
+        const Tile3 tl = shape3(tile);
         const long long ncols = (long long)cdiv(G->n[0], tl.tx) * cdiv(G->n[1], tl.ty);
         const int bpc = dtype == PHX_F64 ? blocks_per_cu<double, true>(tile) : blocks_per_cu<float, false>(tile);
         for (int k = 1; k <= 3 && bpc > 0; ++k) {
@@ -1793,7 +2284,7 @@ int phx_ch_rk4_autotune(const phx_grid* G, int dtype, void* c, void* work, const
         if (G->dim == 3 && zc > G->n[2]) continue;
         if (G->dim != 3 && zc != cand[0]) continue;
         const Launch3 l{G->dim == 3 ? zc : 1, tile};
-        const int pure = tile >= first_fused;
+        const int pure = G->dim == 3 ? is_pair_tile(tile) : tile >= 1;
         const int n = pure ? psteps : steps;
         if ((rc = run_rk4(G, dtype, c, work, p, pure ? 2 : 1, st, l, pure))) return rc;
         PHX_HIP(hipEventRecord(e0, st));

@@ -80,8 +80,8 @@ def test_launch_table_defaults_and_ranges():
     temporally blocked kernels and their error codes."""
     lib = capi.lib()
     g3 = capi.make_grid((40, 36, 34), bc=capi.NOFLUX)
-    assert _launch(g3, capi.F32) == (16, 0)
-    assert _launch(g3, capi.F64) == (32, 3)
+    assert _launch(g3, capi.F32) == (16, 11)    # z-register 128x16, 512 threads
+    assert _launch(g3, capi.F64) == (32, 11)
     g2 = capi.make_grid((1024, 1024), bc=capi.PERIODIC)
     assert _launch(g2, capi.F32) == (1, 4)      # whole-step 32x32
     assert _launch(g2, capi.F64) == (1, 0)      # one-stage
@@ -89,10 +89,10 @@ def test_launch_table_defaults_and_ranges():
     assert _launch(g2s, capi.F32) == (1, 0)     # too small for the whole-step kernel
     g1 = capi.make_grid((64,), bc=capi.PERIODIC)
     assert _launch(g1, capi.F32)[1] == 0
-    # ranges: 3D 0-8, 2D 0-4, 1D 0
-    for t in range(9):
+    # ranges: 3D 0-14, 2D 0-4, 1D 0
+    for t in range(15):
         assert lib.phx_ch_rk4_set_tile(C.byref(g3), capi.F32, t) == 0
-    assert lib.phx_ch_rk4_set_tile(C.byref(g3), capi.F32, 9) == capi.PHX_ERR_INVALID
+    assert lib.phx_ch_rk4_set_tile(C.byref(g3), capi.F32, 15) == capi.PHX_ERR_INVALID
     assert lib.phx_ch_rk4_set_tile(C.byref(g3), capi.F32, -1) == capi.PHX_ERR_INVALID
     assert lib.phx_ch_rk4_set_tile(C.byref(g2), capi.F32, 5) == capi.PHX_ERR_INVALID
     assert lib.phx_ch_rk4_set_tile(C.byref(g1), capi.F32, 1) == capi.PHX_ERR_INVALID
@@ -100,13 +100,14 @@ def test_launch_table_defaults_and_ranges():
     small3 = capi.make_grid((3, 16, 16), bc=capi.NOFLUX)
     assert lib.phx_ch_rk4_set_tile(C.byref(small3), capi.F64, 5) == capi.PHX_ERR_UNSUPPORTED
     assert lib.phx_ch_rk4_set_tile(C.byref(small3), capi.F64, 2) == 0
+    assert lib.phx_ch_rk4_set_tile(C.byref(small3), capi.F64, 9) == 0   # z-register tiles: any n
     assert lib.phx_ch_rk4_set_tile(C.byref(g2s), capi.F32, 3) == capi.PHX_ERR_UNSUPPORTED
     # settings are kept per (dim, shape, dtype)
     assert lib.phx_ch_rk4_set_zchunk(C.byref(g3), capi.F32, 22) == 0
-    assert _launch(g3, capi.F32) == (22, 8)
-    assert _launch(g3, capi.F64) == (32, 3)
+    assert _launch(g3, capi.F32) == (22, 14)
+    assert _launch(g3, capi.F64) == (32, 11)
     assert lib.phx_ch_rk4_set_zchunk(C.byref(g3), capi.F32, 0) == capi.PHX_ERR_INVALID
-    capi.call("phx_ch_rk4_set_tile", C.byref(g3), capi.F32, 0)
+    capi.call("phx_ch_rk4_set_tile", C.byref(g3), capi.F32, 11)
     capi.call("phx_ch_rk4_set_zchunk", C.byref(g3), capi.F32, 16)
 
 
