@@ -165,7 +165,7 @@ int phx_ch_rk4_steps(const phx_grid* grid, int dtype, void* c, void* work,
                      const phx_ch_params* p, int nsteps, phx_stream stream);
 
 /* Launch shape of the 3D stage kernels: z-chunk zc (default 16 FP32, 32
- * FP64; default tiles: 3D FP32 0, 3D FP64 3, 2D FP32 the whole-step tile 4
+ * FP64; default tiles: 3D 11 for both dtypes, 2D FP32 the whole-step tile 4
  * when n >= 8, else 0), per
  * (dim, n, dtype); the planes are split into cdiv(n_z, zc) balanced chunks of
  * floor or ceil(n_z / chunks) planes, one workgroup column each. The
@@ -179,7 +179,12 @@ int phx_ch_rk4_set_zchunk(const phx_grid* grid, int dtype, int zc);
  * 4: 64x4 (x by y columns per 256-thread workgroup). Tiles 5: 64x8, 6: 32x16,
  * 7: 32x8, 8: 64x4 select the two-stage kernel (stages 1+2 and 3+4 each in one
  * launch: 7 field passes per step instead of 16/17; same results bit for bit;
- * needs n >= 4 on every axis, else PHX_ERR_UNSUPPORTED). In 2D: 0 is the
+ * needs n >= 4 on every axis, else PHX_ERR_UNSUPPORTED). Tiles 9: 64x16/256,
+ * 10: 128x8/256, 11: 128x16/512, 12: 256x8/512, 13: 256x16/1024, 14: 256x4/256
+ * (columns / workgroup threads) select the one-stage kernel that keeps each
+ * column's z-neighbours in registers and only one plane of the stage state
+ * and of mu in LDS, so wide tiles (long contiguous rows in HBM) keep their
+ * occupancy; same results bit for bit; any n >= 2. In 2D: 0 is the
  * one-stage kernel (64x16), 1: 64x16, 2: 64x32, 3: 128x16, 4: 32x32 the
  * whole-step kernel (a full RK4 step per launch, 2 field passes; same results
  * bit for bit; needs n >= 8 on both axes). */
