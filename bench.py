@@ -394,8 +394,59 @@ def bench_copy(a, with_cpu):
                 ms, "f32", "constant buffer", cfg, roof, None)
 
 
+def bench_hbmprobe(a, with_cpu):
+    """Bandwidth that simple access patterns reach on this GPU (phx_hbm_probe):
+    read, write and one-float4-per-thread copy of 1 GiB; the 5 streams of an
+    RK4 stage (3 reads, 2 writes) over 256^3 FP32 fields; and the 3D stage
+    kernels' tile access pattern over the 256^3 problem without compute, per
+    tile shape and z-chunk count. Each figure is bytes moved / event-timed
+    duration, the mean of 20 launches after 3 warm-up launches."""
+    import torch
+    from phix_amd import capi
+    stream = torch.cuda.current_stream()
+    sh = capi.stream_handle(stream)
+    big = (1 << 30) // 4
+    x = torch.zeros(big, dtype=torch.float32, device="cuda:0")
+    y = torch.zeros(big, dtype=torch.float32, device="cuda:0")
+    n3 = 256 ** 3
+    f = [torch.zeros(n3, dtype=torch.float32, device="cuda:0") for _ in range(5)]
+    s3 = 258 ** 3
+    t = [torch.zeros(s3, dtype=torch.float32, device="cuda:0") for _ in range(4)]
+    P = capi.ptr
+
+    def probe(kind, bufs, n, side=0, tile=0, blocks=0, reps=20):
+        args = [P(b) if b is not None else None for b in bufs] + [None] * (5 - len(bufs))
+        call = lambda: capi.call("phx_hbm_probe", kind, *args, n, side, tile, blocks, sh)
+        for _ in range(3):
+            call()
+        ms = timed(lambda: [call() for _ in range(reps)], stream)
+        return ms / 1e3 / reps
+
+    res = {}
+    for blocks in (2048, 8192):
+        res[f"read 1GiB, {blocks} workgroups"] = 4 * big / probe(0, [x, y], big, blocks=blocks) / 1e9
+        res[f"write 1GiB, {blocks} workgroups"] = 4 * big / probe(1, [x], big, blocks=blocks) / 1e9
+    res["copy 1GiB, one float4 per thread"] = 8 * big / probe(2, [x, y], big) / 1e9
+    for blocks in (2048, 16384):
+        res[f"RK4-stage streams (3 reads, 2 writes) 256^3, {blocks} workgroups"] = \
+            20 * n3 / probe(3, f, n3, blocks=blocks) / 1e9
+    alg = 20 * n3   # the 5 passes over the interior cells
+    for tile, name in ((0, "64x16"), (1, "128x8"), (2, "256x4")):
+        for chunks in (16, 64):
+            for kind, what in ((4, "with halo"), (5, "cells only")):
+                res[f"stage tile pattern {name}, {chunks} z-chunks, {what}"] = \
+                    alg / probe(kind, t, s3, side=256, tile=tile, blocks=chunks) / 1e9
+    res = {k: round(v, 1) for k, v in res.items()}
+    key = "RK4-stage streams (3 reads, 2 writes) 256^3, 16384 workgroups"
+    roof = {"bound": "hbm", "achieved": res[key], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(res[key] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_probe_stage"}
+    cfg = {"workload": "hbm_probe", "patterns_gbs": res, "parallelism": "single-gpu"}
+    return line("GB/s, HBM access-pattern probes (bytes moved / time)", res[key], "GB/s", 20, 3, 0.0, "f32",
+                "zero-filled buffers", cfg, roof, None)
+
+
 WORKLOADS = {"copycal": bench_copy, "ch3d": bench_ch3d, "ch2d1024": bench_ch2d, "ac1024": bench_ac, "ens512": bench_ens,
-             "euler128": bench_euler}
+             "euler128": bench_euler, "hbmprobe": bench_hbmprobe}
 
 
 def main(argv=None) -> int:

@@ -205,6 +205,22 @@ int phx_ch_rk4_autotune(const phx_grid* grid, int dtype, void* c, void* work,
 int phx_ch_timing(int on);
 int phx_ch_timing_read(int kind, int* launches, double* total_ms);
 
+/* HBM streaming probes (no reference counterpart: measurement only). One
+ * launch of probe `kind` on `stream` over caller-owned FP32 device buffers of
+ * n floats each (the bandwidth simple access patterns reach, beside the
+ * stencil kernels' roofline):
+ *   0 read a (b: sink, never written in practice); 1 write a;
+ *   2 copy a -> b, one float4 per thread; 3 d = b + a/2, e = c + a/4
+ *   (3 reads, 2 writes: an RK4 stage's streams); kinds 0, 1, 3 use `blocks`
+ *   grid-stride workgroups (0: 8192);
+ *   4 the 3D stage kernels' access pattern without compute: TX x TY columns
+ *   (tile 0: 64x16, 1: 128x8, 2: 256x4) of `blocks` z-chunks of a side^3
+ *   grid stored with one ghost layer (n >= (side+2)^3), per plane the
+ *   (TX+4) x (TY+4) halo of a, the cells of b and c read, of d and c written;
+ *   5 as 4 reading only the cells of a. */
+int phx_hbm_probe(int kind, float* a, float* b, float* c, float* d, float* e, long long n, int side, int tile,
+                  int blocks, phx_stream stream);
+
 /* ------------------------------------------ FP64 diagnostics ---- */
 /* Workspace in bytes for phx_mass_energy. */
 size_t phx_reduce_workspace_bytes(const phx_grid* grid);
