@@ -38,10 +38,10 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 # launch tile indices of the 3D CH kernels: one-stage 0-4, two-stage 5-8,
-# one-stage with z-neighbours in registers 9-14 (columns, workgroup threads)
+# one-stage with z-neighbours in registers 9-17 (columns, workgroup threads)
 TILE_NAMES = ["64x16", "64x8", "128x8", "32x16", "64x4", "two-stage 64x8", "two-stage 32x16", "two-stage 32x8",
               "two-stage 64x4", "zreg 64x16/256", "zreg 128x8/256", "zreg 128x16/512", "zreg 256x8/512",
-              "zreg 256x16/1024", "zreg 256x4/256"]
+              "zreg 256x16/1024", "zreg 256x4/256", "zreg 256x8/1024", "zreg 128x16/1024", "zreg 128x8/512"]
 
 
 def cpu_model() -> str:

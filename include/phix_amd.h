@@ -180,8 +180,9 @@ int phx_ch_rk4_set_zchunk(const phx_grid* grid, int dtype, int zc);
  * 7: 32x8, 8: 64x4 select the two-stage kernel (stages 1+2 and 3+4 each in one
  * launch: 7 field passes per step instead of 16/17; same results bit for bit;
  * needs n >= 4 on every axis, else PHX_ERR_UNSUPPORTED). Tiles 9: 64x16/256,
- * 10: 128x8/256, 11: 128x16/512, 12: 256x8/512, 13: 256x16/1024, 14: 256x4/256
- * (columns / workgroup threads) select the one-stage kernel that keeps each
+ * 10: 128x8/256, 11: 128x16/512, 12: 256x8/512, 13: 256x16/1024, 14: 256x4/256,
+ * 15: 256x8/1024, 16: 128x16/1024, 17: 128x8/512 (columns / workgroup
+ * threads) select the one-stage kernel that keeps each
  * column's z-neighbours in registers and only one plane of the stage state
  * and of mu in LDS, so wide tiles (long contiguous rows in HBM) keep their
  * occupancy; same results bit for bit; any n >= 2. In 2D: 0 is the

@@ -628,8 +628,17 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
 // The state plane written in iteration o was last read by the mu pass of
 // iteration o-1 (before its B2); mu(o+1) is written after B2 of iteration o,
 // which follows the last reads of mu(o), and is read after B1 of o+1.
+// Minimum waves per SIMD requested of the compiler: two resident 1024-thread
+// workgroups (8 waves per SIMD, <= 64 VGPRs) for the FP32 variants with two
+// output cells per thread, which need only a few registers more than that.
+template <typename T, int TX, int TY, int NT>
+constexpr int zreg_waves() {
+    return (sizeof(T) == 4 && NT == 1024 && TX * TY / NT <= 2) ? 8 : 1;
+}
+
 template <typename T, int TX, int TY, int NT, bool EXACT>
-__global__ __launch_bounds__(NT) void k_ch_stage_3dz(CHArgs<T> A) {
+__global__ __attribute__((amdgpu_flat_work_group_size(1, NT), amdgpu_waves_per_eu(zreg_waves<T, TX, TY, NT>())))
+void k_ch_stage_3dz(CHArgs<T> A) {
     constexpr int UX = TX + 4, UY = TY + 4, MX = TX + 2, MY = TY + 2;
     constexpr int NI = TX * TY, NRING = 2 * MX + 2 * TY, NOUT = 2 * UX + 2 * MY;
     static_assert(NI % NT == 0, "tile must be a multiple of the workgroup size");
@@ -1746,8 +1755,8 @@ constexpr int kNumTilesP = sizeof(kTilesP) / sizeof(kTilesP[0]);
 struct TileZ {
     int tx, ty, nt;
 };
-static const TileZ kTilesZ[] = {{64, 16, 256}, {128, 8, 256}, {128, 16, 512},
-                                {256, 8, 512}, {256, 16, 1024}, {256, 4, 256}};
+static const TileZ kTilesZ[] = {{64, 16, 256}, {128, 8, 256},  {128, 16, 512},  {256, 8, 512}, {256, 16, 1024},
+                                {256, 4, 256},  {256, 8, 1024}, {128, 16, 1024}, {128, 8, 512}};
 constexpr int kNumTilesZ = sizeof(kTilesZ) / sizeof(kTilesZ[0]);
 constexpr int kFirstZ = kNumTiles3 + kNumTilesP;
 
@@ -1775,7 +1784,14 @@ static void launch3(int tile, dim3 grid, const CHArgs<T>& A, hipStream_t st) {
         case kFirstZ + 4:
             hipLaunchKernelGGL((k_ch_stage_3dz<T, 256, 16, 1024, EXACT>), grid, dim3(1024), 0, st, A);
             break;
-        default: hipLaunchKernelGGL((k_ch_stage_3dz<T, 256, 4, 256, EXACT>), grid, dim3(256), 0, st, A); break;
+        case kFirstZ + 5: hipLaunchKernelGGL((k_ch_stage_3dz<T, 256, 4, 256, EXACT>), grid, dim3(256), 0, st, A); break;
+        case kFirstZ + 6:
+            hipLaunchKernelGGL((k_ch_stage_3dz<T, 256, 8, 1024, EXACT>), grid, dim3(1024), 0, st, A);
+            break;
+        case kFirstZ + 7:
+            hipLaunchKernelGGL((k_ch_stage_3dz<T, 128, 16, 1024, EXACT>), grid, dim3(1024), 0, st, A);
+            break;
+        default: hipLaunchKernelGGL((k_ch_stage_3dz<T, 128, 8, 512, EXACT>), grid, dim3(512), 0, st, A); break;
     }
 }
 
@@ -1850,7 +1866,7 @@ static void lt_end(int kind, hipStream_t st) {
 static int g_xcd_remap = 1;
 
 // Resident workgroups per CU of the 3D kernel of launch tile `tile` (one-stage
-// 0-4; two-stage 5-8, sized by its (3,4) kernel; z-register 9-14), from the
+// 0-4; two-stage 5-8, sized by its (3,4) kernel; z-register 9-17), from the
 // occupancy calculator; 0 if unknown.
 template <typename T, bool EXACT>
 static int blocks_per_cu(int tile) {
@@ -1871,6 +1887,9 @@ static int blocks_per_cu(int tile) {
         case kFirstZ + 3: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 256, 8, 512, EXACT>); break;
         case kFirstZ + 4: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 256, 16, 1024, EXACT>); break;
         case kFirstZ + 5: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 256, 4, 256, EXACT>); break;
+        case kFirstZ + 6: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 256, 8, 1024, EXACT>); break;
+        case kFirstZ + 7: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 128, 16, 1024, EXACT>); break;
+        case kFirstZ + 8: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 128, 8, 512, EXACT>); break;
         default: return 0;
     }
     int nb = 0;
@@ -2136,7 +2155,7 @@ size_t phx_ch_rk4_workspace_bytes(const phx_grid* G, int dtype) {
 }
 
 // Launch of the RK4 kernels per (dim, shape, dtype): tile index (3D: one-stage
-// 0-4, two-stage 5-8, one-stage with z-neighbours in registers 9-14; 2D:
+// 0-4, two-stage 5-8, one-stage with z-neighbours in registers 9-17; 2D:
 // one-stage 0, whole-step 1-4) and, in 3D, the z
 // planes streamed per workgroup. Set by phx_ch_rk4_set_tile/_set_zchunk or the
 // autotuner; defaults below.
@@ -2244,7 +2263,7 @@ int phx_ch_rk4_steps(const phx_grid* G, int dtype, void* c, void* work, const ph
 // Times every launch candidate on `steps` steps of the caller's state
 // (advancing it) and keeps the fastest per step for this shape: in 3D the
 // one-stage kernel (5 tiles), the two-stage kernel (4 tiles) and the
-// z-register kernel (6 tiles) at z-chunks {4, 8, 16, 32, 64} plus the chunk
+// z-register kernel (9 tiles) at z-chunks {4, 8, 16, 32, 64} plus the chunk
 // counts that fill whole waves of resident workgroups; in 2D the one-stage
 // kernel and the whole-step kernel (4 tiles). Two-stage and whole-step
 // candidates run 2*ceil(steps/2) steps.

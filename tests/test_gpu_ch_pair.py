@@ -123,7 +123,7 @@ def test_pair_tile_rejected_on_small_or_2d_grid(dev):
     g = capi.make_grid((3, 8, 8), bc=N)
     assert capi.lib().phx_ch_rk4_set_tile(C.byref(g), capi.F64, 5) == capi.PHX_ERR_UNSUPPORTED
     assert capi.lib().phx_ch_rk4_set_tile(C.byref(g), capi.F64, 9) == 0      # z-register: any n >= 2
-    assert capi.lib().phx_ch_rk4_set_tile(C.byref(g), capi.F64, 15) == capi.PHX_ERR_INVALID
+    assert capi.lib().phx_ch_rk4_set_tile(C.byref(g), capi.F64, 18) == capi.PHX_ERR_INVALID
     g2 = capi.make_grid((64, 64), bc=P)   # 2D has tiles 0-4 only
     assert capi.lib().phx_ch_rk4_set_tile(C.byref(g2), capi.F32, 7) == capi.PHX_ERR_INVALID
 

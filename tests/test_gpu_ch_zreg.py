@@ -1,6 +1,6 @@
 """GPU parity of the 3D one-stage kernel with z-neighbours in registers.
 
-Launch tiles 9..14 select it (TX x TY columns per workgroup of 64..1024
+Launch tiles 9..17 select it (TX x TY columns per workgroup of 256..1024
 threads). Contract: FP64 results equal the FP64 oracle within 1e-12 max-abs
 on every stored cell of the active region, and both dtypes equal the LDS-ring
 one-stage kernel (tiles 0/1) bit for bit, for every BC kind, ragged shapes
@@ -20,7 +20,7 @@ from test_gpu_ch_pair import launch, run_gpu
 pytestmark = pytest.mark.gpu
 P, N, F = capi.PERIODIC, capi.NOFLUX, capi.FIXED
 TOL64 = 1e-12
-Z_TILES = (9, 10, 11, 12, 13, 14)
+Z_TILES = (9, 10, 11, 12, 13, 14, 15, 16, 17)
 ONE_STAGE = {capi.F32: 0, capi.F64: 1}
 
 CASES = [

@@ -89,10 +89,10 @@ def test_launch_table_defaults_and_ranges():
     assert _launch(g2s, capi.F32) == (1, 0)     # too small for the whole-step kernel
     g1 = capi.make_grid((64,), bc=capi.PERIODIC)
     assert _launch(g1, capi.F32)[1] == 0
-    # ranges: 3D 0-14, 2D 0-4, 1D 0
-    for t in range(15):
+    # ranges: 3D 0-17, 2D 0-4, 1D 0
+    for t in range(18):
         assert lib.phx_ch_rk4_set_tile(C.byref(g3), capi.F32, t) == 0
-    assert lib.phx_ch_rk4_set_tile(C.byref(g3), capi.F32, 15) == capi.PHX_ERR_INVALID
+    assert lib.phx_ch_rk4_set_tile(C.byref(g3), capi.F32, 18) == capi.PHX_ERR_INVALID
     assert lib.phx_ch_rk4_set_tile(C.byref(g3), capi.F32, -1) == capi.PHX_ERR_INVALID
     assert lib.phx_ch_rk4_set_tile(C.byref(g2), capi.F32, 5) == capi.PHX_ERR_INVALID
     assert lib.phx_ch_rk4_set_tile(C.byref(g1), capi.F32, 1) == capi.PHX_ERR_INVALID
@@ -104,7 +104,7 @@ def test_launch_table_defaults_and_ranges():
     assert lib.phx_ch_rk4_set_tile(C.byref(g2s), capi.F32, 3) == capi.PHX_ERR_UNSUPPORTED
     # settings are kept per (dim, shape, dtype)
     assert lib.phx_ch_rk4_set_zchunk(C.byref(g3), capi.F32, 22) == 0
-    assert _launch(g3, capi.F32) == (22, 14)
+    assert _launch(g3, capi.F32) == (22, 17)
     assert _launch(g3, capi.F64) == (32, 11)
     assert lib.phx_ch_rk4_set_zchunk(C.byref(g3), capi.F32, 0) == capi.PHX_ERR_INVALID
     capi.call("phx_ch_rk4_set_tile", C.byref(g3), capi.F32, 11)
