@@ -1768,6 +1768,11 @@ static Tile3 shape3(int tile) {
 }
 static int threads3(int tile) { return tile >= kFirstZ ? kTilesZ[tile - kFirstZ].nt : 256; }
 static bool is_pair_tile(int tile) { return tile >= kNumTiles3 && tile < kFirstZ; }
+// The z-register kernel addresses a plane with 32-bit byte offsets.
+static bool zreg_ok(const phx_grid* G, int dtype) {
+    const Layout L = make_layout(G);
+    return L.sxy * (dtype == PHX_F64 ? 8LL : 4LL) < (1LL << 32);
+}
 
 template <typename T, bool EXACT>
 static void launch3(int tile, dim3 grid, const CHArgs<T>& A, hipStream_t st) {
@@ -2214,6 +2219,8 @@ int phx_ch_rk4_set_tile(const phx_grid* G, int dtype, int tile) {
     if (tile < 0 || tile >= ntiles) return fail(PHX_ERR_INVALID, "phx_ch_rk4_set_tile: tile index out of range");
     if (G->dim == 3 && is_pair_tile(tile) && !pair_ok(G))
         return fail(PHX_ERR_UNSUPPORTED, "phx_ch_rk4_set_tile: two-stage tiles need n >= 4 on every axis");
+    if (G->dim == 3 && tile >= kFirstZ && !zreg_ok(G, dtype))
+        return fail(PHX_ERR_UNSUPPORTED, "phx_ch_rk4_set_tile: z-register tiles need a stored plane below 4 GiB");
     if (G->dim == 2 && tile >= 1 && !step2d_ok(G))
         return fail(PHX_ERR_UNSUPPORTED, "phx_ch_rk4_set_tile: whole-step 2D tiles need n >= 8 on both axes");
     Launch3 l = launch_for(G, dtype);
@@ -2234,6 +2241,7 @@ static int run_rk4(const phx_grid* G, int dtype, void* c, void* work, const phx_
                                 : rk4_step2d_steps_t<float, false>(G, (float*)c, (float*)work, p, nsteps, st,
                                                                    l.tile - 1, pure);
     if (G->dim != 3) l.tile = 0;
+    if (G->dim == 3 && l.tile >= kFirstZ && !zreg_ok(G, dtype)) l.tile = dtype == PHX_F32 ? 0 : 1;
     if (G->dim == 3 && is_pair_tile(l.tile)) {
         if (!pair_ok(G)) l.tile = dtype == PHX_F32 ? 0 : 1;
         else
@@ -2286,6 +2294,7 @@ int phx_ch_rk4_autotune(const phx_grid* G, int dtype, void* c, void* work, const
     PHX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     for (int tile = 0; tile < ntiles; ++tile) {
     if (G->dim == 3 && is_pair_tile(tile) && !pair_ok(G)) continue;
+    if (G->dim == 3 && tile >= kFirstZ && !zreg_ok(G, dtype)) continue;
     // z-chunks: fixed sizes, plus chunk counts that fill whole waves of the
     // resident workgroup slots (no partly filled last wave)
     std::vector<int> zcs(cand, cand + sizeof(cand) / sizeof(cand[0]));

@@ -51,6 +51,10 @@ int check_grid_fused(const phx_grid* G, const char* who) {
                                              std::to_string(a));
         if (G->n[a] < 2) return fail(PHX_ERR_UNSUPPORTED, std::string(who) + ": active axes need n >= 2");
     }
+    // the fused kernels address a stored x-y plane with 32-bit offsets
+    const long long sx = G->n[0] + 2LL * G->ghost, sy = G->dim >= 2 ? G->n[1] + 2LL * G->ghost : 1;
+    if (sx * sy >= (1LL << 31))
+        return fail(PHX_ERR_UNSUPPORTED, std::string(who) + ": a stored x-y plane must hold fewer than 2^31 cells");
     return PHX_OK;
 }
 

@@ -140,3 +140,16 @@ def test_hbm_probe_rejects_bad_arguments():
     assert lib.phx_hbm_probe(4, fake, fake, fake, fake, None, 66 ** 3, 64, 2, 4, None) == capi.PHX_ERR_INVALID
     assert lib.phx_hbm_probe(4, fake, fake, fake, fake, None, 66 ** 3 - 1, 64, 0, 4, None) == capi.PHX_ERR_INVALID
     assert lib.phx_hbm_probe(5, fake, fake, fake, fake, None, 66 ** 3, 64, 0, 65, None) == capi.PHX_ERR_INVALID
+
+
+def test_fused_paths_reject_planes_beyond_32bit_offsets():
+    """A stored x-y plane of 2^31 cells or more is refused before any launch;
+    below that, z-register tiles are accepted for FP64 and FP32 alike."""
+    lib = capi.lib()
+    big = capi.make_grid((46340, 46340, 2), bc=capi.PERIODIC)        # 46342^2 stored cells > 2^31
+    p = capi.CHParams(1.0, 0.0, 1.0, 1.0, 1.0, 0.005)
+    fake = C.c_void_p(0x1000)
+    assert lib.phx_ch_rk4_steps(C.byref(big), capi.F32, fake, fake, C.byref(p), 1, None) == capi.PHX_ERR_UNSUPPORTED
+    ok = capi.make_grid((30000, 30000, 2), bc=capi.PERIODIC)          # 3.6 GB FP32 plane: z-register ok
+    assert lib.phx_ch_rk4_set_tile(C.byref(ok), capi.F32, 11) == 0
+    assert lib.phx_ch_rk4_set_tile(C.byref(ok), capi.F64, 11) == capi.PHX_ERR_UNSUPPORTED   # 7.2 GB FP64 plane
