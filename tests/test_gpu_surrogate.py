@@ -272,3 +272,57 @@ def test_trained_surrogate_gains_over_untrained(dev):
     drift_pr = abs(outp.double().mean().item() - c0.mean())
     print(f"mean drift vs FP64 c0 -- plain FP32 solver {drift_plain:.2e}, parareal {drift_pr:.2e}")
     assert drift_pr <= 5 * max(drift_plain, 1e-7)
+
+
+def test_rollout_graph_replays_equal_eager(dev):
+    """The captured batch-1 rollout equals the eager one bit for bit on every
+    replay, with eager work on the same workspace between replays."""
+    import torch
+    from phix_amd import surrogate as S
+    params = np.load(S.WEIGHTS)["params"]
+    n = 64
+    net = S.Surrogate(dev, 1, n=n, params=params)
+    g = S.RolloutGraph(net, 5)
+    bufs = [torch.empty((1, n, n), dtype=torch.float32, device=dev) for _ in range(2)]
+    rng = np.random.default_rng(11)
+    for trial in range(3):
+        u = torch.from_numpy((0.5 + 0.1 * rng.standard_normal((1, n, n))).astype(np.float32)).to(dev)
+        got = g(u).clone()
+        ref = net.rollout(u, 5, bufs).clone()
+        net.forward(u)   # eager work between replays
+        torch.cuda.synchronize()
+        assert torch.equal(got, ref), trial
+
+
+def test_parareal_with_solver_lead_phase(dev):
+    """The parareal form with a plain-solver lead phase: with one slice, one
+    iteration is the fine propagation itself (the plain solver's trajectory
+    up to FP32 rounding: ensemble and single-grid kernels round differently);
+    with 19 slices after a 300-step lead, the late-phase weights end within
+    the stated distance of the FP64 solver at a 6000-step horizon."""
+    import torch
+    from phix_amd import init
+    from phix_amd import surrogate as S
+    c0 = init.uniform(init.substream(1234, 7), 256 * 256, 0.45, 0.55).reshape(256, 256)
+    x0 = torch.from_numpy(c0).float().to(dev)
+    lead, nsteps = 300, 6000
+    plain = S.PlainSolver(dev)
+    plain.set(x0)
+    plain.steps(nsteps)
+    torch.cuda.synchronize()
+    ref32 = plain.phys().cpu().numpy()
+    params = np.load(S.WEIGHTS_LATE)["params"]
+    P1 = S.PararealSolver(dev, params, 1, nsteps - lead)
+    out1, rep1 = P1.run(x0, 1, lead)
+    P1.close()
+    assert rep1.solver_steps == nsteps
+    d1 = float(np.linalg.norm(out1.cpu().numpy() - ref32) / np.linalg.norm(ref32))
+    print(f"one slice, one iteration vs plain FP32 solver: rel-L2 {d1:.2e}")
+    assert d1 <= 1e-4
+    ref = S.fp64_reference(dev, c0, nsteps)
+    P = S.PararealSolver(dev, params, 19, (nsteps - lead) // 19)
+    out, rep = P.run(x0, 1, lead)
+    P.close()
+    e = float(np.linalg.norm(out.cpu().numpy() - ref) / np.linalg.norm(ref))
+    print(f"lead {lead} + 19 slices, one iteration, 6000 steps: rel-L2 vs FP64 {e:.2e} in {rep.seconds * 1e3:.1f} ms")
+    assert e <= 5e-2
