@@ -86,10 +86,12 @@ __global__ __launch_bounds__(256) void k_ac_stage_2d(ACArgs<T> A) {
     for (int f = 0; f < NF; ++f)
 #pragma unroll
         for (int i = 0; i < NS; ++i) hr[f][i] = A.u[f][zb + off[i]];
-    // epilogue operands; a stage that does not need one reads the stage state
+    // epilogue operands; one the stage does not need is read at element 0 (below)
     const int st = A.stage;
     const bool need_c = st == 2 || st == 3 || (EXACT && st == 4);
     const bool need_a = st >= 2;
+    // an operand the stage does not need: every lane reads element 0 (one line)
+    const long long cmask = need_c ? -1LL : 0LL, amask = need_a ? -1LL : 0LL;
     T pc[NF][R], pa[NF][R];
     long long ogi[R];
 #pragma unroll
@@ -105,8 +107,8 @@ __global__ __launch_bounds__(256) void k_ac_stage_2d(ACArgs<T> A) {
         const T* as = need_a ? A.acc[f] : A.u[f];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            pc[f][r] = cs[ogi[r]];
-            pa[f][r] = as[ogi[r]];
+            pc[f][r] = cs[ogi[r] & cmask];
+            pa[f][r] = as[ogi[r] & amask];
         }
     }
 #pragma unroll

@@ -262,21 +262,24 @@ __global__ __launch_bounds__(256) void k_ch_stage_2d(CHArgs<T> A) {
     T r[NS];
 #pragma unroll
     for (int i = 0; i < NS; ++i) r[i] = A.u[zb + off[i]];
-    // epilogue operands of the tile cells, in flight with the halo; a stage
-    // that does not need c_n or acc reads the stage state instead
+    // epilogue operands of the tile cells, in flight with the halo; an
+    // operand the stage does not need is read at element 0 of the stage state
+    // by every lane (one cache line)
     constexpr int NO = (TX * TY) / 256;
     static_assert((TX * TY) % 256 == 0, "tile must be a multiple of 256 cells");
     const int st = A.stage;
-    const T* csrc = (st == 2 || st == 3 || (EXACT && st == 4)) ? A.c : A.u;
-    const T* asrc = st >= 2 ? A.acc : A.u;
+    const bool need_c = st == 2 || st == 3 || (EXACT && st == 4), need_a = st >= 2;
+    const T* csrc = need_c ? A.c : A.u;
+    const T* asrc = need_a ? A.acc : A.u;
+    const long long cmask = need_c ? -1LL : 0LL, amask = need_a ? -1LL : 0LL;
     T pc[NO], pa[NO];
 #pragma unroll
     for (int j = 0; j < NO; ++j) {
         const int t = tid + 256 * j;
         const int oy = t / TX, ox = t - oy * TX;
         const long long gi = A.L.at(min(x0 + ox, A.L.n0 - 1), DIM >= 2 ? min(y0 + oy, A.L.n1 - 1) : 0, kz);
-        pc[j] = csrc[gi];
-        pa[j] = asrc[gi];
+        pc[j] = csrc[gi & cmask];
+        pa[j] = asrc[gi & amask];
     }
 #pragma unroll
     for (int i = 0; i < NS; ++i) {
@@ -497,19 +500,20 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
     const int st = A.stage;
     // Both epilogue operands are loaded for every stage so that each
     // iteration issues the same loads (the compiler then waits for exactly the
-    // ones it uses); a stage that needs neither c_n nor acc reads the stage
-    // state instead, whose cells are in cache from the halo load.
+    // ones it uses); an operand the stage does not need is read from the
+    // first element of the stage state for every lane (one cache line).
     const bool need_c = st == 2 || st == 3 || (EXACT && st == 4);
     const bool need_a = st >= 2;
     const T* csrc = need_c ? A.c : A.u;
     const T* asrc = need_a ? A.acc : A.u;
+    const long long cmask = need_c ? -1LL : 0LL, amask = need_a ? -1LL : 0LL;
     T pc[NO], pa[NO], nc[NO], na[NO];
     auto load_epi = [&](int o, T (&rc)[NO], T (&ra)[NO]) {
         const long long pb = (long long)(o + g) * sxy;
 #pragma unroll
         for (int j = 0; j < NO; ++j) {
-            rc[j] = csrc[pb + ooff[j]];
-            ra[j] = asrc[pb + ooff[j]];
+            rc[j] = csrc[(pb + ooff[j]) & cmask];
+            ra[j] = asrc[(pb + ooff[j]) & amask];
         }
     };
 
@@ -828,18 +832,21 @@ void k_ch_stage_3dz(CHArgs<T> A) {
         return v;
     };
     // epilogue operands (both loaded for every stage: the same loads each
-    // iteration; a stage that needs neither reads the stage state)
+    // iteration; an operand the stage does not need is read from the first
+    // element of the stage state for every lane: one cache line per wave)
     const int st = A.stage;
     const bool need_c = st == 2 || st == 3 || (EXACT && st == 4);
     const bool need_a = st >= 2;
     const T* csrc = need_c ? A.c : A.u;
     const T* asrc = need_a ? A.acc : A.u;
+    const long long cpm = need_c ? -1LL : 0LL, apm = need_a ? -1LL : 0LL;
+    const unsigned com = need_c ? ~0u : 0u, aom = need_a ? ~0u : 0u;
     auto load_epi = [&](int o, T (&rc)[NO], T (&ra)[NO]) {
         const long long pb = (long long)(o + g) * sxy;
 #pragma unroll
         for (int j = 0; j < NO; ++j) {
-            rc[j] = at(csrc + pb, ooff[j]);
-            ra[j] = at(asrc + pb, ooff[j]);
+            rc[j] = at(csrc + (pb & cpm), ooff[j] & com);
+            ra[j] = at(asrc + (pb & apm), ooff[j] & aom);
         }
     };
 

@@ -68,11 +68,16 @@ __global__ __launch_bounds__(256) void k_ens_stage(EnsArgs A) {
     const int G = gridDim.x, last = A.ntiles - 1;
     int w = blockIdx.x;
     if (w > last) return;
-    // epilogue sources: an operand the stage does not need is read from the
-    // stage state instead (in cache from the halo load)
+    // epilogue sources: an operand the stage does not need is still loaded
+    // (every iteration issues the same loads), from the arena's first element
+    // for every lane -- one cache line per wave instead of a second pass over
+    // the stage state (the next tile's operands are in flight before its halo
+    // has reached the cache)
     const int st = A.stage;
-    const float* csrc = (st == 2 || st == 3) ? A.c : A.u;
-    const float* asrc = st >= 2 ? A.acc : A.u;
+    const bool need_c = st == 2 || st == 3, need_a = st >= 2;
+    const float* csrc = need_c ? A.c : A.u;
+    const float* asrc = need_a ? A.acc : A.u;
+    const long long cmask = need_c ? -1LL : 0LL, amask = need_a ? -1LL : 0LL;
 
     float h[NS], pc[NO], pa[NO];
     auto load_halo = [&](const EnsTile& T, const EnsRun& R) {
@@ -93,8 +98,8 @@ __global__ __launch_bounds__(256) void k_ens_stage(EnsArgs A) {
             const int t = tid + 256 * j;
             const int oy = t / TX, ox = t - oy * TX;
             const long long gi = R.off + (long long)min(T.y0 + oy, n - 1) * n + min(T.x0 + ox, n - 1);
-            pc[j] = csrc[gi];
-            pa[j] = asrc[gi];
+            pc[j] = csrc[gi & cmask];
+            pa[j] = asrc[gi & amask];
         }
     };
 
