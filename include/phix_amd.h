@@ -270,7 +270,8 @@ int phx_ensemble_graph_steps(phx_ensemble* ens, float* c, float* work, double dt
 /* out_dev[r] (device, nruns doubles) <- FP64 compensated sum of run r. */
 int phx_ensemble_mass(const phx_ensemble* ens, const float* c, double* out_dev, phx_stream stream);
 /* Tile shape index (0..4: 64x16, 32x16, 64x8, 128x8, 32x32) and persistent
- * grid size; grid <= 0 picks 8 workgroups per CU. */
+ * grid size; grid <= 0 picks 8 workgroups per CU. Every shape returns the
+ * same bits. */
 int phx_ensemble_set_tiling(phx_ensemble* ens, int shape, int grid);
 int phx_ensemble_get_tiling(const phx_ensemble* ens, int* shape, int* grid, int* tx, int* ty);
 /* Times every tile shape x {2,4,8,16} workgroups per CU on `steps` steps of

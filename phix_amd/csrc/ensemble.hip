@@ -11,6 +11,10 @@
 // Per-run mass is reduced in FP64 with Neumaier-compensated sums, one
 // workgroup per run.
 // Tile shape and grid size come from phx_ensemble_autotune (timed candidates).
+// Every kernel evaluates a cell through the same helpers (explicit FMAs,
+// contraction off), so every tiling returns the same bits.
+#pragma clang fp contract(off)
+
 #include "common.hpp"
 
 #include <algorithm>
@@ -48,6 +52,31 @@ struct EnsArgs {
 
 // Periodic wrap for |p - n| < n.
 __device__ __forceinline__ int wrapp(int p, int n) { return p < 0 ? p + n : (p >= n ? p - n : p); }
+
+// mu = f'(c) - kappa*lap(c) on a unit grid, f = c^2(1-c)^2:
+// lap = (xp + xm) + (yp + ym) - 4c, f'(c) = 2c (c - 1) (2c - 1).
+__device__ __forceinline__ float ens_mu(float c, float xp, float xm, float yp, float ym, float nk) {
+    const float lap = __builtin_fmaf(-4.f, c, (xp + xm) + (yp + ym));
+    const float fp = (2.f * c) * (c - 1.f) * __builtin_fmaf(2.f, c, -1.f);
+    return __builtin_fmaf(nk, lap, fp);
+}
+// k = M * lap(mu)
+__device__ __forceinline__ float ens_k(float m, float xp, float xm, float yp, float ym, float M) {
+    return M * __builtin_fmaf(-4.f, m, (xp + xm) + (yp + ym));
+}
+// RK4 stage epilogue (FP32 16-pass form)
+__device__ __forceinline__ void ens_epilogue(const EnsArgs& A, long long gi, int st, float uc, float k, float cn,
+                                             float an) {
+    if (st == 1) {
+        A.acc[gi] = __builtin_fmaf(A.dt6, k, uc);
+        A.out[gi] = __builtin_fmaf(A.a, k, uc);
+    } else if (st < 4) {
+        A.acc[gi] = __builtin_fmaf(A.dt3, k, an);
+        A.out[gi] = __builtin_fmaf(A.a, k, cn);
+    } else {
+        A.out[gi] = __builtin_fmaf(A.dt6, k, an);
+    }
+}
 
 // One persistent workgroup walks tiles w, w + G, w + 2G, ... (G = grid size)
 // and is software pipelined across them: the next tile's halo is in flight
@@ -149,11 +178,7 @@ __global__ __launch_bounds__(256) void k_ens_stage(EnsArgs A) {
         for (int t = tid; t < MX * MY; t += 256) {
             int my = t / MX, mx = t - my * MX;
             const int c0 = (my + 1) * UX + mx + 1;
-            float cc = U[c0];
-            float lap = (U[c0 + 1] - 2.f * cc + U[c0 - 1]) + (U[c0 + UX] - 2.f * cc + U[c0 - UX]);
-            // f'(c) = 2c(c-1)(2c-1) for f = c^2(1-c)^2
-            float fp = 2.f * cc * (cc - 1.f) * (2.f * cc - 1.f);
-            MU[t] = fp + nk * lap;
+            MU[t] = ens_mu(U[c0], U[c0 + 1], U[c0 - 1], U[c0 + UX], U[c0 - UX], nk);
         }
         __syncthreads();
 #pragma unroll
@@ -163,20 +188,9 @@ __global__ __launch_bounds__(256) void k_ens_stage(EnsArgs A) {
             const int x = T.x0 + ox, y = T.y0 + oy;
             if (x < n && y < n) {
                 const int m0 = (oy + 1) * MX + ox + 1;
-                float m = MU[m0];
-                float lm = (MU[m0 + 1] - 2.f * m + MU[m0 - 1]) + (MU[m0 + MX] - 2.f * m + MU[m0 - MX]);
-                float k = R.M * lm;
-                long long gi = off + (long long)y * n + x;
-                float uc = U[(oy + 2) * UX + ox + 2];
-                if (st == 1) {
-                    A.acc[gi] = uc + A.dt6 * k;
-                    A.out[gi] = uc + A.a * k;
-                } else if (st < 4) {
-                    A.acc[gi] = pa[j] + A.dt3 * k;
-                    A.out[gi] = pc[j] + A.a * k;
-                } else {
-                    A.out[gi] = pa[j] + A.dt6 * k;
-                }
+                const float k = ens_k(MU[m0], MU[m0 + 1], MU[m0 - 1], MU[m0 + MX], MU[m0 - MX], R.M);
+                const long long gi = off + (long long)y * n + x;
+                ens_epilogue(A, gi, st, U[(oy + 2) * UX + ox + 2], k, pc[j], pa[j]);
             }
         }
         w += G;
