@@ -63,9 +63,11 @@ def build_lib(jobs: int = 8, verbose: bool = True) -> Path:
 
 
 def build_cpp_tests(jobs: int = 8, verbose: bool = True) -> list[Path]:
-    """Compiles tests/cpp/*.cpp (programs written against the drop-in C++ API)
-    as HIP and links them to libphix_amd.so (rpath to the package dir)."""
-    srcs = sorted((ROOT / "tests" / "cpp").glob("*.cpp"))
+    """Compiles the programs written against the drop-in C++ API -- the test
+    drivers tests/cpp/*.cpp and the config-driven solvers phix_amd/apps/*.cpp --
+    as HIP into phix_amd/_build/bin, linked to libphix_amd.so (rpath to the
+    package dir)."""
+    srcs = sorted((ROOT / "tests" / "cpp").glob("*.cpp")) + sorted((PKG / "apps").glob("*.cpp"))
     out_dir = OBJ / "bin"
     out_dir.mkdir(parents=True, exist_ok=True)
 
@@ -81,7 +83,7 @@ def build_cpp_tests(jobs: int = 8, verbose: bool = True) -> list[Path]:
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         exes = list(ex.map(one, srcs))
     if verbose:
-        print(f"built {len(exes)} C++ API test programs in {out_dir}", flush=True)
+        print(f"built {len(exes)} C++ API programs in {out_dir}", flush=True)
     return exes
 
 

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cctype>
 #include <fstream>
+#include <iomanip>
 #include <iostream>
 #include <limits>
 #include <sstream>
@@ -318,6 +319,39 @@ Header readHeader(std::istream& is, const std::string& path, const char* magic) 
     return h;
 }
 
+// VTK XML StructuredGrid of the physical cells: corner points (n+1 per
+// axis, from origin + i*d), cell-centred data with one array of `ncomp`
+// components (the layout of the reference's VTS writer, FieldIO.cpp:263-312).
+void writeVts(const std::string& path, const Mesh& mesh, const std::string& name,
+              const std::vector<const ScalarField*>& comps) {
+    std::ofstream os(path);
+    if (!os) throw std::runtime_error("IO::writeField: cannot open file: " + path);
+    const int nx = mesh.n[0], ny = mesh.n[1], nz = mesh.n[2];
+    const std::string ext = "0 " + std::to_string(nx) + " 0 " + std::to_string(ny) + " 0 " + std::to_string(nz);
+    os << "<?xml version=\"1.0\"?>\n<VTKFile type=\"StructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\">\n"
+       << "  <StructuredGrid WholeExtent=\"" << ext << "\">\n    <Piece Extent=\"" << ext << "\">\n"
+       << "      <Points>\n        <DataArray type=\"Float64\" NumberOfComponents=\"3\" format=\"ascii\">\n";
+    os << std::scientific << std::setprecision(12);
+    for (int k = 0; k <= nz; ++k)
+        for (int j = 0; j <= ny; ++j)
+            for (int i = 0; i <= nx; ++i)
+                os << "          " << mesh.origin[0] + i * mesh.d[0] << " " << mesh.origin[1] + j * mesh.d[1] << " "
+                   << mesh.origin[2] + k * mesh.d[2] << "\n";
+    os << "        </DataArray>\n      </Points>\n      <CellData " << (comps.size() == 1 ? "Scalars" : "Vectors")
+       << "=\"" << name << "\">\n        <DataArray type=\"Float64\" Name=\"" << name << "\"";
+    if (comps.size() > 1) os << " NumberOfComponents=\"" << comps.size() << "\"";
+    os << " format=\"ascii\">\n";
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                os << "         ";
+                for (const ScalarField* f : comps) os << " " << f->curr[(std::size_t)f->index(i, j, k)];
+                os << "\n";
+            }
+    os << "        </DataArray>\n      </CellData>\n    </Piece>\n  </StructuredGrid>\n</VTKFile>\n";
+    if (!os) throw std::runtime_error("IO::writeField: write failed: " + path);
+}
+
 void checkDims(const Header& h, const Mesh& m, const std::string& path) {
     if (h.nx != m.n[0] || h.ny != m.n[1] || h.nz != m.n[2])
         throw std::runtime_error("IO: mesh dimensions in file (" + std::to_string(h.nx) + "x" + std::to_string(h.ny) +
@@ -343,14 +377,16 @@ void ScalarField::write(const std::string& path, FieldFormat fmt) const {
     } else if (fmt == FieldFormat::DAT) {
         std::ofstream os(path);
         if (!os) throw std::runtime_error("IO::writeField: cannot open file: " + path);
-        os << "# PhiX ScalarField - DAT\n# name " << name << "\n# nx " << mesh.n[0] << " ny " << mesh.n[1] << " nz "
-           << mesh.n[2] << "\n# x y z value\n";
-        os.precision(17);
+        // header lines as the reference writes them (FieldIO.cpp:244-248);
+        // values in scientific notation with 17 significant digits (lossless)
+        os << "# PhiX ScalarField - DAT\n# name: " << name << "\n# nx " << mesh.n[0] << "  ny " << mesh.n[1]
+           << "  nz " << mesh.n[2] << "\n# x y z value\n";
+        os << std::scientific << std::setprecision(16);
         forPhys(*this, [&](int i, int j, int k, std::size_t q) {
             os << mesh.coord(0, i) << " " << mesh.coord(1, j) << " " << mesh.coord(2, k) << " " << curr[q] << "\n";
         });
     } else {
-        throw std::runtime_error("IO::writeField: VTS output is not supported by this library");
+        writeVts(path, mesh, name, {this});
     }
 }
 
@@ -443,7 +479,29 @@ void VectorField::downloadPrevFromDevice() { for (auto& f : components_) f.downl
 void VectorField::downloadAllFromDevice() { for (auto& f : components_) f.downloadAllFromDevice(); }
 
 void VectorField::write(const std::string& path, FieldFormat fmt) const {
-    if (fmt != FieldFormat::BINARY) throw std::runtime_error("VectorField::write: only BINARY is supported by this library");
+    for (const auto& f : components_)
+        if (f.curr.size() != f.storedSize) throw std::runtime_error("VectorField::write: field has no host storage");
+    if (fmt == FieldFormat::VTS) {
+        std::vector<const ScalarField*> comps;
+        for (const auto& f : components_) comps.push_back(&f);
+        writeVts(path, mesh, name, comps);
+        return;
+    }
+    if (fmt == FieldFormat::DAT) {
+        std::ofstream os(path);
+        if (!os) throw std::runtime_error("IO::writeField: cannot open file: " + path);
+        // header lines as the reference writes them (FieldIO.cpp:358-363)
+        os << "# PhiX VectorField - DAT\n# name: " << name << "  nComponents: " << nComponents() << "\n# nx "
+           << mesh.n[0] << "  ny " << mesh.n[1] << "  nz " << mesh.n[2] << "\n# x y z";
+        for (int c = 0; c < nComponents(); ++c) os << " v" << c;
+        os << "\n" << std::scientific << std::setprecision(16);
+        forPhys(components_[0], [&](int i, int j, int k, std::size_t q) {
+            os << mesh.coord(0, i) << " " << mesh.coord(1, j) << " " << mesh.coord(2, k);
+            for (const auto& f : components_) os << " " << f.curr[q];
+            os << "\n";
+        });
+        return;
+    }
     std::ofstream os(path, std::ios::binary);
     if (!os) throw std::runtime_error("IO::writeField: cannot open file: " + path);
     os << "# PhiX VectorField\nname         " << name << "\nnComponents  " << nComponents() << "\nnx " << mesh.n[0]

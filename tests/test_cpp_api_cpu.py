@@ -14,3 +14,32 @@ def test_fieldops_known_answers_cpu():
     print(r.stdout[-2000:])
     assert r.returncode == 0, r.stdout + r.stderr
     assert "FAIL" not in r.stdout
+
+
+GOLDEN = ROOT / "tests" / "golden"
+
+
+def test_config_io_cpu(tmp_path):
+    """JSONC parser, ConfigFile on the reference's two sample settings files,
+    buildBCs rules, OutputWriter files and cadence, .field/.dat round trips and
+    restart resolution (tests/cpp/config_io.cpp), host code only."""
+    r = subprocess.run([str(BIN / "config_io"), str(tmp_path / "scratch"), str(GOLDEN / "settings_ch_2d.jsonc"),
+                        str(GOLDEN / "settings_ch_ac_2d.jsonc")], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-4000:], r.stderr[-2000:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "FAIL" not in r.stdout
+    assert " of " in r.stdout.splitlines()[-1]
+
+
+def test_driver_usage_on_bad_settings(tmp_path):
+    """The config-driven solvers exit 1 with the error and a usage line when the
+    settings file is missing or malformed, before touching the device."""
+    bad = tmp_path / "broken.jsonc"
+    bad.write_text('{ "mesh": { "nx": 8, } }\n')
+    for exe in ("ch_2d", "ch_ac_2d"):
+        for arg in (str(tmp_path / "absent.jsonc"), str(bad)):
+            r = subprocess.run([str(BIN / exe), arg], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+            assert r.returncode == 1, (exe, arg, r.stdout, r.stderr)
+            assert "Usage: " in r.stderr and "[path/to/settings.jsonc]" in r.stderr, r.stderr
+        r = subprocess.run([str(BIN / exe)], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+        assert r.returncode == 1 and "settings/settings.jsonc" in r.stderr, r.stderr

@@ -1,0 +1,160 @@
+"""GPU tests of the config-driven solver programs (phix_amd/apps): ch_2d and
+ch_ac_2d run from a settings file in a scratch working directory the way the
+reference's CH_2D / CH_AC_2D programs do (settings/settings.jsonc,
+settings/initial_field/*.field, output/<name>_<step>.*).
+
+* ch_2d: the fused Euler plan against the FP64 oracle (<= 1e-12), periodic
+  and no-flux; restart from a written step reproduces the uninterrupted run
+  bit for bit; the DAT output holds the same values as the BINARY one; the
+  optional FP32 bulk setting stays within FP32 accuracy.
+* ch_ac_2d: the coupled CH + AC system on the generic device path against a
+  numpy FP64 restatement of the reference program's equations and multi-step
+  order (CH+AC_double-well.cu:95-136, Solver.cu multi-step: mu, then c, then
+  eta reading the updated c); tolerance 1e-11 (different summation order and
+  contraction of the per-term kernels)."""
+import json
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from phix_amd import capi, init
+from oracle import oracle as O
+
+from helpers import ch_params, write_field, read_field
+
+pytestmark = pytest.mark.gpu
+ROOT = Path(__file__).resolve().parent.parent
+BIN = ROOT / "phix_amd" / "_build" / "bin"
+
+CH_CONST = {"M": 5.0, "kappa": 2.0, "rho": 5.0, "ca": 0.3, "cb": 0.7}   # settings_ch_2d.jsonc
+CHAC_CONST = {"M": 5.0, "rho": 1.4142135623730951, "ca": 0.3, "cb": 0.7, "kappa_c": 3.0, "kappa_eta": 3.0,
+              "L": 5.0, "w": 1.0}                                         # settings_ch_ac_2d.jsonc
+
+
+def settings(nx, ny, const, nsteps, start="initial_field", dt=0.001, bcs=("Periodic",) * 4, fmt="ALL",
+             print_every=100, write_every=100, extra=None):
+    s = {"mesh": {"nx": nx, "ny": ny, "dx": 1.0, "dy": 1.0, "x0": 0.0, "y0": 0.0},
+         "initialize": {"start_from": start, "dt": dt, "nSteps": nsteps},
+         "boundary_conditions": dict(zip(("x_min", "x_max", "y_min", "y_max"), bcs)),
+         "constants": const,
+         "output": {"print_interval": print_every, "write_interval": write_every, "format": fmt}}
+    if extra:
+        s["initialize"].update(extra)
+    return "// generated for the test\n" + json.dumps(s, indent=2) + "\n"
+
+
+def prepare(work, text, fields):
+    (work / "settings" / "initial_field").mkdir(parents=True, exist_ok=True)
+    (work / "settings" / "settings.jsonc").write_text(text)
+    for name, phys in fields.items():
+        write_field(work / "settings" / "initial_field" / f"{name}.field", phys, name=name)
+
+
+def run(exe, work, *args, timeout=300):
+    print(f"running {exe} in {work}", flush=True)
+    r = subprocess.run([str(BIN / exe), *map(str, args)], capture_output=True, text=True, timeout=timeout, cwd=work)
+    print(r.stdout[-3000:], r.stderr[-3000:])
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+def dat_values(path):
+    rows = np.loadtxt(path, comments="#")
+    return rows[:, 3]
+
+
+@pytest.mark.parametrize("bcs", [("Periodic",) * 4, ("NoFlux", "NoFlux", "Periodic", "Periodic")])
+def test_ch_2d_matches_oracle_and_restarts(dev, tmp_path, bcs):
+    nx, ny, nsteps = 64, 48, 300
+    c0 = init.ch_field((nx, ny), 11, 0.5, 0.02)
+    work = tmp_path / "run"
+    prepare(work, settings(nx, ny, CH_CONST, nsteps, bcs=bcs, write_every=150), {"c": c0, "mu": np.zeros_like(c0)})
+    out = run("ch_2d", work)
+    assert "solver plan: fused CH Euler" in out
+    assert "[progress] step=300" in out and "Done." in out
+    for step in (0, 150, 300):
+        for ext in ("field", "dat", "vts"):
+            assert (work / "output" / f"c_{step}.{ext}").exists(), (step, ext)
+
+    P, N = capi.PERIODIC, capi.NOFLUX
+    bc = [[P, P], [P, P]] if bcs[0] == "Periodic" else [[N, N], [P, P]]
+    g = capi.make_grid((nx, ny), bc=bc)
+    p = ch_params(rho=5.0, ca=0.3, cb=0.7, kappa=2.0, M=5.0, dt=0.001)
+    c = init.to_stored(c0, g)
+    mu = np.zeros_like(c)
+    O.ch_euler_steps(g, p, c, mu, 150)
+    _, c150 = read_field(work / "output" / "c_150.field")
+    assert np.abs(c150[0] - capi.physical(c, g)).max() <= 1e-12
+    O.ch_euler_steps(g, p, c, mu, 150)
+    _, c300 = read_field(work / "output" / "c_300.field")
+    err = np.abs(c300[0] - capi.physical(c, g)).max()
+    print(f"ch_2d {bcs[0]}: max |c - oracle| after {nsteps} steps = {err:.3e}")
+    assert err <= 1e-12
+    # DAT carries the same values (17 significant digits)
+    assert np.array_equal(dat_values(work / "output" / "c_300.dat"), c300[0].ravel())
+
+    # warm start: "160" resolves to the nearest written step, 150
+    first = (work / "output" / "c_300.field").read_bytes()
+    (work / "settings" / "settings.jsonc").write_text(
+        settings(nx, ny, CH_CONST, nsteps, start="160", bcs=bcs, write_every=150))
+    out = run("ch_2d", work)
+    assert "warm start: loading output/c_150.field" in out
+    assert "Resuming Cahn-Hilliard simulation from step 150" in out
+    assert (work / "output" / "c_300.field").read_bytes() == first
+
+
+def test_ch_2d_fp32_bulk(dev, tmp_path):
+    nx, ny, nsteps = 64, 64, 200
+    c0 = init.ch_field((nx, ny), 12, 0.5, 0.02)
+    work = tmp_path / "run"
+    prepare(work, settings(nx, ny, CH_CONST, nsteps, fmt="BINARY", write_every=200,
+                           extra={"bulk_precision": "FP32"}), {"c": c0, "mu": np.zeros_like(c0)})
+    out = run("ch_2d", work, "settings/settings.jsonc")
+    assert "solver plan: fused CH Euler" in out
+    g = capi.make_grid((nx, ny), bc=capi.PERIODIC)
+    c = init.to_stored(c0, g)
+    O.ch_euler_steps(g, ch_params(rho=5.0, ca=0.3, cb=0.7, kappa=2.0, M=5.0, dt=0.001), c, np.zeros_like(c), nsteps)
+    _, cc = read_field(work / "output" / f"c_{nsteps}.field")
+    err = np.abs(cc[0] - capi.physical(c, g)).max()
+    print(f"ch_2d FP32 bulk: max |c - oracle| = {err:.3e}")
+    assert 0 < err <= 1e-5   # FP32 arithmetic, so not bit-equal to FP64
+
+
+def _lap_periodic(f):
+    return (np.roll(f, 1, 1) + np.roll(f, -1, 1) - 2.0 * f) + (np.roll(f, 1, 0) + np.roll(f, -1, 0) - 2.0 * f)
+
+
+def chac_reference(c, eta, k, dt, nsteps):
+    """FP64 numpy restatement of the CH_AC_2D equations (dx = dy = 1,
+    periodic), one multi-step Euler step = mu from c and eta; c += dt M lap mu;
+    eta += dt RHS(eta, updated c)."""
+    rho, ca, cb = k["rho"], k["ca"], k["cb"]
+    c, eta = c.copy(), eta.copy()
+    for _ in range(nsteps):
+        h = eta ** 3 * (6.0 * eta ** 2 - 15.0 * eta + 10.0)
+        mu = 2.0 * rho * rho * (c - ca) + 2.0 * rho * rho * (ca - cb) * h - k["kappa_c"] * _lap_periodic(c)
+        c = c + dt * (k["M"] * _lap_periodic(mu))
+        bulk = 30.0 * rho * rho * eta ** 2 * (1.0 - eta) ** 2 * (2.0 * c - ca - cb) * (ca - cb)
+        well = 2.0 * k["w"] * eta * (1.0 - eta) * (1.0 - 2.0 * eta)
+        eta = eta + dt * (-k["L"] * (bulk + well) + k["L"] * k["kappa_eta"] * _lap_periodic(eta))
+    return c, eta
+
+
+def test_ch_ac_2d_matches_numpy(dev, tmp_path):
+    nx, ny, nsteps = 48, 40, 200
+    c0 = init.ch_field((nx, ny), 13, 0.5, 0.05)
+    eta0 = init.ch_field((nx, ny), 14, 0.5, 0.3)
+    work = tmp_path / "run"
+    prepare(work, settings(nx, ny, CHAC_CONST, nsteps, fmt="BINARY", print_every=50, write_every=200),
+            {"c": c0, "eta": eta0, "mu": np.zeros_like(c0)})
+    out = run("ch_ac_2d", work)
+    assert "solver plan: generic" in out and "Done." in out
+    c_ref, eta_ref = chac_reference(c0[0], eta0[0], CHAC_CONST, 0.001, nsteps)
+    _, cc = read_field(work / "output" / f"c_{nsteps}.field")
+    _, ee = read_field(work / "output" / f"eta_{nsteps}.field")
+    ec, ee_ = np.abs(cc[0, 0] - c_ref).max(), np.abs(ee[0, 0] - eta_ref).max()
+    print(f"ch_ac_2d: max |c - numpy| = {ec:.3e}, max |eta - numpy| = {ee_:.3e}")
+    assert np.abs(eta_ref - eta0[0]).max() > 1e-3   # the run moved the fields
+    assert ec <= 1e-11 and ee_ <= 1e-11
