@@ -228,7 +228,15 @@ __global__ __launch_bounds__(256) void k_ch_stage_2d(CHArgs<T> A) {
     __shared__ T MU[MY][MX];
     __shared__ T BCV[6];
     const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    int bx = blockIdx.x, by = blockIdx.y;
+    if (A.xcd_remap) {   // XCD-aware tile order (row bands per XCD), as in k_ch_stage_3d
+        const int gx = gridDim.x, nb = gx * gridDim.y;
+        const int h = bx + gx * by;
+        const int lb = (h & 7) * (nb >> 3) + (h >> 3);
+        bx = lb % gx;
+        by = lb / gx;
+    }
+    const int x0 = bx * TX, y0 = by * TY;
     const int kz = 0;
     stage_bc_values(BCV, A);
 
@@ -1942,7 +1950,10 @@ static int rk4_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_params* p,
         grid = dim3(cdiv(A.L.n0, tl.tx), cdiv(A.L.n1, tl.ty), cdiv(A.L.n2, zc));
         A.xcd_remap = g_xcd_remap && (grid.x * grid.y * grid.z) % 8 == 0;
     }
-    else grid = dim3(cdiv(A.L.n0, T2X), cdiv(A.L.n1, T2Y), 1);
+    else {
+        grid = dim3(cdiv(A.L.n0, T2X), cdiv(A.L.n1, T2Y), 1);
+        A.xcd_remap = g_xcd_remap && (grid.x * grid.y) % 8 == 0;
+    }
     for (int s = 0; s < nsteps; ++s) {
         if (s == nsteps - 1 && !skip_fill) {
             int rc = bc_fill_impl(G, sizeof(T) == 8 ? PHX_F64 : PHX_F32, c, st);

@@ -48,6 +48,7 @@ struct EnsArgs {
     int ntiles;
     float a, dt6, dt3;
     int stage;
+    int xcd_walk;   // 1: XCD-aware tile walk (grid size a multiple of 8)
 };
 
 // Periodic wrap for |p - n| < n.
@@ -78,8 +79,13 @@ __device__ __forceinline__ void ens_epilogue(const EnsArgs& A, long long gi, int
     }
 }
 
-// One persistent workgroup walks tiles w, w + G, w + 2G, ... (G = grid size)
-// and is software pipelined across them: the next tile's halo is in flight
+// One persistent workgroup walks tiles w, w + G, w + 2G, ... (G = grid size).
+// With the XCD-aware walk, the tile list is split into 8 contiguous ranges,
+// one per XCD (consecutive workgroups are dispatched to the 8 XCDs in turn),
+// and the G/8 workgroups of an XCD walk its range with stride G/8: tiles
+// that share halo rows are then processed on the same XCD at about the same
+// time, and the shared rows come from its own L2. Each workgroup is software
+// pipelined across its tiles: the next tile's halo is in flight
 // (registers) while the current tile's mu is computed, its epilogue operands
 // while the current tile's outputs are written, and tile descriptors two tiles
 // ahead. Every load is unconditional (wrapped, clamped offsets; clamped tile
@@ -94,8 +100,17 @@ __global__ __launch_bounds__(256) void k_ens_stage(EnsArgs A) {
     __shared__ float U[UY * UX];
     __shared__ float MU[MY * MX];
     const int tid = threadIdx.x;
-    const int G = gridDim.x, last = A.ntiles - 1;
-    int w = blockIdx.x;
+    int w, G, last;   // first tile, stride, last tile of this workgroup's walk
+    if (A.xcd_walk) {
+        const int x = blockIdx.x & 7;
+        w = (int)((long long)x * A.ntiles / 8) + (blockIdx.x >> 3);
+        G = gridDim.x >> 3;
+        last = (int)((long long)(x + 1) * A.ntiles / 8) - 1;
+    } else {
+        w = blockIdx.x;
+        G = gridDim.x;
+        last = A.ntiles - 1;
+    }
     if (w > last) return;
     // epilogue sources: an operand the stage does not need is still loaded
     // (every iteration issues the same loads), from the arena's first element
@@ -377,6 +392,7 @@ static void enqueue_steps(phx_ensemble* E, float* c, float* work, double dt, int
     A.runs = E->d_runs;
     A.tiles = E->d_tiles[E->shape];
     A.ntiles = E->ntiles[E->shape];
+    A.xcd_walk = E->grid % 8 == 0;
     const double dt6 = dt / 6.0;
     A.dt6 = (float)dt6;
     A.dt3 = (float)(2.0 * dt6);
