@@ -93,7 +93,7 @@ def kernel_times(run, steps):
     capi.call("phx_ch_timing", 1)
     run(steps)
     kinds = {}
-    for k in range(4):
+    for k in range(7):
         n, t = C.c_int(), C.c_double()
         capi.call("phx_ch_timing_read", k, C.byref(n), C.byref(t))
         if n.value:
@@ -263,8 +263,19 @@ def bench_ac(a, with_cpu):
 
     run(a.warmup)
     ms = timed(lambda: run(a.steps), stream)
+    kinds = kernel_times(run, a.steps)
     pts = NX * NX
-    roof = roofline(16 / 4 * NF * 4 * pts, (ms / 1e3) / (4 * a.steps), "k_ac_stage_2d", "ac8_rk4_f32_1024")
+    # algorithmic field passes per launch and field: one-stage kernel 16/4 on
+    # average over its 4 stages; two-stage (1,2): read c_n, write acc and the
+    # stage-3 state; (3,4): read the stage-3 state, c_n and acc, write c_{n+1}
+    passes = {4: 16 / 4, 5: 3, 6: 4}
+    names = {4: "k_ac_stage_2d", 5: "k_ac_pair_2d (stages 1+2)", 6: "k_ac_pair_2d (stages 3+4)"}
+    launch = {4: "one-stage", 5: "two-stage (1,2)", 6: "two-stage (3,4)"}
+    dom = max(kinds, key=lambda k: kinds[k][1])
+    roof = roofline(passes[dom] * NF * 4 * pts, kinds[dom][1] / 1e3 / kinds[dom][0], names[dom], "ac8_rk4_f32_1024",
+                    launch[dom])
+    roof["launches"] = {names[k]: {"count": v[0], "avg_us": round(v[1] * 1e3 / v[0], 2),
+                                   "alg_bytes": passes[k] * NF * 4 * pts} for k, v in kinds.items()}
     cpu = None
     if with_cpu:
         from oracle import oracle as O

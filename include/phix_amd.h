@@ -201,8 +201,10 @@ int phx_ch_rk4_autotune(const phx_grid* grid, int dtype, void* c, void* work,
 /* Benchmark timing (no reference counterpart): while on, each launch of the
  * fused RK4 kernels is bracketed by HIP events on its stream. phx_ch_timing(1)
  * also clears the record. phx_ch_timing_read sums the durations of the
- * recorded launches of one kind (0: one-stage kernel, 1: two-stage (1,2),
- * 2: two-stage (3,4)), waiting for them. One thread at a time. */
+ * recorded launches of one kind (CH: 0 one-stage kernel, 1 two-stage (1,2),
+ * 2 two-stage (3,4), 3 2D whole step; Allen-Cahn: 4 one-stage kernel,
+ * 5 two-stage (1,2), 6 two-stage (3,4)), waiting for them. One thread at a
+ * time. */
 int phx_ch_timing(int on);
 int phx_ch_timing_read(int kind, int* launches, double* total_ms);
 
@@ -244,6 +246,11 @@ size_t phx_ac_rk4_workspace_bytes(const phx_grid* grid, int dtype, int nfields);
  * 1 <= nfields <= 8, dim 1 or 2. work is phx_ac_rk4_workspace_bytes (3*nfields fields). */
 int phx_ac_rk4_steps(const phx_grid* grid, int dtype, int nfields, void* const* eta,
                      void* work, const phx_ac_params* p, int nsteps, phx_stream stream);
+/* No reference counterpart. 2D grids run two RK4 stages per launch by default
+ * (stages 1+2, then 3+4: 7 field passes per step instead of 16), with results
+ * equal bit for bit to the one-stage kernel's. on = 0 selects the one-stage
+ * kernel (process-wide; 1D grids always use it). Returns PHX_OK. */
+int phx_ac_set_two_stage(int on);
 
 /* ------------------------------------ batched uneven-size ensemble ---- */
 /* No reference counterpart (PhiX runs one problem per process). An ensemble

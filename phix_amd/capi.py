@@ -133,6 +133,7 @@ SIG = {
     "phx_mass_energy": (C.c_int, [GP, C.c_int, VP, C.POINTER(CHParams), VP, VP, VP]),
     "phx_ac_rk4_workspace_bytes": (C.c_size_t, [GP, C.c_int, C.c_int]),
     "phx_ac_rk4_steps": (C.c_int, [GP, C.c_int, C.c_int, C.POINTER(VP), VP, C.POINTER(ACParams), C.c_int, VP]),
+    "phx_ac_set_two_stage": (C.c_int, [C.c_int]),
     "phx_ensemble_create": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double),
                                       C.POINTER(VP)]),
     "phx_ensemble_destroy": (C.c_int, [VP]),

@@ -1850,7 +1850,7 @@ static bool step2d_ok(const phx_grid* G) { return G->dim == 2 && G->n[0] >= 8 &&
 // Optional per-launch timing for benchmarks: while enabled, every launch of
 // the fused RK4 kernels is bracketed by HIP events on its stream, tagged with
 // its kind (0: one-stage kernel, 1: two-stage (1,2), 2: two-stage (3,4),
-// 3: 2D whole step).
+// 3: 2D whole step; 4-6: the Allen-Cahn kernels, see common.hpp).
 // Not thread safe; meant for one benchmarking thread.
 struct LaunchTimer {
     int on = 0;
@@ -1860,7 +1860,7 @@ struct LaunchTimer {
 };
 static LaunchTimer g_lt;
 
-static void lt_begin(hipStream_t st) {
+void launch_timer_begin(hipStream_t st) {
     if (!g_lt.on) return;
     if (2 * g_lt.used + 2 > g_lt.ev.size()) {
         for (int i = 0; i < 2; ++i) {
@@ -1876,12 +1876,15 @@ static void lt_begin(hipStream_t st) {
     (void)hipEventRecord(g_lt.ev[2 * g_lt.used], st);
 }
 
-static void lt_end(int kind, hipStream_t st) {
+void launch_timer_end(int kind, hipStream_t st) {
     if (!g_lt.on) return;
     (void)hipEventRecord(g_lt.ev[2 * g_lt.used + 1], st);
     g_lt.kind[g_lt.used] = kind;
     ++g_lt.used;
 }
+
+static inline void lt_begin(hipStream_t st) { launch_timer_begin(st); }
+static inline void lt_end(int kind, hipStream_t st) { launch_timer_end(kind, st); }
 
 static int g_xcd_remap = 1;
 
@@ -2362,7 +2365,7 @@ int phx_ch_timing(int on) {
 }
 
 int phx_ch_timing_read(int kind, int* launches, double* total_ms) {
-    if (kind < 0 || kind > 3) return fail(PHX_ERR_INVALID, "phx_ch_timing_read: kind must be in [0, 3]");
+    if (kind < 0 || kind > 6) return fail(PHX_ERR_INVALID, "phx_ch_timing_read: kind must be in [0, 6]");
     int n = 0;
     double ms = 0.0;
     for (size_t i = 0; i < g_lt.used; ++i) {

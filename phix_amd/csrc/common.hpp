@@ -112,4 +112,11 @@ __device__ __forceinline__ T fixed_value(int p, const AxisBC<T>& b) {
 // ------------------------------------------------------------ misc helpers --
 __host__ __device__ inline int cdiv(long long a, int b) { return (int)((a + b - 1) / b); }
 
+// Benchmark launch timer (phx_ch_timing): while enabled, a launch bracketed by
+// these two calls is timed with HIP events on its stream and tagged with a
+// kind. Kinds: 0-3 the CH RK4 kernels, 4 the Allen-Cahn one-stage kernel,
+// 5 / 6 its two-stage kernels (stages 1+2 / 3+4).
+void launch_timer_begin(hipStream_t st);
+void launch_timer_end(int kind, hipStream_t st);
+
 }  // namespace phx
