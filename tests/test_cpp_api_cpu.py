@@ -43,3 +43,23 @@ def test_driver_usage_on_bad_settings(tmp_path):
             assert "Usage: " in r.stderr and "[path/to/settings.jsonc]" in r.stderr, r.stderr
         r = subprocess.run([str(BIN / exe)], capture_output=True, text=True, timeout=120, cwd=tmp_path)
         assert r.returncode == 1 and "settings/settings.jsonc" in r.stderr, r.stderr
+
+
+def test_composite_forms_host_path(tmp_path):
+    """Composite DSL forms (grad/lap of expressions, div of a composite vector
+    expression, Term*Term, VectorField divergence) on non-uniform fields, host
+    path, against the numpy restatement; periodic and zero-flux BCs."""
+    import sys
+    sys.path.insert(0, str(ROOT / "tests"))
+    from helpers import composite_reference, read_field
+    for mode in ("periodic", "noflux"):
+        r = subprocess.run([str(BIN / "composite"), "cpu", str(tmp_path), mode], capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stderr
+        ref = composite_reference(mode)
+        for name, want in ref.items():
+            _, got = read_field(tmp_path / f"{name}_{mode}_cpu.field")
+            err = np.abs(got[0, 0] - want).max()
+            scale = max(1.0, np.abs(want).max())
+            print(f"{mode:8s} {name:10s} max|host - numpy| = {err:.2e} (scale {scale:.1f})")
+            assert err <= 1e-12 * scale, (mode, name, err)

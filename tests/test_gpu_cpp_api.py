@@ -104,3 +104,19 @@ def test_ac_vector_solver(dev, tmp_path, nf, bcmode):
     _, got = read_field(tmp_path / "ac_fp32.vfield")
     err = max(np.abs(got[f] - capi.physical(etas[f], g)).max() for f in range(nf))
     assert err <= 1e-5
+
+
+@pytest.mark.parametrize("mode", ["periodic", "noflux"])
+def test_composite_forms_device_path(dev, tmp_path, mode):
+    """Composite DSL forms on the device path (materialise, BC launch, stencil
+    and product kernels) against the host path and the numpy restatement."""
+    from helpers import composite_reference
+    run([BIN / "composite", "cpu", tmp_path, mode])
+    run([BIN / "composite", "gpu", tmp_path, mode])
+    for name, want in composite_reference(mode).items():
+        _, h = read_field(tmp_path / f"{name}_{mode}_cpu.field")
+        _, d = read_field(tmp_path / f"{name}_{mode}_gpu.field")
+        scale = max(1.0, np.abs(want).max())
+        e_host, e_ref = np.abs(d - h).max(), np.abs(d[0, 0] - want).max()
+        print(f"{mode:8s} {name:10s} device vs host {e_host:.2e}, vs numpy {e_ref:.2e}")
+        assert e_host <= 1e-12 * scale and e_ref <= 1e-12 * scale, (name, e_host, e_ref)
