@@ -93,7 +93,7 @@ def kernel_times(run, steps):
     capi.call("phx_ch_timing", 1)
     run(steps)
     kinds = {}
-    for k in range(7):
+    for k in range(10):
         n, t = C.c_int(), C.c_double()
         capi.call("phx_ch_timing_read", k, C.byref(n), C.byref(t))
         if n.value:
@@ -322,7 +322,20 @@ def bench_ens(a, with_cpu):
     capi.call("phx_ensemble_graph_steps", h, capi.ptr(c), capi.ptr(work), 0.01, per_graph, 1, sh)
     ms = timed(lambda: capi.call("phx_ensemble_graph_steps", h, capi.ptr(c), capi.ptr(work), 0.01, per_graph,
                                  steps // per_graph, sh), stream)
-    roof = roofline(16 / 4 * 4 * cells, (ms / 1e3) / (4 * steps), "k_ens_stage", "ens512_rk4_f32")
+    two = capi.lib().phx_ensemble_get_two_stage(h)
+    # per-launch durations: the same steps again, eager, each launch bracketed
+    # by HIP events (outside the timed region); field passes per launch as
+    # for the Allen-Cahn kernels
+    kinds = kernel_times(lambda k: capi.call("phx_ensemble_rk4_steps", h, capi.ptr(c), capi.ptr(work), 0.01, k, sh),
+                         steps)
+    passes = {7: 16 / 4, 8: 3, 9: 4}
+    names = {7: "k_ens_stage", 8: "k_ens_pair (stages 1+2)", 9: "k_ens_pair (stages 3+4)"}
+    launch = {7: "one-stage", 8: "two-stage (1,2)", 9: "two-stage (3,4)"}
+    dom = max(kinds, key=lambda k: kinds[k][1])
+    roof = roofline(passes[dom] * 4 * cells, kinds[dom][1] / 1e3 / kinds[dom][0], names[dom], "ens512_rk4_f32",
+                    launch[dom])
+    roof["launches"] = {names[k]: {"count": v[0], "avg_us": round(v[1] * 1e3 / v[0], 2),
+                                   "alg_bytes": passes[k] * 4 * cells} for k, v in kinds.items()}
     cpu = None
     if with_cpu:
         from oracle import oracle as O
@@ -341,7 +354,8 @@ def bench_ens(a, with_cpu):
                "sample": f"{csteps} RK4 steps of the first {runs} runs, FP64 oracle, {dt:.1f} s on {cpu_model()}"}
     capi.lib().phx_ensemble_destroy(h)
     cfg = {"workload": "ens512_ch_rk4_periodic", "runs": n, "cells": int(cells), "sides": "64+32*floor(15u)",
-           "dt": 0.01, "scheme": "rk4", "bulk": "f32", "tile": [tx.value, ty.value], "persistent_grid": gr.value,
+           "dt": 0.01, "scheme": "rk4", "bulk": "f32", "tile": [tx.value, ty.value],
+           "kernel": "two-stage, one workgroup per tile" if two == 1 else f"one-stage, persistent grid {gr.value}",
            "hipgraph_steps": per_graph, "parallelism": "single-gpu"}
     return line("cell-updates/s, batched uneven-size CH RK4 ensemble", cells * steps / (ms / 1e3),
                 "cell-updates/s", steps, per_graph, ms, "f32",

@@ -203,8 +203,10 @@ int phx_ch_rk4_autotune(const phx_grid* grid, int dtype, void* c, void* work,
  * also clears the record. phx_ch_timing_read sums the durations of the
  * recorded launches of one kind (CH: 0 one-stage kernel, 1 two-stage (1,2),
  * 2 two-stage (3,4), 3 2D whole step; Allen-Cahn: 4 one-stage kernel,
- * 5 two-stage (1,2), 6 two-stage (3,4)), waiting for them. One thread at a
- * time. */
+ * 5 two-stage (1,2), 6 two-stage (3,4); ensemble: 7 one-stage kernel, 8
+ * two-stage (1,2), 9 two-stage (3,4)), waiting for them. Launches captured
+ * into a graph (phx_ensemble_graph_steps) must not run while it is on. One
+ * thread at a time. */
 int phx_ch_timing(int on);
 int phx_ch_timing_read(int kind, int* launches, double* total_ms);
 
@@ -267,7 +269,8 @@ int       phx_ensemble_create(int nruns, const int* sides, const double* kappa,
 int       phx_ensemble_destroy(phx_ensemble* ens);
 long long phx_ensemble_cells(const phx_ensemble* ens);
 long long phx_ensemble_offset(const phx_ensemble* ens, int run);
-/* nsteps RK4 steps (4 launches each) on `stream`. */
+/* nsteps RK4 steps (2 or 4 launches each, see phx_ensemble_set_two_stage) on
+ * `stream`. */
 int phx_ensemble_rk4_steps(phx_ensemble* ens, float* c, float* work, double dt, int nsteps,
                            phx_stream stream);
 /* Captures steps_per_graph RK4 steps into a hipGraph (once per (c, work,
@@ -281,9 +284,16 @@ int phx_ensemble_mass(const phx_ensemble* ens, const float* c, double* out_dev, 
  * same bits. */
 int phx_ensemble_set_tiling(phx_ensemble* ens, int shape, int grid);
 int phx_ensemble_get_tiling(const phx_ensemble* ens, int* shape, int* grid, int* tx, int* ty);
-/* Times every tile shape x {2,4,8,16} workgroups per CU on `steps` steps of
- * the given state (advancing it: pass a scratch copy), keeps the fastest and
- * returns its ms per step. Blocks the host. */
+/* on = 1 (the default): two RK4 stages per launch (stages 1+2, then 3+4; 7
+ * field passes per step instead of 16), one workgroup per tile; on = 0: the
+ * persistent one-stage kernel. Both return the same bits. get returns 0/1
+ * (-1 for NULL). */
+int phx_ensemble_set_two_stage(phx_ensemble* ens, int on);
+int phx_ensemble_get_two_stage(const phx_ensemble* ens);
+/* Times every tile shape with the two-stage kernel and with the one-stage
+ * kernel at {2,4,8,16} workgroups per CU, on `steps` steps of the given state
+ * (advancing it: pass a scratch copy), keeps the fastest and returns its ms
+ * per step. Blocks the host. */
 int phx_ensemble_autotune(phx_ensemble* ens, float* c, float* work, double dt, int steps,
                           double* best_ms_per_step, phx_stream stream);
 

@@ -145,6 +145,8 @@ SIG = {
     "phx_ensemble_set_tiling": (C.c_int, [VP, C.c_int, C.c_int]),
     "phx_ensemble_get_tiling": (C.c_int, [VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                           C.POINTER(C.c_int)]),
+    "phx_ensemble_set_two_stage": (C.c_int, [VP, C.c_int]),
+    "phx_ensemble_get_two_stage": (C.c_int, [VP]),
     "phx_ensemble_autotune": (C.c_int, [VP, VP, VP, C.c_double, C.c_int, C.POINTER(C.c_double), VP]),
     "phx_sur_param_count": (C.c_int, []),
     "phx_sur_workspace_bytes": (C.c_size_t, [C.c_int, C.c_int, C.c_int]),

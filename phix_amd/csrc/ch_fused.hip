@@ -1850,7 +1850,8 @@ static bool step2d_ok(const phx_grid* G) { return G->dim == 2 && G->n[0] >= 8 &&
 // Optional per-launch timing for benchmarks: while enabled, every launch of
 // the fused RK4 kernels is bracketed by HIP events on its stream, tagged with
 // its kind (0: one-stage kernel, 1: two-stage (1,2), 2: two-stage (3,4),
-// 3: 2D whole step; 4-6: the Allen-Cahn kernels, see common.hpp).
+// 3: 2D whole step; 4-9: the Allen-Cahn and ensemble kernels, see
+// common.hpp).
 // Not thread safe; meant for one benchmarking thread.
 struct LaunchTimer {
     int on = 0;
@@ -2365,7 +2366,7 @@ int phx_ch_timing(int on) {
 }
 
 int phx_ch_timing_read(int kind, int* launches, double* total_ms) {
-    if (kind < 0 || kind > 6) return fail(PHX_ERR_INVALID, "phx_ch_timing_read: kind must be in [0, 6]");
+    if (kind < 0 || kind > 9) return fail(PHX_ERR_INVALID, "phx_ch_timing_read: kind must be in [0, 9]");
     int n = 0;
     double ms = 0.0;
     for (size_t i = 0; i < g_lt.used; ++i) {

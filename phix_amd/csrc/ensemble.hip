@@ -47,6 +47,7 @@ struct EnsArgs {
     const EnsTile* tiles;
     int ntiles;
     float a, dt6, dt3;
+    float a2;       // two-stage kernel: shift of its second stage
     int stage;
     int xcd_walk;   // 1: XCD-aware tile walk (grid size a multiple of 8)
 };
@@ -219,6 +220,130 @@ __global__ __launch_bounds__(256) void k_ens_stage(EnsArgs A) {
     }
 }
 
+// Two RK4 stages per launch (temporal blocking): stages (1,2) (SECOND =
+// false) or (3,4) (SECOND = true), one workgroup per tile. Per step
+//   (1,2): read c_n;                          write acc and the stage-3 state;
+//   (3,4): read the stage-3 state, c_n, acc;  write c_{n+1} into a buffer other
+//          than c_n (the launch reads c_n around its tile),
+// 7 field passes instead of 16. The first stage's input is staged with a
+// 4-cell periodic halo; mu and the first stage's result are formed on the
+// rings the second stage needs (mu_A on tile+3, u_B on tile+2, mu_B on
+// tile+1). Ring cells outside a run are evaluated from the wrapped input,
+// which repeats the operands of the cell they wrap to, in the same order, so
+// the results equal two one-stage launches bit for bit. The workgroup ->
+// tile order is XCD-aware (a contiguous range of the list per XCD).
+// p in [-4, n + 3] wrapped into [0, n) for any n >= 2.
+__device__ __forceinline__ int wrap4(int p, int n) {
+    p = p < 0 ? p + n : p;
+    p = p < 0 ? p + n : p;
+    p = p >= n ? p - n : p;
+    return p >= n ? p - n : p;
+}
+
+template <int TX, int TY, bool SECOND>
+__global__ __launch_bounds__(256) void k_ens_pair(EnsArgs A) {
+    constexpr int UX = TX + 8, UY = TY + 8;   // first-stage input, radius 4
+    constexpr int AX = TX + 6, AY = TY + 6;   // first-stage mu
+    constexpr int BX = TX + 4, BY = TY + 4;   // second-stage input (and c_n, SECOND)
+    constexpr int MX = TX + 2, MY = TY + 2;   // second-stage mu
+    constexpr int NS = (UX * UY + 255) / 256;
+    constexpr int NC = SECOND ? (BX * BY + 255) / 256 : 1;
+    constexpr int NO = (TX * TY) / 256;
+    static_assert((TX * TY) % 256 == 0, "tile must be a multiple of 256 cells");
+    __shared__ float U[UY * UX];
+    __shared__ float MA[AY * AX];
+    __shared__ float UB[BY * BX];
+    __shared__ float MB[MY * MX];
+    __shared__ float CB[SECOND ? BY * BX : 1];
+    __shared__ float KA[TX * TY];   // first-stage RHS at the tile cells
+    const int tid = threadIdx.x;
+    const int nb = gridDim.x;
+    const int h = blockIdx.x;
+    const int w = (nb & 7) == 0 ? (h & 7) * (nb >> 3) + (h >> 3) : h;
+    if (w >= A.ntiles) return;
+    const EnsTile T = A.tiles[w];
+    const EnsRun R = A.runs[T.run];
+    const int n = R.n;
+    const long long off = R.off;
+    const float nk = -R.kappa, M = R.M;
+
+    float hv[NS], cv[NC], pa[NO];
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        const int t = min(tid + 256 * i, UX * UY - 1);
+        const int ly = t / UX, lx = t - ly * UX;
+        const int px = wrap4(min(T.x0 - 4 + lx, n + 3), n), py = wrap4(min(T.y0 - 4 + ly, n + 3), n);
+        hv[i] = A.u[off + (long long)py * n + px];
+    }
+    if constexpr (SECOND) {
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            const int t = min(tid + 256 * i, BX * BY - 1);
+            const int ly = t / BX, lx = t - ly * BX;
+            const int px = wrap4(min(T.x0 - 2 + lx, n + 3), n), py = wrap4(min(T.y0 - 2 + ly, n + 3), n);
+            cv[i] = A.c[off + (long long)py * n + px];
+        }
+#pragma unroll
+        for (int j = 0; j < NO; ++j) {
+            const int t = tid + 256 * j;
+            const int oy = t / TX, ox = t - oy * TX;
+            pa[j] = A.acc[off + (long long)min(T.y0 + oy, n - 1) * n + min(T.x0 + ox, n - 1)];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NS; ++i)
+        if (tid + 256 * i < UX * UY) U[tid + 256 * i] = hv[i];
+    if constexpr (SECOND) {
+#pragma unroll
+        for (int i = 0; i < NC; ++i)
+            if (tid + 256 * i < BX * BY) CB[tid + 256 * i] = cv[i];
+    }
+    __syncthreads();
+    for (int t = tid; t < AX * AY; t += 256) {
+        const int ay = t / AX, ax = t - ay * AX;
+        const int c0 = (ay + 1) * UX + ax + 1;
+        MA[t] = ens_mu(U[c0], U[c0 + 1], U[c0 - 1], U[c0 + UX], U[c0 - UX], nk);
+    }
+    __syncthreads();
+    // first stage: u_B = base + a k on tile+2; k kept at the tile cells
+    for (int t = tid; t < BX * BY; t += 256) {
+        const int by = t / BX, bx = t - by * BX;
+        const int m0 = (by + 1) * AX + bx + 1;
+        const float k = ens_k(MA[m0], MA[m0 + 1], MA[m0 - 1], MA[m0 + AX], MA[m0 - AX], M);
+        const float base = SECOND ? CB[t] : U[(by + 2) * UX + bx + 2];
+        UB[t] = __builtin_fmaf(A.a, k, base);
+        const int ox = bx - 2, oy = by - 2;
+        if (ox >= 0 && ox < TX && oy >= 0 && oy < TY) KA[oy * TX + ox] = k;
+    }
+    __syncthreads();
+    for (int t = tid; t < MX * MY; t += 256) {
+        const int my = t / MX, mx = t - my * MX;
+        const int c0 = (my + 1) * BX + mx + 1;
+        MB[t] = ens_mu(UB[c0], UB[c0 + 1], UB[c0 - 1], UB[c0 + BX], UB[c0 - BX], nk);
+    }
+    __syncthreads();
+    // second stage and the RK4 epilogue at the tile cells
+#pragma unroll
+    for (int j = 0; j < NO; ++j) {
+        const int t = tid + 256 * j;
+        const int oy = t / TX, ox = t - oy * TX;
+        const int x = T.x0 + ox, y = T.y0 + oy;
+        if (x >= n || y >= n) continue;
+        const int m0 = (oy + 1) * MX + ox + 1;
+        const float k = ens_k(MB[m0], MB[m0 + 1], MB[m0 - 1], MB[m0 + MX], MB[m0 - MX], M);
+        const long long gi = off + (long long)y * n + x;
+        if constexpr (!SECOND) {
+            const float cn = U[(oy + 4) * UX + ox + 4];
+            const float am = __builtin_fmaf(A.dt6, KA[t], cn);   // accumulator after stage 1
+            A.acc[gi] = __builtin_fmaf(A.dt3, k, am);
+            A.out[gi] = __builtin_fmaf(A.a2, k, cn);
+        } else {
+            const float am = __builtin_fmaf(A.dt3, KA[t], pa[j]);   // after stage 3
+            A.out[gi] = __builtin_fmaf(A.dt6, k, am);
+        }
+    }
+}
+
 // FP64 compensated per-run sum: one workgroup per run.
 __global__ __launch_bounds__(256) void k_ens_mass(const float* __restrict__ c, const EnsRun* __restrict__ runs,
                                                   double* __restrict__ out) {
@@ -260,6 +385,23 @@ static void launch_stage(int grid, const EnsArgs& A, hipStream_t st) {
     hipLaunchKernelGGL((k_ens_stage<TX, TY>), dim3(grid), dim3(256), 0, st, A);
 }
 
+template <int TX, int TY>
+static void launch_pair_t(bool second, const EnsArgs& A, hipStream_t st) {
+    const unsigned grid = (unsigned)((A.ntiles + 7) / 8 * 8);
+    if (second) hipLaunchKernelGGL((k_ens_pair<TX, TY, true>), dim3(grid), dim3(256), 0, st, A);
+    else hipLaunchKernelGGL((k_ens_pair<TX, TY, false>), dim3(grid), dim3(256), 0, st, A);
+}
+
+static void launch_pair(int shape, bool second, const EnsArgs& A, hipStream_t st) {
+    switch (shape) {
+        case 0: launch_pair_t<64, 16>(second, A, st); break;
+        case 1: launch_pair_t<32, 16>(second, A, st); break;
+        case 2: launch_pair_t<64, 8>(second, A, st); break;
+        case 3: launch_pair_t<128, 8>(second, A, st); break;
+        default: launch_pair_t<32, 32>(second, A, st); break;
+    }
+}
+
 static void launch_shape(int shape, int grid, const EnsArgs& A, hipStream_t st) {
     switch (shape) {
         case 0: launch_stage<64, 16>(grid, A, st); break;
@@ -285,6 +427,7 @@ struct phx_ensemble {
     int ntiles[kNumShapes] = {};
     int shape = 0;
     int grid = 0;
+    int two_stage = 1;   // 1: two RK4 stages per launch (k_ens_pair)
     int cus = 256;
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
@@ -372,6 +515,18 @@ int phx_ensemble_set_tiling(phx_ensemble* E, int shape, int grid) {
     return PHX_OK;
 }
 
+int phx_ensemble_set_two_stage(phx_ensemble* E, int on) {
+    if (!E) return fail(PHX_ERR_INVALID, "phx_ensemble_set_two_stage: NULL");
+    E->two_stage = on ? 1 : 0;
+    if (E->graph) {
+        (void)hipGraphExecDestroy(E->graph);
+        E->graph = nullptr;
+    }
+    return PHX_OK;
+}
+
+int phx_ensemble_get_two_stage(const phx_ensemble* E) { return E ? E->two_stage : -1; }
+
 int phx_ensemble_get_tiling(const phx_ensemble* E, int* shape, int* grid, int* tx, int* ty) {
     if (!E) return fail(PHX_ERR_INVALID, "phx_ensemble_get_tiling: NULL");
     if (shape) *shape = E->shape;
@@ -397,6 +552,30 @@ static void enqueue_steps(phx_ensemble* E, float* c, float* work, double dt, int
     A.dt6 = (float)dt6;
     A.dt3 = (float)(2.0 * dt6);
     const float shift[4] = {(float)(dt * 0.5), (float)(dt * 0.5), (float)dt, 0.f};
+    if (E->two_stage) {
+        // c_n alternates between c and w0; after an odd step count the state
+        // is copied back into c
+        for (int s = 0; s < nsteps; ++s) {
+            float* cur = s % 2 == 0 ? c : w0;
+            float* nxt = s % 2 == 0 ? w0 : c;
+            A.u = cur;
+            A.c = cur;
+            A.out = w1;
+            A.a = shift[0];
+            A.a2 = shift[1];
+            launch_timer_begin(st);
+            launch_pair(E->shape, false, A, st);
+            launch_timer_end(8, st);
+            A.u = w1;
+            A.out = nxt;
+            A.a = shift[2];
+            launch_timer_begin(st);
+            launch_pair(E->shape, true, A, st);
+            launch_timer_end(9, st);
+        }
+        if (nsteps % 2) (void)hipMemcpyAsync(c, w0, (size_t)N * sizeof(float), hipMemcpyDeviceToDevice, st);
+        return;
+    }
     const float* uin[4] = {c, w0, w1, w0};
     float* uout[4] = {w0, w1, w0, c};
     for (int s = 0; s < nsteps; ++s)
@@ -405,7 +584,9 @@ static void enqueue_steps(phx_ensemble* E, float* c, float* work, double dt, int
             A.a = shift[k];
             A.u = uin[k];
             A.out = uout[k];
+            launch_timer_begin(st);
             launch_shape(E->shape, E->grid, A, st);
+            launch_timer_end(7, st);
         }
 }
 
@@ -462,11 +643,14 @@ int phx_ensemble_autotune(phx_ensemble* E, float* c, float* work, double dt, int
     PHX_HIP(hipEventCreate(&e0));
     PHX_HIP(hipEventCreate(&e1));
     float best = 1e30f;
-    int bs = 0, bg = 0;
-    const int mult[] = {2, 4, 8, 16};
+    int bs = 0, bg = 0, b2 = 0;
+    // one-stage candidates: (shape, persistent grid); two-stage: shape (one
+    // workgroup per tile, m = 0)
+    const int mult[] = {0, 2, 4, 8, 16};
     for (int sh = 0; sh < kNumShapes; ++sh)
         for (int m : mult) {
-            int grid = std::min(E->ntiles[sh], m * E->cus);
+            int grid = m ? std::min(E->ntiles[sh], m * E->cus) : std::min(E->ntiles[sh], 8 * E->cus);
+            E->two_stage = m == 0;
             E->shape = sh;
             E->grid = grid;
             enqueue_steps(E, c, work, dt, 1, st);   // warm
@@ -480,6 +664,7 @@ int phx_ensemble_autotune(phx_ensemble* E, float* c, float* work, double dt, int
                 best = ms;
                 bs = sh;
                 bg = grid;
+                b2 = E->two_stage;
             }
         }
     (void)hipEventDestroy(e0);
@@ -490,6 +675,7 @@ int phx_ensemble_autotune(phx_ensemble* E, float* c, float* work, double dt, int
     }
     E->shape = bs;
     E->grid = bg;
+    E->two_stage = b2;
     if (best_ms) *best_ms = best / steps;
     PHX_LAUNCH_CHECK("phx_ensemble_autotune");
     return PHX_OK;

@@ -110,3 +110,39 @@ def test_config4_full_ensemble_mass(dev):
     means = (m1.cpu().numpy() / (N.astype(np.float64) ** 2))
     assert np.abs(means - cbar).max() < 0.02
     E.close()
+
+
+@pytest.mark.parametrize("steps", [1, 2, 7])
+def test_two_stage_equals_one_stage(dev, steps):
+    """The two-stage kernel (stages 1+2 and 3+4 per launch, one workgroup per
+    tile) returns the persistent one-stage kernel's bits for every tile shape,
+    odd and even step counts (the state alternates between c and the
+    workspace), tiny runs (sides 2, 3, 5: halos wrap more than once) and the
+    captured graph."""
+    import torch
+    sides, kappa = [2, 3, 5, 64, 96, 33, 130], [0.3, 0.9, 0.5, 0.6, 0.25, 1.0, 0.7]
+    E = Ens(sides, kappa)
+    x0 = torch.from_numpy(make_state(sides, [0.5, 0.4, 0.6, 0.5, 0.45, 0.55, 0.35], 6)).float().to(dev)
+    work = torch.empty(3 * E.cells, dtype=torch.float32, device=dev)
+    capi.call("phx_ensemble_set_two_stage", E.h, 0)
+    capi.call("phx_ensemble_set_tiling", E.h, 0, 0)
+    ref = x0.clone()
+    capi.call("phx_ensemble_rk4_steps", E.h, capi.ptr(ref), capi.ptr(work), 0.01, steps, stream())
+    capi.call("phx_ensemble_set_two_stage", E.h, 1)
+    assert capi.lib().phx_ensemble_get_two_stage(E.h) == 1
+    for shape in range(5):
+        capi.call("phx_ensemble_set_tiling", E.h, shape, 0)
+        t = x0.clone()
+        capi.call("phx_ensemble_rk4_steps", E.h, capi.ptr(t), capi.ptr(work), 0.01, steps, stream())
+        torch.cuda.synchronize()
+        print(f"steps {steps} shape {shape}: max |two - one| {(t - ref).abs().max().item():.3e}", flush=True)
+        assert torch.equal(t, ref), shape
+    g = x0.clone()
+    capi.call("phx_ensemble_graph_steps", E.h, capi.ptr(g), capi.ptr(work), 0.01, steps, 2, stream())
+    r2 = ref.clone()
+    capi.call("phx_ensemble_set_two_stage", E.h, 0)
+    capi.call("phx_ensemble_rk4_steps", E.h, capi.ptr(r2), capi.ptr(work), 0.01, steps, stream())
+    torch.cuda.synchronize()
+    assert torch.equal(g, r2)
+    assert not torch.equal(ref, x0)
+    E.close()
