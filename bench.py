@@ -152,11 +152,13 @@ def bench_ch3d(a, with_cpu):
     ms = timed(lambda: run(a.steps), stream)
     kinds = kernel_times(run, a.steps)
     cells = NX ** 3
-    # algorithmic field passes per launch: one-stage kernel 16/4 (FP32) or 17/4
-    # (FP64: c_n re-read in stage 4) on average over its 4 stages; two-stage
-    # (1,2): read c_n, write acc and the stage-3 state; (3,4): read the stage-3
-    # state, c_n and acc, write c_{n+1}
-    passes = {0: (16 if dtype == capi.F32 else 17) / 4, 1: 3, 2: 4}
+    # algorithmic field passes per launch: one-stage kernel 13/4 (FP32: stage
+    # states u2, u3, u4 and no accumulator) or 17/4 (FP64: accumulator, c_n
+    # re-read in stage 4) on average over its 4 stages; two-stage (1,2): read
+    # c_n, write the stage-3 state and the accumulator (FP64) or u2 (FP32);
+    # (3,4): read the stage-3 state, c_n and the accumulator or u2, write
+    # c_{n+1} (FP32 also reads the stage-3 state at the cells, counted once)
+    passes = {0: (13 if dtype == capi.F32 else 17) / 4, 1: 3, 2: 4}
     names = {0: "k_ch_stage_3dz" if tile.value >= 9 else "k_ch_stage_3d", 1: "k_ch_pair_3d (stages 1+2)",
              2: "k_ch_pair_3d (stages 3+4)"}
     tags = {0: "ch3d_rk4", 1: "ch3d_pair12", 2: "ch3d_pair34"}

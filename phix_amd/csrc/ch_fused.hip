@@ -68,6 +68,10 @@ struct CHArgs {
     T two_rho, ca, cb, nkappa, M;
     T a;            // stage shift dt/2, dt/2, dt
     T dt6, dt3;
+    // 3D FP32: stage 4 reads the stage states u2, u3 instead of an accumulator
+    const T* s2;
+    const T* s3;
+    T third;        // 1/3
     FastCoef<T> F;  // FP32 arithmetic form
     int stage;      // 1..4
     int zc;         // z planes per block (3D)
@@ -136,6 +140,18 @@ template <typename T, bool EXACT>
 __device__ __forceinline__ T axpy(T a, T k, T c) {
     if constexpr (EXACT) return c + a * k;
     else return fma_t(a, k, c);
+}
+
+// FP32 RK4 update of the 3D kernels from the stage states u2 = c + dt/2 k1,
+// u3 = c + dt/2 k2, u4 = c + dt k3 and k4:
+//   c_{n+1} = c + dt/6 k4 + ((u2 - c) + 2 (u3 - c) + (u4 - c)) / 3,
+// the classical RK4 sum with the increments recovered from the states. No
+// accumulator field: a step reads and writes 13 fields instead of 16
+// (stage 1: c -> u2; 2: u2, c -> u3; 3: u3, c -> u4; 4: u4, c, u2, u3 -> c).
+template <typename T>
+__device__ __forceinline__ T rk4_fin(T dt6, T third, T c, T u2, T u3, T u4, T k) {
+    const T d = (u2 - c) + fma_t((T)2, u3 - c, u4 - c);
+    return c + fma_t(dt6, k, third * d);
 }
 
 // The six FIXED constants (x lo, x hi, y lo, y hi, z lo, z hi) staged in LDS
@@ -510,18 +526,23 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
     // iteration issues the same loads (the compiler then waits for exactly the
     // ones it uses); an operand the stage does not need is read from the
     // first element of the stage state for every lane (one cache line).
-    const bool need_c = st == 2 || st == 3 || (EXACT && st == 4);
-    const bool need_a = st >= 2;
+    // FP64: c_n (stages 2-4) and the accumulator (2-4); FP32: c_n (2-4) and
+    // the stage states u2, u3 (4)
+    const bool need_c = EXACT ? (st == 2 || st == 3 || st == 4) : st >= 2;
+    const bool need_a = EXACT ? st >= 2 : st == 4;
+    const bool need_b = !EXACT && st == 4;
     const T* csrc = need_c ? A.c : A.u;
-    const T* asrc = need_a ? A.acc : A.u;
-    const long long cmask = need_c ? -1LL : 0LL, amask = need_a ? -1LL : 0LL;
-    T pc[NO], pa[NO], nc[NO], na[NO];
-    auto load_epi = [&](int o, T (&rc)[NO], T (&ra)[NO]) {
+    const T* asrc = need_a ? (EXACT ? A.acc : A.s2) : A.u;
+    const T* bsrc = need_b ? A.s3 : A.u;
+    const long long cmask = need_c ? -1LL : 0LL, amask = need_a ? -1LL : 0LL, bmask = need_b ? -1LL : 0LL;
+    T pc[NO], pa[NO], pq[NO], nc[NO], na[NO], nq[NO];
+    auto load_epi = [&](int o, T (&rc)[NO], T (&ra)[NO], T (&rb)[NO]) {
         const long long pb = (long long)(o + g) * sxy;
 #pragma unroll
         for (int j = 0; j < NO; ++j) {
             rc[j] = csrc[(pb + ooff[j]) & cmask];
             ra[j] = asrc[(pb + ooff[j]) & amask];
+            if constexpr (!EXACT) rb[j] = bsrc[(pb + ooff[j]) & bmask];
         }
     };
 
@@ -537,7 +558,7 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
     store_plane(z0, p2);
     load_plane(z0 + 1, p2);
     load_plane(z0 + 2, pa0);
-    load_epi(z0, pc, pa);
+    load_epi(z0, pc, pa, pq);
     load_plane(min(z0 + 3, z1 + 1), pa1);
     store_plane(z0 + 1, p2);
     __syncthreads();
@@ -554,7 +575,7 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
         __syncthreads();   // A
         // unconditional prefetch (clamped planes): the same loads every iteration
         load_plane(min(o + 4, z1 + 1), P);
-        load_epi(min(o + 1, z1 - 1), nc, na);
+        load_epi(min(o + 1, z1 - 1), nc, na, nq);
         compute_mu(o + 1);
         __syncthreads();   // B
         const T* mlo = MU[(o + 2) % 3];
@@ -584,21 +605,16 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
                     A.out[gi] = pc[j] + A.dt6 * (pa[j] + k);
                 }
             } else {
-                if (st == 1) {
-                    A.acc[gi] = axpy<T, false>(A.dt6, k, uc);
-                    A.out[gi] = axpy<T, false>(A.a, k, uc);
-                } else if (st < 4) {
-                    A.acc[gi] = axpy<T, false>(A.dt3, k, pa[j]);
-                    A.out[gi] = axpy<T, false>(A.a, k, pc[j]);
-                } else {
-                    A.out[gi] = axpy<T, false>(A.dt6, k, pa[j]);
-                }
+                if (st == 1) A.out[gi] = axpy<T, false>(A.a, k, uc);
+                else if (st < 4) A.out[gi] = axpy<T, false>(A.a, k, pc[j]);
+                else A.out[gi] = rk4_fin<T>(A.dt6, A.third, pc[j], pa[j], pq[j], uc, k);
             }
         }
 #pragma unroll
         for (int j = 0; j < NO; ++j) {
             pc[j] = nc[j];
             pa[j] = na[j];
+            pq[j] = nq[j];
         }
     };
     int o = z0;
@@ -640,17 +656,9 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
 // The state plane written in iteration o was last read by the mu pass of
 // iteration o-1 (before its B2); mu(o+1) is written after B2 of iteration o,
 // which follows the last reads of mu(o), and is read after B1 of o+1.
-// Minimum waves per SIMD requested of the compiler: two resident 1024-thread
-// workgroups (8 waves per SIMD, <= 64 VGPRs) for the FP32 variants with two
-// output cells per thread, which need only a few registers more than that.
-template <typename T, int TX, int TY, int NT>
-constexpr int zreg_waves() {
-    return (sizeof(T) == 4 && NT == 1024 && TX * TY / NT <= 2) ? 8 : 1;
-}
 
 template <typename T, int TX, int TY, int NT, bool EXACT>
-__global__ __attribute__((amdgpu_flat_work_group_size(1, NT), amdgpu_waves_per_eu(zreg_waves<T, TX, TY, NT>())))
-void k_ch_stage_3dz(CHArgs<T> A) {
+__global__ __attribute__((amdgpu_flat_work_group_size(1, NT))) void k_ch_stage_3dz(CHArgs<T> A) {
     constexpr int UX = TX + 4, UY = TY + 4, MX = TX + 2, MY = TY + 2;
     constexpr int NI = TX * TY, NRING = 2 * MX + 2 * TY, NOUT = 2 * UX + 2 * MY;
     static_assert(NI % NT == 0, "tile must be a multiple of the workgroup size");
@@ -839,28 +847,33 @@ void k_ch_stage_3dz(CHArgs<T> A) {
         if (mcode[s] & GC_FIX) v = ghost_value_xy(mcode[s], BCV);
         return v;
     };
-    // epilogue operands (both loaded for every stage: the same loads each
+    // epilogue operands (all loaded for every stage: the same loads each
     // iteration; an operand the stage does not need is read from the first
-    // element of the stage state for every lane: one cache line per wave)
+    // element of the stage state for every lane: one cache line per wave).
+    // FP64: c_n (stages 2-4) and the accumulator (2-4); FP32: c_n (2-4) and
+    // the stage states u2, u3 (4).
     const int st = A.stage;
-    const bool need_c = st == 2 || st == 3 || (EXACT && st == 4);
-    const bool need_a = st >= 2;
+    const bool need_c = EXACT ? (st == 2 || st == 3 || st == 4) : st >= 2;
+    const bool need_a = EXACT ? st >= 2 : st == 4;
+    const bool need_b = !EXACT && st == 4;
     const T* csrc = need_c ? A.c : A.u;
-    const T* asrc = need_a ? A.acc : A.u;
-    const long long cpm = need_c ? -1LL : 0LL, apm = need_a ? -1LL : 0LL;
-    const unsigned com = need_c ? ~0u : 0u, aom = need_a ? ~0u : 0u;
-    auto load_epi = [&](int o, T (&rc)[NO], T (&ra)[NO]) {
+    const T* asrc = need_a ? (EXACT ? A.acc : A.s2) : A.u;
+    const T* bsrc = need_b ? A.s3 : A.u;
+    const long long cpm = need_c ? -1LL : 0LL, apm = need_a ? -1LL : 0LL, bpm = need_b ? -1LL : 0LL;
+    const unsigned com = need_c ? ~0u : 0u, aom = need_a ? ~0u : 0u, bom = need_b ? ~0u : 0u;
+    auto load_epi = [&](int o, T (&rc)[NO], T (&ra)[NO], T (&rb)[NO]) {
         const long long pb = (long long)(o + g) * sxy;
 #pragma unroll
         for (int j = 0; j < NO; ++j) {
             rc[j] = at(csrc + (pb & cpm), ooff[j] & com);
             ra[j] = at(asrc + (pb & apm), ooff[j] & aom);
+            if constexpr (!EXACT) rb[j] = at(bsrc + (pb & bpm), ooff[j] & bom);
         }
     };
 
     T u0[NS], u1[NS], u2[NS], upf[NS], qv[NQ];
     T mum[NO], mu0[NO], mup[NS];
-    T pc[NO], pa[NO], nc[NO], na[NO];
+    T pc[NO], pa[NO], pq[NO], nc[NO], na[NO], nq[NO];
     bool ufix, qfix;
     T uval, qval;
     // ---- prologue: state planes z0-2 .. z0+1 at the mu slots, z0-1 and z0
@@ -875,7 +888,7 @@ void k_ch_stage_3dz(CHArgs<T> A) {
         load_cols(z0 + 1, r3, f3, v3);
         load_outer(z0 - 1, o0, g0, w0);
         load_outer(z0, o1, g1, w1);
-        load_epi(z0, pc, pa);
+        load_epi(z0, pc, pa, pq);
         T a0[NS], a1[NS];
         resolve_cols(r0, f0, v0, a0);
         resolve_cols(r1, f1, v1, a1);
@@ -913,7 +926,7 @@ void k_ch_stage_3dz(CHArgs<T> A) {
         // unconditional prefetch (clamped planes): the same loads every iteration
         load_cols(min(o + 3, z1 + 1), upf, ufix, uval);
         load_outer(min(o + 2, z1), qv, qfix, qval);
-        load_epi(min(o + 1, z1 - 1), nc, na);
+        load_epi(min(o + 1, z1 - 1), nc, na, nq);
         __syncthreads();   // B1
         bool zfix, znf;
         T zv;
@@ -948,15 +961,9 @@ void k_ch_stage_3dz(CHArgs<T> A) {
                     out = pc[j] + A.dt6 * (pa[j] + k);
                 }
             } else {
-                if (st == 1) {
-                    acc = axpy<T, false>(A.dt6, k, uc);
-                    out = axpy<T, false>(A.a, k, uc);
-                } else if (st < 4) {
-                    acc = axpy<T, false>(A.dt3, k, pa[j]);
-                    out = axpy<T, false>(A.a, k, pc[j]);
-                } else {
-                    out = axpy<T, false>(A.dt6, k, pa[j]);
-                }
+                if (st == 1) out = axpy<T, false>(A.a, k, uc);
+                else if (st < 4) out = axpy<T, false>(A.a, k, pc[j]);
+                else out = rk4_fin<T>(A.dt6, A.third, pc[j], pa[j], pq[j], uc, k);
             }
         }
         __syncthreads();   // B2
@@ -974,6 +981,7 @@ void k_ch_stage_3dz(CHArgs<T> A) {
             mu0[j] = mup[j];
             pc[j] = nc[j];
             pa[j] = na[j];
+            pq[j] = nq[j];
         }
     }
 }
@@ -1010,7 +1018,8 @@ template <typename T>
 struct PairArgs {
     const T* u;    // first stage's input state: c_n for (1,2), the stage-3 state for (3,4)
     const T* c;    // c_n (read by (3,4))
-    T* acc;        // RK4 accumulator: written by (1,2), read by (3,4)
+    T* acc;        // RK4 accumulator (FP64) or the stage-2 state u2 (FP32):
+                   // written by (1,2), read by (3,4)
     T* out;        // (1,2): stage-3 state; (3,4): c_{n+1}
     Layout L;
     AxisBC<T> bx, by, bz;
@@ -1018,6 +1027,7 @@ struct PairArgs {
     T two_rho, ca, cb, nkappa, M;
     T a1, a2;      // stage shifts: (dt/2, dt/2) for (1,2); (dt, unused) for (3,4)
     T dt6, dt3;
+    T third;       // 1/3 (FP32 RK4 update)
     FastCoef<T> F; // FP32 arithmetic form
     int zc;
     int xcd_remap;
@@ -1094,7 +1104,7 @@ __global__ __launch_bounds__(256) void k_ch_pair_3d(PairArgs<T> A) {
     constexpr int S4 = (N4 + 255) / 256, S3 = (N3 + 255) / 256, S2 = (N2 + 255) / 256, S1 = (N1 + 255) / 256;
     constexpr int S0 = N0 / 256;
     static_assert(N0 % 256 == 0, "tile must be a multiple of 256 cells");
-    constexpr bool NEED_CR = !SECOND || EXACT;   // c_n of the tile cells needed by the epilogue
+    constexpr bool NEED_CR = true;   // c_n of the tile cells, read by both epilogues
     __shared__ T UA[3][N4];
     __shared__ T MA[3][N3];
     __shared__ T UB[3][N2];
@@ -1186,6 +1196,7 @@ __global__ __launch_bounds__(256) void k_ch_pair_3d(PairArgs<T> A) {
     int ooff[S0];
     bool oval[S0];
     T areg[SECOND ? S0 : 1];
+    T breg[(SECOND && !EXACT) ? S0 : 1];   // FP32 (3,4): the stage-3 state at the tile cells
 #pragma unroll
     for (int j = 0; j < S0; ++j) {
         const int t = tid + 256 * j;
@@ -1227,6 +1238,10 @@ __global__ __launch_bounds__(256) void k_ch_pair_3d(PairArgs<T> A) {
         const long long pb = (long long)(o + g) * sxy;
 #pragma unroll
         for (int j = 0; j < (SECOND ? S0 : 1); ++j) areg[j] = A.acc[pb + ooff[j]];
+        if constexpr (SECOND && !EXACT) {
+#pragma unroll
+            for (int j = 0; j < S0; ++j) breg[j] = A.u[pb + ooff[j]];
+        }
     };
     // ghost rule of plane pz along z (decides before y and x)
     auto zrule = [&](int pz, bool& zfix, bool& zsw, T& zv) {
@@ -1311,8 +1326,8 @@ __global__ __launch_bounds__(256) void k_ch_pair_3d(PairArgs<T> A) {
                     dst[t] = v;
                     const int ox = lx - 2, oy = ly - 2;
                     if (ox >= 0 && ox < TX && oy >= 0 && oy < TY) {
-                        // stage 1 (FP32 form) keeps c + dt/6*k1, the others k
-                        kr[oy * TX + ox] = (!SECOND && !EXACT) ? axpy<T, EXACT>(A.dt6, k, cc) : k;
+                        // FP64 keeps k; FP32 the first stage's state (u2 or u4)
+                        kr[oy * TX + ox] = EXACT ? k : v;
                         if constexpr (NEED_CR) cr[oy * TX + ox] = cc;
                     }
                 }
@@ -1372,18 +1387,17 @@ __global__ __launch_bounds__(256) void k_ch_pair_3d(PairArgs<T> A) {
             const T k = k_star<T, EXACT>(A, s7);
             const long long gi = pb + ooff[j];
             if constexpr (!SECOND) {
-                // stage 2: acc = k1 + 2*k2 (FP64) or (c + dt/6*k1) + dt/3*k2; state c + dt/2*k2
+                // stage 2: FP64 acc = k1 + 2*k2; FP32 keeps u2 there; state c + dt/2*k2
                 if (EXACT) A.acc[gi] = kr[t] + (T)2 * k;
-                else A.acc[gi] = axpy<T, EXACT>(A.dt3, k, kr[t]);
+                else A.acc[gi] = kr[t];
                 A.out[gi] = axpy<T, EXACT>(A.a2, k, cr[t]);
             } else {
-                // stage 3 accumulates k3, stage 4 finishes the step
+                // stage 3 accumulates k3 (FP64) or forms u4 (FP32), stage 4 finishes the step
                 if (EXACT) {
                     const T ac = areg[j] + (T)2 * kr[t];
                     A.out[gi] = cr[t] + A.dt6 * (ac + k);
                 } else {
-                    const T ac = axpy<T, EXACT>(A.dt3, kr[t], areg[j]);
-                    A.out[gi] = axpy<T, EXACT>(A.dt6, k, ac);
+                    A.out[gi] = rk4_fin<T>(A.dt6, A.third, cr[t], areg[j], breg[j], kr[t], k);
                 }
             }
         }
@@ -1943,11 +1957,17 @@ static int rk4_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_params* p,
     A.dt3 = (T)(2.0 * dt6);
     A.c = c;
     A.acc = acc;
+    A.s2 = acc;
+    A.s3 = w0;
+    A.third = (T)(1.0 / 3.0);
     A.zc = zc;
     A.xcd_remap = 0;
     const T shift[4] = {(T)(dt * 0.5), (T)(dt * 0.5), (T)dt, (T)0};
-    const T* uin[4] = {c, w0, w1, w0};
-    T* uout[4] = {w0, w1, w0, c};
+    // FP64 and 2D: stage states in w0, w1, w0 and the accumulator in acc;
+    // 3D FP32: u2, u3, u4 in acc, w0, w1 (stage 4 reads u2 and u3)
+    const bool states = G->dim == 3 && !EXACT;
+    const T* uin[4] = {c, states ? acc : w0, states ? w0 : w1, states ? w1 : w0};
+    T* uout[4] = {states ? acc : w0, states ? w0 : w1, states ? w1 : w0, c};
     dim3 grid;
     if (G->dim == 3) {
         const Tile3 tl = shape3(tile);
@@ -2004,6 +2024,7 @@ static int rk4_pair_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_param
     const double dt = p->dt, dt6 = dt / 6.0;
     P.dt6 = (T)dt6;
     P.dt3 = (T)(2.0 * dt6);
+    P.third = (T)(1.0 / 3.0);
     P.acc = acc;
     P.zc = zc;
     const Tile3 tl = kTilesP[ptile];
@@ -2070,6 +2091,9 @@ static int rk4_pair_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_param
     A.dt3 = P.dt3;
     A.c = c;
     A.acc = acc;
+    A.s2 = acc;   // FP32: u2 and u3 from the (1,2) launch
+    A.s3 = w0;
+    A.third = P.third;
     A.zc = 16;
     const int stile = sizeof(T) == 8 ? 1 : 0;
     const dim3 sgrid(cdiv(L.n0, kTiles3[stile].tx), cdiv(L.n1, kTiles3[stile].ty), cdiv(L.n2, 16));
@@ -2114,6 +2138,7 @@ static int rk4_step2d_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_par
     const double dt = p->dt, dt6 = dt / 6.0;
     P.dt6 = (T)dt6;
     P.dt3 = (T)(2.0 * dt6);
+    P.third = (T)(1.0 / 3.0);
     P.a1 = (T)(dt * 0.5);
     P.a2 = (T)dt;
     P.c = nullptr;
