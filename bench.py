@@ -357,7 +357,7 @@ def bench_ens(a, with_cpu):
     capi.lib().phx_ensemble_destroy(h)
     cfg = {"workload": "ens512_ch_rk4_periodic", "runs": n, "cells": int(cells), "sides": "64+32*floor(15u)",
            "dt": 0.01, "scheme": "rk4", "bulk": "f32", "tile": [tx.value, ty.value],
-           "kernel": "two-stage, one workgroup per tile" if two == 1 else f"one-stage, persistent grid {gr.value}",
+           "kernel": ("two-stage" if two == 1 else "one-stage") + f", persistent grid {gr.value}",
            "hipgraph_steps": per_graph, "parallelism": "single-gpu"}
     return line("cell-updates/s, batched uneven-size CH RK4 ensemble", cells * steps / (ms / 1e3),
                 "cell-updates/s", steps, per_graph, ms, "f32",

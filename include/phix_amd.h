@@ -285,13 +285,13 @@ int phx_ensemble_mass(const phx_ensemble* ens, const float* c, double* out_dev, 
 int phx_ensemble_set_tiling(phx_ensemble* ens, int shape, int grid);
 int phx_ensemble_get_tiling(const phx_ensemble* ens, int* shape, int* grid, int* tx, int* ty);
 /* on = 1 (the default): two RK4 stages per launch (stages 1+2, then 3+4; 7
- * field passes per step instead of 16), one workgroup per tile; on = 0: the
- * persistent one-stage kernel. Both return the same bits. get returns 0/1
- * (-1 for NULL). */
+ * field passes per step instead of 16); on = 0: one stage per launch. Both
+ * kernels are persistent (grid from phx_ensemble_set_tiling) and return the
+ * same bits. get returns 0/1 (-1 for NULL). */
 int phx_ensemble_set_two_stage(phx_ensemble* ens, int on);
 int phx_ensemble_get_two_stage(const phx_ensemble* ens);
-/* Times every tile shape with the two-stage kernel and with the one-stage
- * kernel at {2,4,8,16} workgroups per CU, on `steps` steps of the given state
+/* Times both kernels at every tile shape and {2,4,8,16} workgroups per CU,
+ * on `steps` steps of the given state
  * (advancing it: pass a scratch copy), keeps the fastest and returns its ms
  * per step. Blocks the host. */
 int phx_ensemble_autotune(phx_ensemble* ens, float* c, float* work, double dt, int steps,

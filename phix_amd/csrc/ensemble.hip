@@ -221,7 +221,8 @@ __global__ __launch_bounds__(256) void k_ens_stage(EnsArgs A) {
 }
 
 // Two RK4 stages per launch (temporal blocking): stages (1,2) (SECOND =
-// false) or (3,4) (SECOND = true), one workgroup per tile. Per step
+// false) or (3,4) (SECOND = true), persistent like k_ens_stage, each tile's
+// loads issued while the previous tile is computed. Per step
 //   (1,2): read c_n;                          write acc and the stage-3 state;
 //   (3,4): read the stage-3 state, c_n, acc;  write c_{n+1} into a buffer other
 //          than c_n (the launch reads c_n around its tile),
@@ -230,8 +231,7 @@ __global__ __launch_bounds__(256) void k_ens_stage(EnsArgs A) {
 // rings the second stage needs (mu_A on tile+3, u_B on tile+2, mu_B on
 // tile+1). Ring cells outside a run are evaluated from the wrapped input,
 // which repeats the operands of the cell they wrap to, in the same order, so
-// the results equal two one-stage launches bit for bit. The workgroup ->
-// tile order is XCD-aware (a contiguous range of the list per XCD).
+// the results equal two one-stage launches bit for bit.
 // p in [-4, n + 3] wrapped into [0, n) for any n >= 2.
 __device__ __forceinline__ int wrap4(int p, int n) {
     p = p < 0 ? p + n : p;
@@ -257,90 +257,139 @@ __global__ __launch_bounds__(256) void k_ens_pair(EnsArgs A) {
     __shared__ float CB[SECOND ? BY * BX : 1];
     __shared__ float KA[TX * TY];   // first-stage RHS at the tile cells
     const int tid = threadIdx.x;
-    const int nb = gridDim.x;
-    const int h = blockIdx.x;
-    const int w = (nb & 7) == 0 ? (h & 7) * (nb >> 3) + (h >> 3) : h;
-    if (w >= A.ntiles) return;
-    const EnsTile T = A.tiles[w];
-    const EnsRun R = A.runs[T.run];
-    const int n = R.n;
-    const long long off = R.off;
-    const float nk = -R.kappa, M = R.M;
-
-    float hv[NS], cv[NC], pa[NO];
-#pragma unroll
-    for (int i = 0; i < NS; ++i) {
-        const int t = min(tid + 256 * i, UX * UY - 1);
-        const int ly = t / UX, lx = t - ly * UX;
-        const int px = wrap4(min(T.x0 - 4 + lx, n + 3), n), py = wrap4(min(T.y0 - 4 + ly, n + 3), n);
-        hv[i] = A.u[off + (long long)py * n + px];
+    // persistent walk, as k_ens_stage (XCD-aware when the grid allows)
+    int w, G, last;
+    if (A.xcd_walk) {
+        const int x = blockIdx.x & 7;
+        w = (int)((long long)x * A.ntiles / 8) + (blockIdx.x >> 3);
+        G = gridDim.x >> 3;
+        last = (int)((long long)(x + 1) * A.ntiles / 8) - 1;
+    } else {
+        w = blockIdx.x;
+        G = gridDim.x;
+        last = A.ntiles - 1;
     }
-    if constexpr (SECOND) {
+    if (w > last) return;
+    auto uidx = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
+    using CTile = const __attribute__((address_space(4))) EnsTile;
+    using CRun = const __attribute__((address_space(4))) EnsRun;
+    CTile* ctiles = (CTile*)A.tiles;
+    CRun* cruns = (CRun*)A.runs;
+    auto tile_at = [&](int i) {
+        EnsTile t;
+        t.run = ctiles[i].run;
+        t.x0 = ctiles[i].x0;
+        t.y0 = ctiles[i].y0;
+        return t;
+    };
+    auto run_at = [&](int i) {
+        EnsRun r;
+        r.off = cruns[i].off;
+        r.n = cruns[i].n;
+        r.kappa = cruns[i].kappa;
+        r.M = cruns[i].M;
+        return r;
+    };
+    // a tile's loads, issued one tile ahead (registers) so that they are in
+    // flight while the current tile is computed
+    float hv[NS], cv[NC], pa[NO];
+    auto load = [&](const EnsTile& T, const EnsRun& R) {
+        const int n = R.n;
+        const long long off = R.off;
 #pragma unroll
-        for (int i = 0; i < NC; ++i) {
-            const int t = min(tid + 256 * i, BX * BY - 1);
-            const int ly = t / BX, lx = t - ly * BX;
-            const int px = wrap4(min(T.x0 - 2 + lx, n + 3), n), py = wrap4(min(T.y0 - 2 + ly, n + 3), n);
-            cv[i] = A.c[off + (long long)py * n + px];
+        for (int i = 0; i < NS; ++i) {
+            const int t = min(tid + 256 * i, UX * UY - 1);
+            const int ly = t / UX, lx = t - ly * UX;
+            const int px = wrap4(min(T.x0 - 4 + lx, n + 3), n), py = wrap4(min(T.y0 - 4 + ly, n + 3), n);
+            hv[i] = A.u[off + (long long)py * n + px];
         }
+        if constexpr (SECOND) {
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                const int t = min(tid + 256 * i, BX * BY - 1);
+                const int ly = t / BX, lx = t - ly * BX;
+                const int px = wrap4(min(T.x0 - 2 + lx, n + 3), n), py = wrap4(min(T.y0 - 2 + ly, n + 3), n);
+                cv[i] = A.c[off + (long long)py * n + px];
+            }
+#pragma unroll
+            for (int j = 0; j < NO; ++j) {
+                const int t = tid + 256 * j;
+                const int oy = t / TX, ox = t - oy * TX;
+                pa[j] = A.acc[off + (long long)min(T.y0 + oy, n - 1) * n + min(T.x0 + ox, n - 1)];
+            }
+        }
+    };
+    EnsTile T = tile_at(uidx(w));
+    EnsRun R = run_at(uidx(T.run));
+    load(T, R);
+    for (;;) {
+        const int wn = w + G;
+        const EnsTile Tn = tile_at(uidx(min(wn, last)));
+        const EnsRun Rn = run_at(uidx(Tn.run));
+#pragma unroll
+        for (int i = 0; i < NS; ++i)
+            if (tid + 256 * i < UX * UY) U[tid + 256 * i] = hv[i];
+        float pc[NO];
+        if constexpr (SECOND) {
+#pragma unroll
+            for (int i = 0; i < NC; ++i)
+                if (tid + 256 * i < BX * BY) CB[tid + 256 * i] = cv[i];
+#pragma unroll
+            for (int j = 0; j < NO; ++j) pc[j] = pa[j];
+        }
+        __syncthreads();
+        load(Tn, Rn);   // next tile (the last tile again at the end: valid, unused)
+        const int n = R.n;
+        const long long off = R.off;
+        const float nk = -R.kappa, M = R.M;
+        for (int t = tid; t < AX * AY; t += 256) {
+            const int ay = t / AX, ax = t - ay * AX;
+            const int c0 = (ay + 1) * UX + ax + 1;
+            MA[t] = ens_mu(U[c0], U[c0 + 1], U[c0 - 1], U[c0 + UX], U[c0 - UX], nk);
+        }
+        __syncthreads();
+        // first stage: u_B = base + a k on tile+2; k kept at the tile cells
+        for (int t = tid; t < BX * BY; t += 256) {
+            const int by = t / BX, bx = t - by * BX;
+            const int m0 = (by + 1) * AX + bx + 1;
+            const float k = ens_k(MA[m0], MA[m0 + 1], MA[m0 - 1], MA[m0 + AX], MA[m0 - AX], M);
+            const float base = SECOND ? CB[t] : U[(by + 2) * UX + bx + 2];
+            UB[t] = __builtin_fmaf(A.a, k, base);
+            const int ox = bx - 2, oy = by - 2;
+            if (ox >= 0 && ox < TX && oy >= 0 && oy < TY) KA[oy * TX + ox] = k;
+        }
+        __syncthreads();
+        for (int t = tid; t < MX * MY; t += 256) {
+            const int my = t / MX, mx = t - my * MX;
+            const int c0 = (my + 1) * BX + mx + 1;
+            MB[t] = ens_mu(UB[c0], UB[c0 + 1], UB[c0 - 1], UB[c0 + BX], UB[c0 - BX], nk);
+        }
+        __syncthreads();
+        // second stage and the RK4 epilogue at the tile cells
 #pragma unroll
         for (int j = 0; j < NO; ++j) {
             const int t = tid + 256 * j;
             const int oy = t / TX, ox = t - oy * TX;
-            pa[j] = A.acc[off + (long long)min(T.y0 + oy, n - 1) * n + min(T.x0 + ox, n - 1)];
+            const int x = T.x0 + ox, y = T.y0 + oy;
+            if (x >= n || y >= n) continue;
+            const int m0 = (oy + 1) * MX + ox + 1;
+            const float k = ens_k(MB[m0], MB[m0 + 1], MB[m0 - 1], MB[m0 + MX], MB[m0 - MX], M);
+            const long long gi = off + (long long)y * n + x;
+            if constexpr (!SECOND) {
+                const float cn = U[(oy + 4) * UX + ox + 4];
+                const float am = __builtin_fmaf(A.dt6, KA[t], cn);   // accumulator after stage 1
+                A.acc[gi] = __builtin_fmaf(A.dt3, k, am);
+                A.out[gi] = __builtin_fmaf(A.a2, k, cn);
+            } else {
+                const float am = __builtin_fmaf(A.dt3, KA[t], pc[j]);   // after stage 3
+                A.out[gi] = __builtin_fmaf(A.dt6, k, am);
+            }
         }
-    }
-#pragma unroll
-    for (int i = 0; i < NS; ++i)
-        if (tid + 256 * i < UX * UY) U[tid + 256 * i] = hv[i];
-    if constexpr (SECOND) {
-#pragma unroll
-        for (int i = 0; i < NC; ++i)
-            if (tid + 256 * i < BX * BY) CB[tid + 256 * i] = cv[i];
-    }
-    __syncthreads();
-    for (int t = tid; t < AX * AY; t += 256) {
-        const int ay = t / AX, ax = t - ay * AX;
-        const int c0 = (ay + 1) * UX + ax + 1;
-        MA[t] = ens_mu(U[c0], U[c0 + 1], U[c0 - 1], U[c0 + UX], U[c0 - UX], nk);
-    }
-    __syncthreads();
-    // first stage: u_B = base + a k on tile+2; k kept at the tile cells
-    for (int t = tid; t < BX * BY; t += 256) {
-        const int by = t / BX, bx = t - by * BX;
-        const int m0 = (by + 1) * AX + bx + 1;
-        const float k = ens_k(MA[m0], MA[m0 + 1], MA[m0 - 1], MA[m0 + AX], MA[m0 - AX], M);
-        const float base = SECOND ? CB[t] : U[(by + 2) * UX + bx + 2];
-        UB[t] = __builtin_fmaf(A.a, k, base);
-        const int ox = bx - 2, oy = by - 2;
-        if (ox >= 0 && ox < TX && oy >= 0 && oy < TY) KA[oy * TX + ox] = k;
-    }
-    __syncthreads();
-    for (int t = tid; t < MX * MY; t += 256) {
-        const int my = t / MX, mx = t - my * MX;
-        const int c0 = (my + 1) * BX + mx + 1;
-        MB[t] = ens_mu(UB[c0], UB[c0 + 1], UB[c0 - 1], UB[c0 + BX], UB[c0 - BX], nk);
-    }
-    __syncthreads();
-    // second stage and the RK4 epilogue at the tile cells
-#pragma unroll
-    for (int j = 0; j < NO; ++j) {
-        const int t = tid + 256 * j;
-        const int oy = t / TX, ox = t - oy * TX;
-        const int x = T.x0 + ox, y = T.y0 + oy;
-        if (x >= n || y >= n) continue;
-        const int m0 = (oy + 1) * MX + ox + 1;
-        const float k = ens_k(MB[m0], MB[m0 + 1], MB[m0 - 1], MB[m0 + MX], MB[m0 - MX], M);
-        const long long gi = off + (long long)y * n + x;
-        if constexpr (!SECOND) {
-            const float cn = U[(oy + 4) * UX + ox + 4];
-            const float am = __builtin_fmaf(A.dt6, KA[t], cn);   // accumulator after stage 1
-            A.acc[gi] = __builtin_fmaf(A.dt3, k, am);
-            A.out[gi] = __builtin_fmaf(A.a2, k, cn);
-        } else {
-            const float am = __builtin_fmaf(A.dt3, KA[t], pa[j]);   // after stage 3
-            A.out[gi] = __builtin_fmaf(A.dt6, k, am);
-        }
+        if (wn > last) break;
+        w = wn;
+        T = Tn;
+        R = Rn;
+        __syncthreads();   // LDS is reused by the next tile
     }
 }
 
@@ -386,19 +435,18 @@ static void launch_stage(int grid, const EnsArgs& A, hipStream_t st) {
 }
 
 template <int TX, int TY>
-static void launch_pair_t(bool second, const EnsArgs& A, hipStream_t st) {
-    const unsigned grid = (unsigned)((A.ntiles + 7) / 8 * 8);
+static void launch_pair_t(bool second, int grid, const EnsArgs& A, hipStream_t st) {
     if (second) hipLaunchKernelGGL((k_ens_pair<TX, TY, true>), dim3(grid), dim3(256), 0, st, A);
     else hipLaunchKernelGGL((k_ens_pair<TX, TY, false>), dim3(grid), dim3(256), 0, st, A);
 }
 
-static void launch_pair(int shape, bool second, const EnsArgs& A, hipStream_t st) {
+static void launch_pair(int shape, bool second, int grid, const EnsArgs& A, hipStream_t st) {
     switch (shape) {
-        case 0: launch_pair_t<64, 16>(second, A, st); break;
-        case 1: launch_pair_t<32, 16>(second, A, st); break;
-        case 2: launch_pair_t<64, 8>(second, A, st); break;
-        case 3: launch_pair_t<128, 8>(second, A, st); break;
-        default: launch_pair_t<32, 32>(second, A, st); break;
+        case 0: launch_pair_t<64, 16>(second, grid, A, st); break;
+        case 1: launch_pair_t<32, 16>(second, grid, A, st); break;
+        case 2: launch_pair_t<64, 8>(second, grid, A, st); break;
+        case 3: launch_pair_t<128, 8>(second, grid, A, st); break;
+        default: launch_pair_t<32, 32>(second, grid, A, st); break;
     }
 }
 
@@ -564,13 +612,13 @@ static void enqueue_steps(phx_ensemble* E, float* c, float* work, double dt, int
             A.a = shift[0];
             A.a2 = shift[1];
             launch_timer_begin(st);
-            launch_pair(E->shape, false, A, st);
+            launch_pair(E->shape, false, E->grid, A, st);
             launch_timer_end(8, st);
             A.u = w1;
             A.out = nxt;
             A.a = shift[2];
             launch_timer_begin(st);
-            launch_pair(E->shape, true, A, st);
+            launch_pair(E->shape, true, E->grid, A, st);
             launch_timer_end(9, st);
         }
         if (nsteps % 2) (void)hipMemcpyAsync(c, w0, (size_t)N * sizeof(float), hipMemcpyDeviceToDevice, st);
@@ -644,13 +692,13 @@ int phx_ensemble_autotune(phx_ensemble* E, float* c, float* work, double dt, int
     PHX_HIP(hipEventCreate(&e1));
     float best = 1e30f;
     int bs = 0, bg = 0, b2 = 0;
-    // one-stage candidates: (shape, persistent grid); two-stage: shape (one
-    // workgroup per tile, m = 0)
-    const int mult[] = {0, 2, 4, 8, 16};
+    // candidates: kernel (two-stage, one-stage) x shape x persistent grid
+    const int mult[] = {2, 4, 8, 16};
+    for (int two = 1; two >= 0; --two)
     for (int sh = 0; sh < kNumShapes; ++sh)
         for (int m : mult) {
-            int grid = m ? std::min(E->ntiles[sh], m * E->cus) : std::min(E->ntiles[sh], 8 * E->cus);
-            E->two_stage = m == 0;
+            int grid = std::min(E->ntiles[sh], m * E->cus);
+            E->two_stage = two;
             E->shape = sh;
             E->grid = grid;
             enqueue_steps(E, c, work, dt, 1, st);   // warm
