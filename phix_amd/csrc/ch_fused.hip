@@ -2222,12 +2222,12 @@ static Launch3 launch_for(const phx_grid* G, int dtype) {
     auto it = g_zc.find(std::make_tuple(G->dim, G->n[0], G->n[1], G->n[2], dtype));
     if (it != g_zc.end()) return it->second;
     // defaults: the launches measured fastest at 1024^2 and 256^3 on MI355X
-    // (FP32 2D: whole-step 32x32; 3D: z-register kernel with 128x16 columns
-    // per 512-thread workgroup, z-chunk 16 in FP32 and 32 in FP64); every
-    // launch returns the same bits
+    // (FP32 2D: whole-step 32x32; 3D: z-register kernel with 64x16 columns
+    // per 256-thread workgroup at z-chunk 16 in FP32, 128x16 per 512 threads
+    // at z-chunk 32 in FP64); every launch returns the same bits
     if (G->dim == 2) return Launch3{1, dtype == PHX_F32 && step2d_ok(G) ? 4 : 0};
     if (G->dim != 3) return Launch3{1, 0};
-    return dtype == PHX_F32 ? Launch3{16, kFirstZ + 2} : Launch3{32, kFirstZ + 2};
+    return dtype == PHX_F32 ? Launch3{16, kFirstZ + 0} : Launch3{32, kFirstZ + 2};
 }
 
 static void set_launch(const phx_grid* G, int dtype, Launch3 l) {

@@ -80,7 +80,7 @@ def test_launch_table_defaults_and_ranges():
     temporally blocked kernels and their error codes."""
     lib = capi.lib()
     g3 = capi.make_grid((40, 36, 34), bc=capi.NOFLUX)
-    assert _launch(g3, capi.F32) == (16, 11)    # z-register 128x16, 512 threads
+    assert _launch(g3, capi.F32) == (16, 9)     # z-register 64x16, 256 threads
     assert _launch(g3, capi.F64) == (32, 11)
     g2 = capi.make_grid((1024, 1024), bc=capi.PERIODIC)
     assert _launch(g2, capi.F32) == (1, 4)      # whole-step 32x32
