@@ -385,7 +385,9 @@ def bench_euler(a, with_cpu):
     run(10)
     ms = timed(lambda: run(steps), stream)
     cells = NX * NX
-    roof = roofline(2 * 8 * cells, (ms / 1e3) / (2 * steps), "k_euler_mu+k_euler_c", "ch2d_euler_f64_128")
+    # one launch per step (k_ch_stage_2d in its exact form, Euler stage; the
+    # last step of a call uses k_euler_mu + k_euler_c): read c, write c'
+    roof = roofline(2 * 8 * cells, (ms / 1e3) / steps, "k_ch_stage_2d (Euler step)", "ch2d_euler_f64_128")
     cpu = None
     if with_cpu:
         from oracle import oracle as O

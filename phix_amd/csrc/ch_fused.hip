@@ -207,10 +207,15 @@ __device__ __forceinline__ T ghost_value_xy(int code, const T* bcv) {
 }
 
 // Stage epilogue for one physical output cell; cn, an: c_n and the
-// accumulator at the cell (loaded beforehand, used by stages 2-4).
+// accumulator at the cell (loaded beforehand, used by stages 2-4). Stage 0 is
+// an explicit Euler step, u + a k with a = dt (no accumulator).
 template <typename T, bool EXACT>
 __device__ __forceinline__ void epilogue(const CHArgs<T>& A, long long gi, T uc, T k, T cn, T an) {
     const int s = A.stage;
+    if (s == 0) {
+        A.out[gi] = EXACT ? uc + A.a * k : axpy<T, false>(A.a, k, uc);
+        return;
+    }
     if (EXACT) {
         if (s == 1) {
             A.acc[gi] = k;
@@ -1743,8 +1748,61 @@ static FastCoef<T> fast_coef(const phx_grid* G, const phx_ch_params* p) {
     return F;
 }
 
+static int g_xcd_remap = 1;
+static int g_euler_fused = 1;
+
+// Multi-step Euler, one launch per step (1D and 2D): the one-stage stencil
+// kernel in its exact form evaluates mu on a ring around each 32x8 tile (the
+// ghost mu follows the BC fill of mu, as k_euler_mu's ghost cells do) and
+// writes c + dt * (M lap mu), the operations of k_euler_mu / k_euler_c in the
+// same order, so the results are the same bits. The state alternates
+// between c and mu (free until the last step); every step but the last runs
+// this way, the last one with the two kernels on c, which leaves c's ghost
+// layer and mu as the two-kernel path does.
+template <typename T>
+static int euler_fused_steps_t(const phx_grid* G, T* c, T* mu, const phx_ch_params* p, int nsteps, hipStream_t st) {
+    CHArgs<T> A;
+    A.L = make_layout(G);
+    A.bx = axis_bc<T>(G, 0);
+    A.by = axis_bc<T>(G, 1);
+    A.bz = axis_bc<T>(G, 2);
+    fill_common<T>(G, p, A.idx2, A.idy2, A.idz2, A.two_rho, A.ca, A.cb, A.nkappa, A.M);
+    A.F = fast_coef<T>(G, p);
+    A.a = (T)p->dt;
+    A.dt6 = A.dt3 = (T)0;
+    A.third = (T)0;
+    A.s2 = A.s3 = nullptr;
+    A.stage = 0;
+    A.zc = 1;
+    const dim3 grid(cdiv(A.L.n0, 32), G->dim >= 2 ? cdiv(A.L.n1, 8) : 1, 1);
+    A.xcd_remap = g_xcd_remap && (grid.x * grid.y) % 8 == 0;
+    const int fused = nsteps - 1;
+    T* X = c;
+    T* Y = mu;
+    if (fused % 2) {   // odd count: start from a copy in mu so the state ends in c
+        PHX_HIP(hipMemcpyAsync(mu, c, (size_t)A.L.total() * sizeof(T), hipMemcpyDeviceToDevice, st));
+        X = mu;
+        Y = c;
+    }
+    for (int s = 0; s < fused; ++s) {
+        A.u = X;
+        A.c = X;
+        A.acc = Y;
+        A.out = Y;
+        if (G->dim == 2) hipLaunchKernelGGL((k_ch_stage_2d<T, 2, 32, 8, true>), grid, dim3(256), 0, st, A);
+        else hipLaunchKernelGGL((k_ch_stage_2d<T, 1, 32, 8, true>), grid, dim3(256), 0, st, A);
+        std::swap(X, Y);
+    }
+    return PHX_OK;
+}
+
 template <typename T>
 static int euler_steps_t(const phx_grid* G, T* c, T* mu, const phx_ch_params* p, int nsteps, hipStream_t st) {
+    if (g_euler_fused && G->dim <= 2 && nsteps >= 2) {
+        int rc = euler_fused_steps_t<T>(G, c, mu, p, nsteps, st);
+        if (rc) return rc;
+        nsteps = 1;
+    }
     EulerArgs<T> A;
     A.c = c;
     A.mu = mu;
@@ -1901,7 +1959,6 @@ void launch_timer_end(int kind, hipStream_t st) {
 static inline void lt_begin(hipStream_t st) { launch_timer_begin(st); }
 static inline void lt_end(int kind, hipStream_t st) { launch_timer_end(kind, st); }
 
-static int g_xcd_remap = 1;
 
 // Resident workgroups per CU of the 3D kernel of launch tile `tile` (one-stage
 // 0-4; two-stage 5-8, sized by its (3,4) kernel; z-register 9-17), from the
@@ -2184,6 +2241,11 @@ static int check_ch_params(const phx_ch_params* p, const char* who) {
 using namespace phx;
 
 extern "C" {
+
+int phx_ch_euler_set_fused(int on) {
+    g_euler_fused = on ? 1 : 0;
+    return PHX_OK;
+}
 
 int phx_ch_euler_steps(const phx_grid* G, int dtype, void* c, void* mu, const phx_ch_params* p, int nsteps,
                        phx_stream s) {

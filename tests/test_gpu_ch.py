@@ -187,3 +187,44 @@ def test_mass_energy_against_oracle(dev, dtype, case):
     out2 = torch.empty(2, dtype=torch.float64, device=dev)
     capi.call("phx_mass_energy", C.byref(g), dtype, capi.ptr(tc), C.byref(p), capi.ptr(red), capi.ptr(out2), stream())
     assert torch.equal(out, out2)
+
+
+@pytest.mark.parametrize("dtype", [capi.F64, capi.F32])
+@pytest.mark.parametrize("n,bc,vals,nsteps", [
+    ((128, 128), P, 0.0, 2),
+    ((128, 128), P, 0.0, 37),
+    ((70, 33), N, 0.0, 10),
+    ((41, 30), [[F, N], [P, P]], [[0.3, 0], [0, 0]], 5),
+    ((30, 41), [[P, P], [N, F]], [[0, 0], [0, 0.6]], 4),
+    ((3, 2), N, 0.0, 3),
+    ((50,), P, 0.0, 9),
+])
+def test_euler_one_launch_per_step_equals_two(dev, dtype, n, bc, vals, nsteps):
+    """1D/2D Euler: the one-launch-per-step path returns the two-kernel path's
+    bits for c and mu (ghost layers included), odd and even step counts; FP64
+    also against the oracle."""
+    g = capi.make_grid(n, bc=bc, bc_value=vals)
+    p = ch_params(rho=1.0, ca=0.0, cb=1.0, kappa=0.5, M=1.0, dt=0.01)
+    c0 = init.to_stored(init.ch_field(n, 9, 0.5, 0.05), g)
+    out = {}
+    for fused in (0, 1):
+        capi.call("phx_ch_euler_set_fused", fused)
+        try:
+            tc = to_dev(c0, dev) if dtype == capi.F64 else to_dev(c0.astype(np.float32), dev)
+            tm = torch_zeros_like(tc)
+            capi.call("phx_ch_euler_steps", C.byref(g), dtype, capi.ptr(tc), capi.ptr(tm), C.byref(p), nsteps,
+                      stream())
+            out[fused] = (to_np(tc), to_np(tm))
+        finally:
+            capi.call("phx_ch_euler_set_fused", 1)
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    if dtype == capi.F64:
+        c, mu = c0.copy(), np.zeros_like(c0)
+        O.ch_euler_steps(g, p, c, mu, nsteps)
+        assert np.abs(capi.active(out[1][0], g) - capi.active(c, g)).max() <= TOL64
+        assert np.abs(capi.active(out[1][1], g) - capi.active(mu, g)).max() <= TOL64
+
+
+def torch_zeros_like(t):
+    import torch
+    return torch.zeros_like(t)
