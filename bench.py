@@ -102,12 +102,26 @@ def kernel_times(run, steps):
     return kinds
 
 
-def roofline(bytes_per_launch, launch_s, kernel, traffic_name, launch=None):
+# Algorithmic bytes follow one fixed accounting per workload: the bytes the
+# classic stage-fused RK4 moves per unit of work with mu kept on chip, 16
+# field passes per step (read c_n; S1 writes u2 and acc; S2, S3 read u, c_n,
+# acc and write u', acc; S4 reads u4, acc and writes c_{n+1}): 64 B per cell-step
+# in FP32, 128 B in FP64, x fields; Euler: read c, write c. A launch is
+# credited with the units of work it completes (a one-stage launch a quarter
+# step, a two-stage launch half a step, a whole-step launch one step), so
+# kernels that move fewer bytes for the same work (temporal blocking, the 3D
+# FP32 stage-state form) can exceed the one-stage bound. The kernel's own
+# bytes and the HBM traffic rocprofv3 measured are reported beside it.
+def roofline(bytes_per_launch, launch_s, kernel, traffic_name, launch=None, kernel_bytes=None):
     ach = bytes_per_launch / launch_s / 1e9
-    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(traffic_name, launch),
-            "kernel": kernel,
-            "alg_bytes_per_launch": bytes_per_launch, "avg_launch_us": round(launch_s * 1e6, 2)}
+    r = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(traffic_name, launch),
+         "kernel": kernel,
+         "alg_bytes_per_launch": bytes_per_launch, "avg_launch_us": round(launch_s * 1e6, 2)}
+    if kernel_bytes is not None:
+        r["kernel_bytes_per_launch"] = kernel_bytes
+        r["kernel_bytes_frac"] = round(kernel_bytes / launch_s / 1e9 / HBM_PEAK_GBS, 4)
+    return r
 
 
 def line(metric, value, unit, steps, warmup, ms, dtype, data, config, roof, cpu):
@@ -152,21 +166,23 @@ def bench_ch3d(a, with_cpu):
     ms = timed(lambda: run(a.steps), stream)
     kinds = kernel_times(run, a.steps)
     cells = NX ** 3
-    # algorithmic field passes per launch: one-stage kernel 13/4 (FP32: stage
-    # states u2, u3, u4 and no accumulator) or 17/4 (FP64: accumulator, c_n
-    # re-read in stage 4) on average over its 4 stages; two-stage (1,2): read
-    # c_n, write the stage-3 state and the accumulator (FP64) or u2 (FP32);
-    # (3,4): read the stage-3 state, c_n and the accumulator or u2, write
-    # c_{n+1} (FP32 also reads the stage-3 state at the cells, counted once)
-    passes = {0: (13 if dtype == capi.F32 else 17) / 4, 1: 3, 2: 4}
+    # algorithmic passes per launch (accounting above): one-stage 16/4,
+    # two-stage 16/2. The kernels' own passes: one-stage 13/4 (FP32 stage-state
+    # form) or 17/4 (FP64: the reference's order re-reads c_n in stage 4);
+    # two-stage (1,2): read c_n, write the stage-3 state and the accumulator
+    # or u2; (3,4): read the stage-3 state, c_n and the accumulator or u2,
+    # write c_{n+1}
+    alg = {0: 16 / 4, 1: 16 / 2, 2: 16 / 2}
+    own = {0: (13 if dtype == capi.F32 else 17) / 4, 1: 3, 2: 4}
     names = {0: "k_ch_stage_3dz" if tile.value >= 9 else "k_ch_stage_3d", 1: "k_ch_pair_3d (stages 1+2)",
              2: "k_ch_pair_3d (stages 3+4)"}
     tags = {0: "ch3d_rk4", 1: "ch3d_pair12", 2: "ch3d_pair34"}
     dom = max(kinds, key=lambda k: kinds[k][1])
-    roof = roofline(passes[dom] * esz * cells, kinds[dom][1] / 1e3 / kinds[dom][0], names[dom],
-                    f"{tags[dom]}_{a.dtype}_256", TILE_NAMES[tile.value])
+    roof = roofline(alg[dom] * esz * cells, kinds[dom][1] / 1e3 / kinds[dom][0], names[dom],
+                    f"{tags[dom]}_{a.dtype}_256", TILE_NAMES[tile.value], own[dom] * esz * cells)
     roof["launches"] = {names[k]: {"count": v[0], "avg_us": round(v[1] * 1e3 / v[0], 2),
-                                   "alg_bytes": passes[k] * esz * cells} for k, v in kinds.items()}
+                                   "alg_bytes": alg[k] * esz * cells, "kernel_bytes": own[k] * esz * cells}
+                        for k, v in kinds.items()}
     cpu = None
     if with_cpu:
         import numpy as np
@@ -218,16 +234,19 @@ def bench_ch2d(a, with_cpu):
     ms = timed(lambda: run(a.steps), stream)
     kinds = kernel_times(run, a.steps)
     cells = NX * NX
-    # passes per launch: one-stage 16/4 on average; whole step: read c_n, write c_{n+1}
-    passes = {0: 4, 3: 2}
+    # algorithmic passes per launch (accounting above): one-stage 16/4, whole
+    # step 16; the whole-step kernel itself reads c_n and writes c_{n+1}
+    alg = {0: 16 / 4, 3: 16}
+    own = {0: 4, 3: 2}
     names = {0: "k_ch_stage_2d", 3: "k_ch_step_2d"}
     dom = max(kinds, key=lambda k: kinds[k][1])
     tile_name = ["64x16 one-stage", "whole-step 64x16", "whole-step 64x32", "whole-step 128x16",
                  "whole-step 32x32"][tile.value]
-    roof = roofline(passes[dom] * 4 * cells, kinds[dom][1] / 1e3 / kinds[dom][0], names[dom],
-                    f"{'ch2d_rk4' if dom == 0 else 'ch2d_step'}_f32_1024", tile_name)
+    roof = roofline(alg[dom] * 4 * cells, kinds[dom][1] / 1e3 / kinds[dom][0], names[dom],
+                    f"{'ch2d_rk4' if dom == 0 else 'ch2d_step'}_f32_1024", tile_name, own[dom] * 4 * cells)
     roof["launches"] = {names[k]: {"count": v[0], "avg_us": round(v[1] * 1e3 / v[0], 2),
-                                   "alg_bytes": passes[k] * 4 * cells} for k, v in kinds.items()}
+                                   "alg_bytes": alg[k] * 4 * cells, "kernel_bytes": own[k] * 4 * cells}
+                        for k, v in kinds.items()}
     cpu = None
     if with_cpu:
         from oracle import oracle as O
@@ -267,17 +286,20 @@ def bench_ac(a, with_cpu):
     ms = timed(lambda: run(a.steps), stream)
     kinds = kernel_times(run, a.steps)
     pts = NX * NX
-    # algorithmic field passes per launch and field: one-stage kernel 16/4 on
-    # average over its 4 stages; two-stage (1,2): read c_n, write acc and the
-    # stage-3 state; (3,4): read the stage-3 state, c_n and acc, write c_{n+1}
-    passes = {4: 16 / 4, 5: 3, 6: 4}
+    # algorithmic passes per launch and field (accounting above): one-stage
+    # 16/4, two-stage 16/2. The two-stage kernels' own passes: (1,2) read c_n,
+    # write acc and the stage-3 state; (3,4) read the stage-3 state, c_n and
+    # acc, write c_{n+1}
+    alg = {4: 16 / 4, 5: 16 / 2, 6: 16 / 2}
+    own = {4: 16 / 4, 5: 3, 6: 4}
     names = {4: "k_ac_stage_2d", 5: "k_ac_pair_2d (stages 1+2)", 6: "k_ac_pair_2d (stages 3+4)"}
     launch = {4: "one-stage", 5: "two-stage (1,2)", 6: "two-stage (3,4)"}
     dom = max(kinds, key=lambda k: kinds[k][1])
-    roof = roofline(passes[dom] * NF * 4 * pts, kinds[dom][1] / 1e3 / kinds[dom][0], names[dom], "ac8_rk4_f32_1024",
-                    launch[dom])
+    roof = roofline(alg[dom] * NF * 4 * pts, kinds[dom][1] / 1e3 / kinds[dom][0], names[dom], "ac8_rk4_f32_1024",
+                    launch[dom], own[dom] * NF * 4 * pts)
     roof["launches"] = {names[k]: {"count": v[0], "avg_us": round(v[1] * 1e3 / v[0], 2),
-                                   "alg_bytes": passes[k] * NF * 4 * pts} for k, v in kinds.items()}
+                                   "alg_bytes": alg[k] * NF * 4 * pts, "kernel_bytes": own[k] * NF * 4 * pts}
+                        for k, v in kinds.items()}
     cpu = None
     if with_cpu:
         from oracle import oracle as O
@@ -330,14 +352,16 @@ def bench_ens(a, with_cpu):
     # for the Allen-Cahn kernels
     kinds = kernel_times(lambda k: capi.call("phx_ensemble_rk4_steps", h, capi.ptr(c), capi.ptr(work), 0.01, k, sh),
                          steps)
-    passes = {7: 16 / 4, 8: 3, 9: 4}
+    alg = {7: 16 / 4, 8: 16 / 2, 9: 16 / 2}
+    own = {7: 16 / 4, 8: 3, 9: 4}
     names = {7: "k_ens_stage", 8: "k_ens_pair (stages 1+2)", 9: "k_ens_pair (stages 3+4)"}
     launch = {7: "one-stage", 8: "two-stage (1,2)", 9: "two-stage (3,4)"}
     dom = max(kinds, key=lambda k: kinds[k][1])
-    roof = roofline(passes[dom] * 4 * cells, kinds[dom][1] / 1e3 / kinds[dom][0], names[dom], "ens512_rk4_f32",
-                    launch[dom])
+    roof = roofline(alg[dom] * 4 * cells, kinds[dom][1] / 1e3 / kinds[dom][0], names[dom], "ens512_rk4_f32",
+                    launch[dom], own[dom] * 4 * cells)
     roof["launches"] = {names[k]: {"count": v[0], "avg_us": round(v[1] * 1e3 / v[0], 2),
-                                   "alg_bytes": passes[k] * 4 * cells} for k, v in kinds.items()}
+                                   "alg_bytes": alg[k] * 4 * cells, "kernel_bytes": own[k] * 4 * cells}
+                        for k, v in kinds.items()}
     cpu = None
     if with_cpu:
         from oracle import oracle as O
