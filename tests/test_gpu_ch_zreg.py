@@ -33,6 +33,10 @@ CASES = [
     ((3, 2, 2), (1.0, 1.0, 1.0), 1, P, 0.0),
     ((2, 2, 3), (1.0, 1.0, 1.0), 1, N, 0.0),
     ((270, 9, 13), (1.0, 1.0, 1.0), 1, [[N, F], [F, F], [P, P]], [[0, 0.41], [0.39, 0.43], [0, 0]]),
+    # every tile inside the domain (the kernel's FULL form) for every z-register tile
+    ((256, 16, 20), (1.0, 0.9, 1.1), 1, [[P, P], [N, F], [N, N]], [[0, 0], [0, 0.45], [0, 0]]),
+    # FULL for the 64- and 128-wide tiles, not for the 256-wide ones; two ghost layers
+    ((128, 32, 9), (1.0, 1.0, 1.0), 2, N, 0.0),
 ]
 
 
