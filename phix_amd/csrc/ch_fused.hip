@@ -666,7 +666,10 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
 // thread's tile cells are then never ghosts and sit RS = NT / TX rows apart,
 // so their LDS indices and global offsets are one base plus constants
 // (immediate offsets) instead of per-slot tables: ~25 fewer VGPRs.
-template <typename T, int TX, int TY, int NT, bool EXACT, bool FULL = false>
+// SC: 0 reads the stage from A.stage; 1, 2 (stages 2 and 3, which share
+// their epilogue form) and 4 fix it at compile time, so the stage tests and
+// the unused epilogue loads fold away.
+template <typename T, int TX, int TY, int NT, bool EXACT, bool FULL = false, int SC = 0>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, NT))) void k_ch_stage_3dz(CHArgs<T> A) {
     constexpr int UX = TX + 4, UY = TY + 4, MX = TX + 2, MY = TY + 2;
     constexpr int NI = TX * TY, NRING = 2 * MX + 2 * TY, NOUT = 2 * UX + 2 * MY;
@@ -883,7 +886,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, NT))) void k_ch_stage_3
     // element of the stage state for every lane: one cache line per wave).
     // FP64: c_n (stages 2-4) and the accumulator (2-4); FP32: c_n (2-4) and
     // the stage states u2, u3 (4).
-    const int st = A.stage;
+    const int st = SC ? SC : A.stage;
     const bool need_c = EXACT ? (st == 2 || st == 3 || st == 4) : st >= 2;
     const bool need_a = EXACT ? st >= 2 : st == 4;
     const bool need_b = !EXACT && st == 4;
@@ -1893,10 +1896,15 @@ static bool zreg_ok(const phx_grid* G, int dtype) {
 // z-register launch: the FULL form when every tile lies inside the domain
 template <typename T, int TX, int TY, int NT, bool EXACT>
 static void launch_z(dim3 grid, const CHArgs<T>& A, hipStream_t st) {
-    if (A.L.n0 % TX == 0 && A.L.n1 % TY == 0)
-        hipLaunchKernelGGL((k_ch_stage_3dz<T, TX, TY, NT, EXACT, true>), grid, dim3(NT), 0, st, A);
-    else
+    if (A.L.n0 % TX == 0 && A.L.n1 % TY == 0) {
+        switch (A.stage) {
+            case 1: hipLaunchKernelGGL((k_ch_stage_3dz<T, TX, TY, NT, EXACT, true, 1>), grid, dim3(NT), 0, st, A); break;
+            case 4: hipLaunchKernelGGL((k_ch_stage_3dz<T, TX, TY, NT, EXACT, true, 4>), grid, dim3(NT), 0, st, A); break;
+            default: hipLaunchKernelGGL((k_ch_stage_3dz<T, TX, TY, NT, EXACT, true, 2>), grid, dim3(NT), 0, st, A);
+        }
+    } else {
         hipLaunchKernelGGL((k_ch_stage_3dz<T, TX, TY, NT, EXACT, false>), grid, dim3(NT), 0, st, A);
+    }
 }
 
 template <typename T, bool EXACT>
