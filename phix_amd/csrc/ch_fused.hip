@@ -666,9 +666,9 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
 // thread's tile cells are then never ghosts and sit RS = NT / TX rows apart,
 // so their LDS indices and global offsets are one base plus constants
 // (immediate offsets) instead of per-slot tables: ~25 fewer VGPRs.
-// SC: 0 reads the stage from A.stage; 1, 2 (stages 2 and 3, which share
-// their epilogue form) and 4 fix it at compile time, so the stage tests and
-// the unused epilogue loads fold away.
+// SC: 1, 2 (stages 2 and 3, which share their epilogue form) and 4 fix the
+// stage class at compile time, so the stage tests and the unused epilogue
+// loads fold away (0 would read it from A.stage).
 template <typename T, int TX, int TY, int NT, bool EXACT, bool FULL = false, int SC = 0>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, NT))) void k_ch_stage_3dz(CHArgs<T> A) {
     constexpr int UX = TX + 4, UY = TY + 4, MX = TX + 2, MY = TY + 2;
@@ -1894,17 +1894,19 @@ static bool zreg_ok(const phx_grid* G, int dtype) {
 }
 
 // z-register launch: the FULL form when every tile lies inside the domain
+template <typename T, int TX, int TY, int NT, bool EXACT, bool FULL>
+static void launch_zs(dim3 grid, const CHArgs<T>& A, hipStream_t st) {
+    switch (A.stage) {
+        case 1: hipLaunchKernelGGL((k_ch_stage_3dz<T, TX, TY, NT, EXACT, FULL, 1>), grid, dim3(NT), 0, st, A); break;
+        case 4: hipLaunchKernelGGL((k_ch_stage_3dz<T, TX, TY, NT, EXACT, FULL, 4>), grid, dim3(NT), 0, st, A); break;
+        default: hipLaunchKernelGGL((k_ch_stage_3dz<T, TX, TY, NT, EXACT, FULL, 2>), grid, dim3(NT), 0, st, A);
+    }
+}
+
 template <typename T, int TX, int TY, int NT, bool EXACT>
 static void launch_z(dim3 grid, const CHArgs<T>& A, hipStream_t st) {
-    if (A.L.n0 % TX == 0 && A.L.n1 % TY == 0) {
-        switch (A.stage) {
-            case 1: hipLaunchKernelGGL((k_ch_stage_3dz<T, TX, TY, NT, EXACT, true, 1>), grid, dim3(NT), 0, st, A); break;
-            case 4: hipLaunchKernelGGL((k_ch_stage_3dz<T, TX, TY, NT, EXACT, true, 4>), grid, dim3(NT), 0, st, A); break;
-            default: hipLaunchKernelGGL((k_ch_stage_3dz<T, TX, TY, NT, EXACT, true, 2>), grid, dim3(NT), 0, st, A);
-        }
-    } else {
-        hipLaunchKernelGGL((k_ch_stage_3dz<T, TX, TY, NT, EXACT, false>), grid, dim3(NT), 0, st, A);
-    }
+    if (A.L.n0 % TX == 0 && A.L.n1 % TY == 0) launch_zs<T, TX, TY, NT, EXACT, true>(grid, A, st);
+    else launch_zs<T, TX, TY, NT, EXACT, false>(grid, A, st);
 }
 
 template <typename T, bool EXACT>
@@ -2007,8 +2009,9 @@ static inline void lt_end(int kind, hipStream_t st) { launch_timer_end(kind, st)
 
 
 // Resident workgroups per CU of the 3D kernel of launch tile `tile` (one-stage
-// 0-4; two-stage 5-8, sized by its (3,4) kernel; z-register 9-17), from the
-// occupancy calculator; 0 if unknown.
+// 0-4; two-stage 5-8, sized by its (3,4) kernel; z-register 9-17, sized by
+// its stage 2-3 kernel for tiles that do not cover the domain exactly), from
+// the occupancy calculator; 0 if unknown.
 template <typename T, bool EXACT>
 static int blocks_per_cu(int tile) {
     const void* f = nullptr;
@@ -2022,15 +2025,15 @@ static int blocks_per_cu(int tile) {
         case 6: f = reinterpret_cast<const void*>(&k_ch_pair_3d<T, 32, 16, EXACT, true>); break;
         case 7: f = reinterpret_cast<const void*>(&k_ch_pair_3d<T, 32, 8, EXACT, true>); break;
         case 8: f = reinterpret_cast<const void*>(&k_ch_pair_3d<T, 64, 4, EXACT, true>); break;
-        case kFirstZ + 0: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 64, 16, 256, EXACT>); break;
-        case kFirstZ + 1: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 128, 8, 256, EXACT>); break;
-        case kFirstZ + 2: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 128, 16, 512, EXACT>); break;
-        case kFirstZ + 3: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 256, 8, 512, EXACT>); break;
-        case kFirstZ + 4: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 256, 16, 1024, EXACT>); break;
-        case kFirstZ + 5: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 256, 4, 256, EXACT>); break;
-        case kFirstZ + 6: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 256, 8, 1024, EXACT>); break;
-        case kFirstZ + 7: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 128, 16, 1024, EXACT>); break;
-        case kFirstZ + 8: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 128, 8, 512, EXACT>); break;
+        case kFirstZ + 0: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 64, 16, 256, EXACT, false, 2>); break;
+        case kFirstZ + 1: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 128, 8, 256, EXACT, false, 2>); break;
+        case kFirstZ + 2: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 128, 16, 512, EXACT, false, 2>); break;
+        case kFirstZ + 3: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 256, 8, 512, EXACT, false, 2>); break;
+        case kFirstZ + 4: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 256, 16, 1024, EXACT, false, 2>); break;
+        case kFirstZ + 5: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 256, 4, 256, EXACT, false, 2>); break;
+        case kFirstZ + 6: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 256, 8, 1024, EXACT, false, 2>); break;
+        case kFirstZ + 7: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 128, 16, 1024, EXACT, false, 2>); break;
+        case kFirstZ + 8: f = reinterpret_cast<const void*>(&k_ch_stage_3dz<T, 128, 8, 512, EXACT, false, 2>); break;
         default: return 0;
     }
     int nb = 0;
