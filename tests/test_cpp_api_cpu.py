@@ -1,5 +1,6 @@
 """CPU checks of the drop-in C++ API: the known-answer program (host path) and
 the .field format round trip through ScalarField::write/readFromFile."""
+import shutil
 import subprocess
 from pathlib import Path
 
@@ -36,13 +37,25 @@ def test_driver_usage_on_bad_settings(tmp_path):
     settings file is missing or malformed, before touching the device."""
     bad = tmp_path / "broken.jsonc"
     bad.write_text('{ "mesh": { "nx": 8, } }\n')
-    for exe in ("ch_2d", "ch_ac_2d"):
+    for exe in ("ch_2d", "ch_ac_2d", "gfa_2d"):
         for arg in (str(tmp_path / "absent.jsonc"), str(bad)):
             r = subprocess.run([str(BIN / exe), arg], capture_output=True, text=True, timeout=120, cwd=tmp_path)
             assert r.returncode == 1, (exe, arg, r.stdout, r.stderr)
             assert "Usage: " in r.stderr and "[path/to/settings.jsonc]" in r.stderr, r.stderr
         r = subprocess.run([str(BIN / exe)], capture_output=True, text=True, timeout=120, cwd=tmp_path)
         assert r.returncode == 1 and "settings/settings.jsonc" in r.stderr, r.stderr
+
+
+def test_gfa_2d_reads_reference_settings(tmp_path):
+    """gfa_2d parses the reference's FeB_ex1 settings file (JSONC with UTF-8
+    comments), reads every constant it names, and stops at the missing initial
+    field with the reference's error, before any device call."""
+    (tmp_path / "settings").mkdir()
+    shutil.copy(GOLDEN / "settings_gfa_2d.jsonc", tmp_path / "settings" / "settings.jsonc")
+    r = subprocess.run([str(BIN / "gfa_2d")], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert r.returncode == 1, (r.stdout, r.stderr)
+    assert "x: n=600  d=0.0003" in r.stdout
+    assert "IO::readField: cannot open file: settings/initial_field/c.field" in r.stderr, r.stderr
 
 
 def test_composite_forms_host_path(tmp_path):

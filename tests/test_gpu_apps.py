@@ -158,3 +158,57 @@ def test_ch_ac_2d_matches_numpy(dev, tmp_path):
     print(f"ch_ac_2d: max |c - numpy| = {ec:.3e}, max |eta - numpy| = {ee_:.3e}")
     assert np.abs(eta_ref - eta0[0]).max() > 1e-3   # the run moved the fields
     assert ec <= 1e-11 and ee_ <= 1e-11
+
+
+def _feb_constants():
+    """The constants block of the reference's FeB_ex1 settings (JSONC: the
+    // comments are stripped before json parses it)."""
+    import re
+    text = (ROOT / "tests" / "golden" / "settings_gfa_2d.jsonc").read_text(encoding="utf-8")
+    return json.loads(re.sub(r"//[^\n]*", "", text))["constants"]
+
+
+@pytest.mark.parametrize("bc", ["NoFlux", "Periodic"])
+def test_gfa_2d_matches_oracle_and_restarts(dev, tmp_path, bc):
+    """gfa_2d (GFA.cu) against the FP64 numpy oracle/gfa.py: a B-rich crystal
+    nucleus (phi = 1, eta = 0, c = 0.33 + U(-0.01, 0.01), radius 8 cells) in
+    an amorphous matrix (phi = 0, eta = 1, c = 0.25), the reference's FeB_ex1
+    constants, 48 x 40 cells of 20 um, dt = 0.2 us (inside the explicit limit
+    of the eta gradient term), 200 steps. Tolerance 1e-10 max-abs on c, phi,
+    eta (order-1 fields; different summation order, FMA contraction and the
+    device log). A warm start from step 100 reproduces step 200 bit for bit."""
+    from oracle import gfa as G
+    k = _feb_constants()
+    nx, ny, dx, dt, nsteps = 48, 40, 2e-5, 2e-7, 200
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    inside = (xx + 0.5 - nx / 2) ** 2 + (yy + 0.5 - ny / 2) ** 2 <= 8.0 ** 2
+    noise = np.random.default_rng(21).uniform(-0.01, 0.01, (ny, nx))
+    c0 = np.where(inside, 0.33 + noise, 0.25)
+    phi0 = np.where(inside, 1.0, 0.0)
+    eta0 = np.where(inside, 0.0, 1.0)
+    text = settings(nx, ny, k, nsteps, dt=dt, bcs=(bc,) * 4, fmt="BINARY", print_every=50, write_every=100)
+    text = text.replace('"dx": 1.0', f'"dx": {dx!r}').replace('"dy": 1.0', f'"dy": {dx!r}')
+    work = tmp_path / "run"
+    prepare(work, text, {"c": c0[None], "phi": phi0[None], "eta": eta0[None], "mu": np.zeros((1, ny, nx))})
+    out = run("gfa_2d", work)
+    assert "Starting GFA simulation (200 steps" in out and "Done." in out
+
+    m = G.Model(k)
+    c, phi, eta, _, _ = G.steps(m, c0, phi0, eta0, dx, dx, (bc, bc), dt, nsteps)
+    errs = {}
+    for name, ref in (("c", c), ("phi", phi), ("eta", eta)):
+        _, got = read_field(work / "output" / f"{name}_{nsteps}.field")
+        errs[name] = np.abs(got[0, 0] - ref).max()
+    print(f"gfa_2d {bc}: max |GPU - oracle| after {nsteps} steps: " +
+          ", ".join(f"{n} {e:.3e}" for n, e in errs.items()))
+    assert max(errs.values()) <= 1e-10, errs
+    # the run moved every field
+    assert np.abs(c - c0).max() > 1e-3 and np.abs(phi - phi0).max() > 1e-4 and np.abs(eta - eta0).max() > 0.5
+
+    finals = {n: (work / "output" / f"{n}_{nsteps}.field").read_bytes() for n in ("c", "phi", "eta")}
+    (work / "settings" / "settings.jsonc").write_text(text.replace('"start_from": "initial_field"',
+                                                                   '"start_from": "100"'))
+    out = run("gfa_2d", work)
+    assert "Resuming GFA simulation from step 100" in out
+    for n, b in finals.items():
+        assert (work / "output" / f"{n}_{nsteps}.field").read_bytes() == b, n
