@@ -93,7 +93,7 @@ def kernel_times(run, steps):
     capi.call("phx_ch_timing", 1)
     run(steps)
     kinds = {}
-    for k in range(10):
+    for k in range(11):
         n, t = C.c_int(), C.c_double()
         capi.call("phx_ch_timing_read", k, C.byref(n), C.byref(t))
         if n.value:
@@ -429,6 +429,69 @@ def bench_euler(a, with_cpu):
                 cfg, roof, cpu)
 
 
+def bench_gfa(a, with_cpu):
+    """The reference's FeB_ex1 case of the GFA_2D program (Fe-B glass
+    formation): 600^2 cells of 0.3 mm, zero-flux faces, dt = 1 us, its
+    constants, a B-rich crystal nucleus of radius 2 cells (c = 0.33 +
+    U(-0.01, 0.01), phi = 1, eta = 0) in an amorphous matrix (c = 0.25,
+    phi = 0, eta = 1), as its gen_initial_field.py lays out. One fused launch
+    per step (phx_gfa_euler_steps)."""
+    import torch
+    import numpy as np
+    from phix_amd import capi
+    NX = 600
+    DX = 3.0e-4
+    k = {"M_eta": 1.0, "D_Am": 3.0e-12, "eps_sq": 2.86e-3, "beta_sq": 1.0e-4, "w_phi": 441.0, "w_eta": 10.0,
+         "w_ex": 100.0, "T": 1000.0, "T_g": 710.0, "alpha": 0.45, "V_m_L": 2.64e-5, "V_m_S": 2.44e-5}
+    dev = torch.device("cuda:0")
+    g = capi.make_grid((NX, NX), d=(DX, DX), bc=capi.NOFLUX)
+    p = capi.GfaParams(dt=1.0e-6, **k)
+    yy, xx = np.mgrid[0:NX, 0:NX]
+    inside = (xx + 0.5 - NX / 2) ** 2 + (yy + 0.5 - NX / 2) ** 2 <= 2.0 ** 2
+    noise = np.random.default_rng(42).uniform(-0.01, 0.01, (NX, NX))
+    phys = [np.where(inside, 0.33 + noise, 0.25), np.where(inside, 1.0, 0.0), np.where(inside, 0.0, 1.0)]
+    fields = []
+    for f in phys:
+        a3 = np.zeros(capi.stored_shape(g))
+        capi.physical(a3, g)[0] = f
+        fields.append(torch.from_numpy(a3).to(dev))
+    work = torch.zeros(capi.lib().phx_gfa_workspace_bytes(C.byref(g)) // 8, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+    sh = capi.stream_handle(stream)
+    steps = 8 * a.steps   # even (the state ends in place); default 4000, a launch is ~12 us
+
+    def run(n):
+        capi.call("phx_gfa_euler_steps", C.byref(g), *(capi.ptr(f) for f in fields), capi.ptr(work), C.byref(p), n,
+                  sh)
+
+    run(a.warmup)
+    ms = timed(lambda: run(steps), stream)
+    cells = NX * NX
+    # The timed region is one k_gfa_step launch per step on the timed stream
+    # (an even step count: no final copy), so its average is the launch
+    # duration; events around each ~12 us launch would add ~2 us.
+    # Per cell-step: read c, phi, eta and write them, FP64; mu and D stay on chip.
+    roof = roofline(48 * cells, (ms / 1e3) / steps, "k_gfa_step", "gfa2d_feb_f64_600", kernel_bytes=48 * cells)
+    cpu = None
+    if with_cpu:
+        from oracle import gfa as G
+        sample = 40
+        t0 = time.perf_counter()
+        G.steps(G.Model(k), *phys, DX, DX, ("NoFlux", "NoFlux"), 1.0e-6, sample)
+        dt = time.perf_counter() - t0
+        cpu = {"value": cells * sample / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+               "sample": f"{sample} steps of the same 600^2 problem, FP64 numpy oracle (oracle/gfa.py), {dt:.1f} s "
+                         f"on {cpu_model()}"}
+    cfg = {"workload": "gfa2d_feb_ex1_600sq_noflux", "grid": [NX, NX], "dx": DX, "bc": "noflux",
+           "scheme": "euler, six multi-steps per step", "dt": 1.0e-6, "T": 1000.0, "bulk": "f64",
+           "kernel": "k_gfa_step 32x8", "parallelism": "single-gpu",
+           "note": "three 2.9 MB fields stay cache resident between steps"}
+    return line("cell-updates/s, Fe-B glass-formation (GFA_2D) Euler step", cells * steps / (ms / 1e3),
+                "cell-updates/s", steps, a.warmup, ms, "f64",
+                "synthetic (FeB_ex1 layout: radius-2 nucleus, c noise U(-0.01, 0.01) from numpy default_rng(42))",
+                cfg, roof, cpu)
+
+
 def bench_copy(a, with_cpu):
     """Plain device copy of 2 GiB (the STREAM-copy bandwidth of this box, and the
     known-size kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters)."""
@@ -499,7 +562,7 @@ def bench_hbmprobe(a, with_cpu):
 
 
 WORKLOADS = {"copycal": bench_copy, "ch3d": bench_ch3d, "ch2d1024": bench_ch2d, "ac1024": bench_ac, "ens512": bench_ens,
-             "euler128": bench_euler, "hbmprobe": bench_hbmprobe}
+             "euler128": bench_euler, "gfa600": bench_gfa, "hbmprobe": bench_hbmprobe}
 
 
 def main(argv=None) -> int:

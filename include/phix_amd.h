@@ -159,6 +159,35 @@ int phx_ch_euler_steps(const phx_grid* grid, int dtype, void* c, void* mu,
  * keeps two launches per step (process-wide). Returns PHX_OK. */
 int phx_ch_euler_set_fused(int on);
 
+/* Constants of the Fe-B glass-formation model as the reference's GFA_2D
+ * settings name them ("constants" block, GFA.cu:104-115), plus the step dt.
+ * Units: SI (J, mol, m, s, K). */
+typedef struct phx_gfa_params {
+    double dt;
+    double M_eta, eps_sq, beta_sq;
+    double w_phi, w_eta, w_ex;
+    double T, T_g, alpha;
+    double V_m_L, V_m_S, D_Am;
+} phx_gfa_params;
+
+/* Workspace for phx_gfa_euler_steps, in bytes (three stored FP64 fields). */
+size_t phx_gfa_workspace_bytes(const phx_grid* grid);
+/* nsteps Euler steps of the GFA_2D system on FP64 fields c, phi, eta (stored
+ * layout of `grid`, 2D, periodic or zero-flux faces): the reference's six
+ * multi-steps per step (GFA.cu:143-201 with Solver.cu:317-338: mu; D; D; c;
+ * phi; eta) as one fused launch -- mu and D are evaluated in LDS on each tile
+ * and a one-cell ring through the BC index maps, then c, phi (reading the new
+ * c) and eta (reading the new phi) are updated at each cell. The temperature-
+ * only quantities (M_phi, D_L, D_S, f_S, df_Am, the SGTE energies) are formed
+ * once on the host as GFA.cu:117-121 does. On return the physical cells hold
+ * the state after nsteps and the x/y ghost cells the BC fill of the state
+ * before the last step, as the reference leaves them; z ghost planes are not
+ * touched. work: phx_gfa_workspace_bytes(grid). Returns PHX_OK or
+ * PHX_ERR_INVALID (dim != 2, a FIXED face, ghost > n, a null pointer, a
+ * non-finite constant). */
+int phx_gfa_euler_steps(const phx_grid* grid, double* c, double* phi, double* eta, void* work,
+                        const phx_gfa_params* p, int nsteps, phx_stream stream);
+
 /* Workspace for phx_ch_rk4_steps, in bytes (three stored fields). */
 size_t phx_ch_rk4_workspace_bytes(const phx_grid* grid, int dtype);
 /* nsteps of classic RK4 on dc/dt = M*lap(f'(c) - kappa*lap(c)) with the BCs of
@@ -171,8 +200,8 @@ int phx_ch_rk4_steps(const phx_grid* grid, int dtype, void* c, void* work,
                      const phx_ch_params* p, int nsteps, phx_stream stream);
 
 /* Launch shape of the 3D stage kernels: z-chunk zc (default 16 FP32, 32
- * FP64; default tiles: 3D 11 for both dtypes, 2D FP32 the whole-step tile 4
- * when n >= 8, else 0), per
+ * FP64; default tiles: 3D 9 (z-register 64x16/256) FP32 and 11 (z-register
+ * 128x16/512) FP64, 2D FP32 the whole-step tile 4 when n >= 8, else 0), per
  * (dim, n, dtype); the planes are split into cdiv(n_z, zc) balanced chunks of
  * floor or ceil(n_z / chunks) planes, one workgroup column each. The
  * autotuner times every tile shape x z-chunk in {4,8,16,32,64} plus the
@@ -210,7 +239,8 @@ int phx_ch_rk4_autotune(const phx_grid* grid, int dtype, void* c, void* work,
  * recorded launches of one kind (CH: 0 one-stage kernel, 1 two-stage (1,2),
  * 2 two-stage (3,4), 3 2D whole step; Allen-Cahn: 4 one-stage kernel,
  * 5 two-stage (1,2), 6 two-stage (3,4); ensemble: 7 one-stage kernel, 8
- * two-stage (1,2), 9 two-stage (3,4)), waiting for them. Launches captured
+ * two-stage (1,2), 9 two-stage (3,4); GFA: 10 the fused step kernel),
+ * waiting for them. Launches captured
  * into a graph (phx_ensemble_graph_steps) must not run while it is on. One
  * thread at a time. */
 int phx_ch_timing(int on);

@@ -20,9 +20,18 @@
 // Settings: "mesh" {nx, ny, dx, dy, x0, y0}; "initialize" {start_from, dt,
 // nSteps}; "boundary_conditions"; "constants" {M_eta, eps_sq, beta_sq, w_phi,
 // w_eta, w_ex, T, T_g, alpha, V_m_L, V_m_S, D_Am}; "output". Initial fields
-// c, phi, eta and (cold start) mu. The equations run on the generic device path.
+// c, phi, eta and (cold start) mu.
+//
+// By default a step is one fused launch (phx_gfa_euler_steps: mu and D on
+// chip, c, phi, eta read and written once). "initialize": {"solver":
+// "generic"} (absent in the reference's settings) runs the six equations as
+// written below on the generic per-term device path instead; both give the
+// same fields to rounding.
+#include <algorithm>
 #include <iostream>
 #include <string>
+
+#include <hip/hip_runtime.h>
 
 #include "IO/ConfigFile.h"
 #include "IO/FieldIO.h"
@@ -30,7 +39,34 @@
 #include "boundary/BCFactory.h"
 #include "equation/Equation.h"
 #include "gfa_model.h"
+#include "phix_amd.h"
 #include "settings_run.h"
+
+namespace {
+
+// The fused path's grid: the mesh, the fields' ghost width, and the faces of
+// the settings' "boundary_conditions" (buildBCs has accepted them: Periodic
+// or NoFlux, periodic on both sides of an axis or neither).
+phx_grid fusedGrid(const PhiX::Mesh& mesh, int ghost, const PhiX::IO::Json& b) {
+    phx_grid g{};
+    g.dim = mesh.dim;
+    for (int a = 0; a < 3; ++a) {
+        g.n[a] = mesh.n[a];
+        g.d[a] = mesh.d[a];
+    }
+    g.ghost = ghost;
+    const char* faces[2][2] = {{"x_min", "x_max"}, {"y_min", "y_max"}};
+    for (int a = 0; a < 2; ++a)
+        for (int s = 0; s < 2; ++s)
+            g.bc[a][s] = b[faces[a][s]].get<std::string>() == "Periodic" ? PHX_BC_PERIODIC : PHX_BC_NOFLUX;
+    return g;
+}
+
+void hipCheck(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+}  // namespace
 
 int main(int argc, char* argv[]) {
     using namespace PhiX;
@@ -103,6 +139,44 @@ int main(int argc, char* argv[]) {
                             return -M_eta * dF;
                         }) +
                      M_eta * beta_sq * lap(eta));
+
+        const std::string plan = init.value("solver", "fused");
+        if (plan != "fused" && plan != "generic")
+            throw std::runtime_error("initialize.solver must be \"fused\" or \"generic\", got \"" + plan + "\"");
+        if (plan == "fused") {
+            const phx_grid g = fusedGrid(mesh, c.ghost, cfg["boundary_conditions"]);
+            const phx_gfa_params p{dt, M_eta, eps_sq, beta_sq, w_phi, w_eta, w_ex, T, T_g, alpha, V_m_L, V_m_S, D_Am};
+            void* work = nullptr;
+            hipCheck(hipMalloc(&work, phx_gfa_workspace_bytes(&g)), "gfa_2d: workspace");
+            std::cout << "  solver plan: fused GFA step (phx_gfa_euler_steps)\n";
+            IO::OutputWriter writer(cfg["output"]);
+            int step = start;
+            double time = start * dt;
+            if (start == 0) {
+                for (ScalarField* f : {&c, &phi, &eta}) writer.writeFields(*f, 0, time);
+                std::cout << "Starting GFA simulation (" << nSteps << " steps, dt=" << dt << ", T=" << T << " K)\n";
+            } else {
+                std::cout << "Resuming GFA simulation from step " << start << " (t=" << start * dt << "), "
+                          << nSteps - start << " steps remaining, dt=" << dt << "\n";
+            }
+            writer.resetTimer();
+            // the same cadence as apps::timeLoop: one device call up to the next
+            // print or write step; time accumulates dt per step as Solver does
+            auto nextMultiple = [&](int every) { return (step / every + 1) * every; };
+            while (step < nSteps) {
+                const int stop = std::min({nextMultiple(writer.printInterval), nextMultiple(writer.writeInterval), nSteps});
+                if (phx_gfa_euler_steps(&g, c.d_curr, phi.d_curr, eta.d_curr, work, &p, stop - step, nullptr) != PHX_OK)
+                    throw std::runtime_error(std::string("phx_gfa_euler_steps: ") + phx_last_error());
+                hipCheck(hipDeviceSynchronize(), "gfa_2d");
+                for (; step < stop; ++step) time += dt;
+                if (writer.shouldPrint(step)) writer.printProgress(step, time);
+                if (writer.shouldWrite(step))
+                    for (ScalarField* f : {&c, &phi, &eta}) writer.writeFields(*f, step, time);
+            }
+            hipCheck(hipFree(work), "gfa_2d: workspace");
+            std::cout << "Done.\n";
+            return 0;
+        }
 
         Solver solver({{&c, bc.ptrs, &eqMu, EquationType::STEADY},
                        {&mu, bc.ptrs, &eqD, EquationType::STEADY},

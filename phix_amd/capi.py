@@ -38,6 +38,11 @@ class ACParams(C.Structure):
     _fields_ = [(k, C.c_double) for k in ("L", "kappa", "gamma", "dt")]
 
 
+class GfaParams(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("dt", "M_eta", "eps_sq", "beta_sq", "w_phi", "w_eta", "w_ex", "T", "T_g",
+                                          "alpha", "V_m_L", "V_m_S", "D_Am")]
+
+
 def make_grid(n, d=(1.0, 1.0, 1.0), ghost=1, bc=PERIODIC, bc_value=0.0) -> Grid:
     """n: tuple of active sizes (len = dim). bc: one kind for every face, or a
     list per axis of (lo, hi) kinds. bc_value likewise."""
@@ -120,6 +125,8 @@ SIG = {
     "phx_ch_euler_set_fused": (C.c_int, [C.c_int]),
     "phx_ch_euler_steps": (C.c_int, [GP, C.c_int, VP, VP, C.POINTER(CHParams), C.c_int, VP]),
     "phx_ch_rk4_workspace_bytes": (C.c_size_t, [GP, C.c_int]),
+    "phx_gfa_workspace_bytes": (C.c_size_t, [GP]),
+    "phx_gfa_euler_steps": (C.c_int, [GP, VP, VP, VP, VP, C.POINTER(GfaParams), C.c_int, VP]),
     "phx_ch_rk4_steps": (C.c_int, [GP, C.c_int, VP, VP, C.POINTER(CHParams), C.c_int, VP]),
     "phx_ch_rk4_set_zchunk": (C.c_int, [GP, C.c_int, C.c_int]),
     "phx_ch_set_xcd_remap": (C.c_int, [C.c_int]),

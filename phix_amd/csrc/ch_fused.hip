@@ -2502,7 +2502,7 @@ int phx_ch_timing(int on) {
 }
 
 int phx_ch_timing_read(int kind, int* launches, double* total_ms) {
-    if (kind < 0 || kind > 9) return fail(PHX_ERR_INVALID, "phx_ch_timing_read: kind must be in [0, 9]");
+    if (kind < 0 || kind > 10) return fail(PHX_ERR_INVALID, "phx_ch_timing_read: kind must be in [0, 10]");
     int n = 0;
     double ms = 0.0;
     for (size_t i = 0; i < g_lt.used; ++i) {

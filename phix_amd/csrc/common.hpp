@@ -116,7 +116,8 @@ __host__ __device__ inline int cdiv(long long a, int b) { return (int)((a + b - 
 // these two calls is timed with HIP events on its stream and tagged with a
 // kind. Kinds: 0-3 the CH RK4 kernels, 4 the Allen-Cahn one-stage kernel,
 // 5 / 6 its two-stage kernels (stages 1+2 / 3+4), 7 the ensemble one-stage
-// kernel, 8 / 9 its two-stage kernels. Not for launches under graph capture.
+// kernel, 8 / 9 its two-stage kernels, 10 the GFA step kernel. Not for
+// launches under graph capture.
 void launch_timer_begin(hipStream_t st);
 void launch_timer_end(int kind, hipStream_t st);
 

@@ -168,9 +168,11 @@ def _feb_constants():
     return json.loads(re.sub(r"//[^\n]*", "", text))["constants"]
 
 
+@pytest.mark.parametrize("plan", [None, "generic"])
 @pytest.mark.parametrize("bc", ["NoFlux", "Periodic"])
-def test_gfa_2d_matches_oracle_and_restarts(dev, tmp_path, bc):
-    """gfa_2d (GFA.cu) against the FP64 numpy oracle/gfa.py: a B-rich crystal
+def test_gfa_2d_matches_oracle_and_restarts(dev, tmp_path, bc, plan):
+    """gfa_2d (GFA.cu) on its default fused step and (plan "generic") on the
+    per-term path against the FP64 numpy oracle/gfa.py: a B-rich crystal
     nucleus (phi = 1, eta = 0, c = 0.33 + U(-0.01, 0.01), radius 8 cells) in
     an amorphous matrix (phi = 0, eta = 1, c = 0.25), the reference's FeB_ex1
     constants, 48 x 40 cells of 20 um, dt = 0.2 us (inside the explicit limit
@@ -186,12 +188,15 @@ def test_gfa_2d_matches_oracle_and_restarts(dev, tmp_path, bc):
     c0 = np.where(inside, 0.33 + noise, 0.25)
     phi0 = np.where(inside, 1.0, 0.0)
     eta0 = np.where(inside, 0.0, 1.0)
-    text = settings(nx, ny, k, nsteps, dt=dt, bcs=(bc,) * 4, fmt="BINARY", print_every=50, write_every=100)
+    text = settings(nx, ny, k, nsteps, dt=dt, bcs=(bc,) * 4, fmt="BINARY", print_every=50, write_every=100,
+                    extra={"solver": plan} if plan else None)
     text = text.replace('"dx": 1.0', f'"dx": {dx!r}').replace('"dy": 1.0', f'"dy": {dx!r}')
     work = tmp_path / "run"
     prepare(work, text, {"c": c0[None], "phi": phi0[None], "eta": eta0[None], "mu": np.zeros((1, ny, nx))})
     out = run("gfa_2d", work)
     assert "Starting GFA simulation (200 steps" in out and "Done." in out
+    assert ("solver plan: generic multi-step" if plan else "solver plan: fused GFA step") in out
+    assert "[progress] step=200" in out
 
     m = G.Model(k)
     c, phi, eta, _, _ = G.steps(m, c0, phi0, eta0, dx, dx, (bc, bc), dt, nsteps)
@@ -199,7 +204,7 @@ def test_gfa_2d_matches_oracle_and_restarts(dev, tmp_path, bc):
     for name, ref in (("c", c), ("phi", phi), ("eta", eta)):
         _, got = read_field(work / "output" / f"{name}_{nsteps}.field")
         errs[name] = np.abs(got[0, 0] - ref).max()
-    print(f"gfa_2d {bc}: max |GPU - oracle| after {nsteps} steps: " +
+    print(f"gfa_2d {bc} {plan or 'fused'}: max |GPU - oracle| after {nsteps} steps: " +
           ", ".join(f"{n} {e:.3e}" for n, e in errs.items()))
     assert max(errs.values()) <= 1e-10, errs
     # the run moved every field

@@ -1,0 +1,151 @@
+"""GPU parity of the fused Fe-B glass-formation step (phx_gfa_euler_steps)
+against the FP64 numpy oracle (oracle/gfa.py, the reference's GFA.cu six
+multi-steps), called through the C ABI.
+
+Contract: physical cells of c, phi, eta within 1e-10 max-abs of the oracle
+(order-1 fields; summation order, FMA contraction and the device log differ);
+the x/y ghost cells equal, bit for bit, the BC fill of the state before the
+last step (what the reference's multi-step leaves); the z ghost planes of the
+caller's arrays are not touched; odd and even step counts (the state ends in
+the workspace or in place); ragged tiles, mixed faces, two ghost layers, the
+smallest axes; one timed launch (kind 10) per step."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from phix_amd import capi
+from oracle import gfa as G
+
+from helpers import to_dev, to_np, stream
+
+pytestmark = pytest.mark.gpu
+KIND = {"Periodic": capi.PERIODIC, "NoFlux": capi.NOFLUX}
+# the reference's FeB_ex1 constants (tests/golden/settings_gfa_2d.jsonc)
+FEB = {"M_eta": 1.0, "D_Am": 3.0e-12, "eps_sq": 2.86e-3, "beta_sq": 1.0e-4, "w_phi": 441.0, "w_eta": 10.0,
+       "w_ex": 100.0, "T": 1000.0, "T_g": 710.0, "alpha": 0.45, "V_m_L": 2.64e-5, "V_m_S": 2.44e-5}
+DT = 2e-7
+CASES = [
+    # (nx, ny), (dx, dy), ghost, (x faces, y faces)
+    ((48, 40), (2e-5, 2e-5), 1, ("NoFlux", "NoFlux")),
+    ((37, 29), (2e-5, 2.5e-5), 1, ("Periodic", "Periodic")),
+    ((70, 9), (2e-5, 2e-5), 2, ("Periodic", "NoFlux")),
+    ((3, 2), (2e-5, 2e-5), 1, ("NoFlux", "Periodic")),
+]
+
+
+def nucleus(n, seed):
+    """B-rich crystal nucleus (phi = 1, eta = 0, c = 0.33 + U(-0.01, 0.01)) in
+    an amorphous matrix (phi = 0, eta = 1, c = 0.25); numpy (ny, nx)."""
+    nx, ny = n
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    r = max(1.0, min(nx, ny) / 5.0)
+    inside = (xx + 0.5 - nx / 2) ** 2 + (yy + 0.5 - ny / 2) ** 2 <= r * r
+    noise = np.random.default_rng(seed).uniform(-0.01, 0.01, (ny, nx))
+    return np.where(inside, 0.33 + noise, 0.25), np.where(inside, 1.0, 0.0), np.where(inside, 0.0, 1.0)
+
+
+def grid(case):
+    n, d, ghost, bc = CASES[case]
+    return capi.make_grid(n, d=d, ghost=ghost, bc=[[KIND[bc[0]]] * 2, [KIND[bc[1]]] * 2])
+
+
+def params():
+    return capi.GfaParams(dt=DT, **FEB)
+
+
+def stored(phys, g, zfill=7.0):
+    """Stored array with the physical cells set, zeros on the x/y ghosts and
+    `zfill` on the z ghost planes."""
+    a = np.zeros(capi.stored_shape(g))
+    a[:g.ghost] = zfill
+    a[g.ghost + 1:] = zfill
+    capi.physical(a, g)[0] = phys
+    return a
+
+
+def run(g, fields, nsteps, dev):
+    import torch
+    t = [to_dev(f, dev, torch.float64) for f in fields]
+    work = torch.zeros(capi.lib().phx_gfa_workspace_bytes(C.byref(g)) // 8, dtype=torch.float64, device=dev)
+    p = params()
+    capi.call("phx_gfa_euler_steps", C.byref(g), *(capi.ptr(x) for x in t), capi.ptr(work), C.byref(p), nsteps,
+              stream())
+    torch.cuda.synchronize()
+    return [to_np(x) for x in t]
+
+
+def ghost_fill(a, g, bc):
+    """The BC fill of a stored 2D state on its x/y ghost ring (corners too),
+    the zero-flux mirror or periodic wrap of each layer."""
+    gh, (nx, ny) = g.ghost, (g.n[0], g.n[1])
+    out = a.copy()
+    plane = out[gh]
+
+    def src(l, hi, n, kind):
+        if kind == "Periodic":
+            return l - 1 if hi else n - l
+        return n - l if hi else l - 1
+
+    xs = {gh + i: gh + i for i in range(nx)}
+    ys = {gh + j: gh + j for j in range(ny)}
+    for l in range(1, gh + 1):
+        xs[gh - l] = gh + src(l, False, nx, bc[0])
+        xs[gh + nx - 1 + l] = gh + src(l, True, nx, bc[0])
+        ys[gh - l] = gh + src(l, False, ny, bc[1])
+        ys[gh + ny - 1 + l] = gh + src(l, True, ny, bc[1])
+    for y, sy in ys.items():
+        for x, sx in xs.items():
+            plane[y, x] = a[gh][sy, sx]
+    return out
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_gfa_fused_matches_oracle(dev, case):
+    n, d, ghost, bc = CASES[case]
+    g = grid(case)
+    m = G.Model(FEB)
+    c0, p0, e0 = nucleus(n, 40 + case)
+    worst = 0.0
+    for nsteps in (1, 2, 7, 60):
+        print(f"case {case} n={n} bc={bc} ghost={ghost} steps={nsteps}", flush=True)
+        got = run(g, [stored(f, g) for f in (c0, p0, e0)], nsteps, dev)
+        ref = G.steps(m, c0, p0, e0, d[0], d[1], bc, DT, nsteps)[:3]
+        for name, a, r in zip(("c", "phi", "eta"), got, ref):
+            err = np.abs(capi.physical(a, g)[0] - r).max()
+            worst = max(worst, err)
+            assert err <= 1e-10, (name, nsteps, err)
+            # z ghost planes untouched
+            assert (a[:ghost] == 7.0).all() and (a[ghost + 1:] == 7.0).all(), (name, nsteps)
+    print(f"gfa fused case {case}: worst max-abs vs oracle = {worst:.3e}")
+    if n == (48, 40):
+        assert np.abs(ref[2] - e0).max() > 0.3   # the run moved the fields
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_gfa_fused_ghosts_are_the_fill_of_the_previous_state(dev, case):
+    n, d, ghost, bc = CASES[case]
+    g = grid(case)
+    c0, p0, e0 = nucleus(n, 50 + case)
+    for before in (4, 5):      # the last step ends in place (odd total) or in the workspace
+        prev = run(g, [stored(f, g) for f in (c0, p0, e0)], before, dev)
+        nxt = run(g, [a.copy() for a in prev], 1, dev)
+        for name, a, b in zip(("c", "phi", "eta"), prev, nxt):
+            want = ghost_fill(a, g, bc)[ghost]
+            got = b[ghost]
+            mask = np.ones_like(got, dtype=bool)
+            mask[ghost:ghost + n[1], ghost:ghost + n[0]] = False
+            assert np.array_equal(got[mask], want[mask]), (name, before)
+
+
+def test_gfa_fused_timing_kind(dev):
+    g = grid(0)
+    c0, p0, e0 = nucleus(CASES[0][0], 60)
+    capi.call("phx_ch_timing", 1)
+    try:
+        run(g, [stored(f, g) for f in (c0, p0, e0)], 5, dev)
+        cnt, ms = C.c_int(), C.c_double()
+        capi.call("phx_ch_timing_read", 10, C.byref(cnt), C.byref(ms))
+        assert cnt.value == 5 and ms.value > 0.0
+    finally:
+        capi.call("phx_ch_timing", 0)

@@ -116,11 +116,11 @@ def test_timing_record_without_launches():
     reads zero launches, and kinds outside 0-9 are rejected."""
     lib = capi.lib()
     capi.call("phx_ch_timing", 1)
-    for kind in range(10):
+    for kind in range(11):
         n, ms = C.c_int(-1), C.c_double(-1.0)
         capi.call("phx_ch_timing_read", kind, C.byref(n), C.byref(ms))
         assert n.value == 0 and ms.value == 0.0
-    assert lib.phx_ch_timing_read(10, None, None) == capi.PHX_ERR_INVALID
+    assert lib.phx_ch_timing_read(11, None, None) == capi.PHX_ERR_INVALID
     assert lib.phx_ch_timing_read(-1, None, None) == capi.PHX_ERR_INVALID
     capi.call("phx_ch_timing", 0)
 

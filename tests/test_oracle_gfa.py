@@ -199,3 +199,31 @@ def test_model_header_matches_oracle_on_host():
         assert abs(got - want) <= 1e-13 * scale, (name, args, got, want)
         seen.add(name)
     assert seen == set(fns)
+
+
+def test_fused_entry_point_rejects_bad_arguments():
+    """phx_gfa_euler_steps checks its arguments before any device work:
+    3D grids, FIXED faces, ghost layers beyond the axis, null pointers, a
+    non-finite constant, negative nsteps and a missing workspace are
+    PHX_ERR_INVALID; nsteps = 0 returns PHX_OK without reading the fields."""
+    import ctypes as C
+    lib = capi.lib()
+    p = capi.GfaParams(dt=1e-7, **FEB)
+    g = capi.make_grid((8, 6), d=(1e-5, 1e-5), bc=capi.NOFLUX)
+    x = C.c_void_p(16)   # never dereferenced: every call below fails or returns first
+    inv = capi.PHX_ERR_INVALID
+
+    def call(grid, *ptrs, params=p, nsteps=1):
+        return lib.phx_gfa_euler_steps(C.byref(grid), *ptrs, C.byref(params), nsteps, None)
+
+    assert lib.phx_gfa_workspace_bytes(C.byref(g)) == 3 * 8 * (10 * 8 * 3)
+    assert call(capi.make_grid((8, 6, 4), bc=capi.NOFLUX), x, x, x, x) == inv
+    assert call(capi.make_grid((8, 6), bc=[[capi.FIXED] * 2, [capi.NOFLUX] * 2]), x, x, x, x) == inv
+    assert call(capi.make_grid((8, 2), ghost=2, bc=[[capi.NOFLUX] * 2, [capi.PERIODIC] * 2]), x, x, x, x) != capi.PHX_OK
+    assert call(g, None, x, x, x) == inv
+    assert call(g, x, x, x, None) == inv
+    assert call(g, x, x, x, x, nsteps=-1) == inv
+    bad = capi.GfaParams(dt=1e-7, **dict(FEB, T=float("nan")))
+    assert call(g, x, x, x, x, params=bad) == inv
+    assert "non-finite" in capi.lib().phx_last_error().decode()
+    assert call(g, x, x, x, None, nsteps=0) == capi.PHX_OK
