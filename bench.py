@@ -18,9 +18,10 @@ Other workloads (--workload), each a configuration of the same path:
 Prints ONE JSON line. `roofline` is computed for the dominant kernel from its
 algorithmic bytes per launch divided by its average launch duration, measured
 with HIP events bracketing every launch on the launch stream over a second run
-of the same steps (the events would add ~9% to the timed region itself). `cpu_baseline` times the FP64 CPU oracle
-(a restatement of the reference's per-term CPU path, one thread) on a bounded
-sample of the same workload.
+of the same steps (the events would add ~9% to the timed region itself).
+`cpu_baseline` times the FP64 CPU oracle (a restatement of the reference's
+per-term CPU path) on one thread on a bounded sample of the same workload; the
+default workload adds the same passes on all host cores (`all_cores`).
 """
 from __future__ import annotations
 
@@ -187,6 +188,7 @@ def bench_ch3d(a, with_cpu):
     if with_cpu:
         import numpy as np
         from oracle import oracle as O
+        O.set_threads(1)   # the reference's CPU path is serial
         steps = 10
         cc = init.to_stored(init.ch_field((NX,) * 3, 2, 0.4, 0.02), g)
         w = np.zeros((6,) + cc.shape)
@@ -196,6 +198,14 @@ def bench_ch3d(a, with_cpu):
         cpu = {"value": cells * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
                "sample": f"{steps} RK4 steps of the same 256^3 problem, FP64, reference-structured per-term passes "
                          f"(oracle/phx_oracle.c orc_ch_rk4_steps), {dt:.1f} s on {cpu_model()}"}
+        # the same passes on every host core (OpenMP over k, j; same results)
+        nt = O.set_threads(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+        cc = init.to_stored(init.ch_field((NX,) * 3, 2, 0.4, 0.02), g)
+        t0 = time.perf_counter()
+        O.ch_rk4_steps(g, p, cc, steps, w)
+        dta = time.perf_counter() - t0
+        O.set_threads(1)
+        cpu["all_cores"] = {"value": cells * steps / dta, "cores": nt, "sample": f"the same {steps} steps, {dta:.1f} s"}
     cfg = {"workload": "ch3d_rk4_256cube_noflux", "grid": [NX] * 3, "bc": "noflux", "scheme": "rk4", "dt": 0.005,
            "kappa": 1.0, "M": 1.0, "f": "c^2(1-c)^2", "bulk": a.dtype, "parallelism": "single-gpu",
            "zchunk": zc.value, "tile": TILE_NAMES[tile.value]}
@@ -250,6 +260,7 @@ def bench_ch2d(a, with_cpu):
     cpu = None
     if with_cpu:
         from oracle import oracle as O
+        O.set_threads(1)   # the reference's CPU path is serial
         steps = 20
         cc = init.to_stored(init.ch_field((NX, NX), 2, 0.4, 0.02), g)
         t0 = time.perf_counter()
@@ -303,6 +314,7 @@ def bench_ac(a, with_cpu):
     cpu = None
     if with_cpu:
         from oracle import oracle as O
+        O.set_threads(1)   # the reference's CPU path is serial
         steps = 1
         es = [init.to_stored(x[f], g) for f in range(NF)]
         t0 = time.perf_counter()
@@ -365,6 +377,7 @@ def bench_ens(a, with_cpu):
     cpu = None
     if with_cpu:
         from oracle import oracle as O
+        O.set_threads(1)   # the reference's CPU path is serial
         csteps, runs = 20, 16
         done, off = 0, 0
         t0 = time.perf_counter()
@@ -415,6 +428,7 @@ def bench_euler(a, with_cpu):
     cpu = None
     if with_cpu:
         from oracle import oracle as O
+        O.set_threads(1)   # the reference's CPU path is serial
         cc = c0.copy()
         t0 = time.perf_counter()
         O.ch_euler_steps(g, p, cc, np.zeros_like(cc), steps)

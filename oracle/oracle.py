@@ -38,6 +38,7 @@ _SIG = {
     "orc_mass_energy": (None, [GP, C.POINTER(CHParams), D, D]),
     "orc_substream": (C.c_uint64, [C.c_uint64, C.c_uint64]),
     "orc_uniform": (None, [C.c_uint64, C.c_uint64, C.c_size_t, C.c_double, C.c_double, D]),
+    "orc_set_threads": (C.c_int, [C.c_int]),
     "orc_conv3x3_fwd": (None, [C.c_int] * 5 + [D, D, D, D]),
     "orc_conv3x3_dgrad": (None, [C.c_int] * 5 + [D, D, D]),
     "orc_conv3x3_wgrad": (None, [C.c_int] * 5 + [D, D, D, D]),
@@ -131,6 +132,12 @@ def mass_energy(g, p, c):
 
 def energy_density(g, p, c, i, j, k):
     return lib().orc_energy_density(C.byref(g), C.byref(p), _p(c), i, j, k)
+
+
+def set_threads(n: int) -> int:
+    """Threads of the per-cell passes (1 = serial, as the reference's CPU
+    path); returns the count now in effect. Results do not depend on it."""
+    return lib().orc_set_threads(int(n))
 
 
 def uniform(seed, n, a, b, offset=0):

@@ -32,9 +32,33 @@
 #include "../include/phix_amd.h"
 
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+
+/* The per-cell passes run on OpenMP threads (each cell writes only its own
+ * output, so the results do not depend on the thread count); orc_set_threads
+ * fixes the count (1 = the reference's serial loops). Reductions stay serial. */
+#ifdef _OPENMP
+#define PAR_KJ _Pragma("omp parallel for collapse(2) schedule(static)")
+#define PAR_Q _Pragma("omp parallel for schedule(static)")
+#else
+#define PAR_KJ
+#define PAR_Q
+#endif
+
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
 
 /* ----------------------------------------------------------------------- */
 /* layout                                                                  */
@@ -118,6 +142,7 @@ void orc_lap_accumulate(const phx_grid* G, double* rhs, const double* src, doubl
     double idx2 = 1.0 / (G->d[0] * G->d[0]);
     double idy2 = (dim >= 2) ? 1.0 / (G->d[1] * G->d[1]) : 0.0;
     double idz2 = (dim >= 3) ? 1.0 / (G->d[2] * G->d[2]) : 0.0;
+    PAR_KJ
     for (int k = 0; k < L.n[2]; ++k)
         for (int j = 0; j < L.n[1]; ++j)
             for (int i = 0; i < L.n[0]; ++i) {
@@ -134,6 +159,7 @@ void orc_grad_accumulate(const phx_grid* G, double* rhs, const double* src, int 
     lay_t L = mklay(G);
     double inv_2d = 0.5 / G->d[axis];
     long st = axis == 0 ? 1 : (axis == 1 ? L.sx : L.sxy);
+    PAR_KJ
     for (int k = 0; k < L.n[2]; ++k)
         for (int j = 0; j < L.n[1]; ++j)
             for (int i = 0; i < L.n[0]; ++i) {
@@ -145,6 +171,7 @@ void orc_grad_accumulate(const phx_grid* G, double* rhs, const double* src, int 
 /* rhs += coeff*s1*s2: Equation.cu:410-420 */
 void orc_mul_accumulate(const phx_grid* G, double* rhs, const double* s1, const double* s2, double coeff) {
     lay_t L = mklay(G);
+    PAR_KJ
     for (int k = 0; k < L.n[2]; ++k)
         for (int j = 0; j < L.n[1]; ++j)
             for (int i = 0; i < L.n[0]; ++i) {
@@ -163,6 +190,7 @@ double orc_dw(double c, double rho, double ca, double cb) {
 void orc_pw_dw_accumulate(const phx_grid* G, double* rhs, const double* src,
                           double rho, double ca, double cb, double coeff) {
     lay_t L = mklay(G);
+    PAR_KJ
     for (int k = 0; k < L.n[2]; ++k)
         for (int j = 0; j < L.n[1]; ++j)
             for (int i = 0; i < L.n[0]; ++i) {
@@ -206,6 +234,7 @@ void orc_ch_euler_steps(const phx_grid* G, const phx_ch_params* p, double* c, do
         orc_bc_fill(G, mu);
         memset(scratch, 0, sizeof(double) * L.total);
         orc_lap_accumulate(G, scratch, mu, p->M);
+        PAR_Q
         for (long q = 0; q < L.total; ++q) c[q] += p->dt * scratch[q];
     }
 }
@@ -225,15 +254,19 @@ void orc_ch_rk4_steps(const phx_grid* G, const phx_ch_params* p, double* phi, do
     for (int s = 0; s < nsteps; ++s) {
         orc_bc_fill(G, phi);
         orc_ch_rhs(G, p, phi, mu, k1);
+        PAR_Q
         for (long q = 0; q < N; ++q) tmp[q] = phi[q] + dt2 * k1[q];
         orc_bc_fill(G, tmp);
         orc_ch_rhs(G, p, tmp, mu, k2);
+        PAR_Q
         for (long q = 0; q < N; ++q) tmp[q] = phi[q] + dt2 * k2[q];
         orc_bc_fill(G, tmp);
         orc_ch_rhs(G, p, tmp, mu, k3);
+        PAR_Q
         for (long q = 0; q < N; ++q) tmp[q] = phi[q] + dt * k3[q];
         orc_bc_fill(G, tmp);
         orc_ch_rhs(G, p, tmp, mu, k4);
+        PAR_Q
         for (long q = 0; q < N; ++q) phi[q] += dt6 * (k1[q] + 2.0 * k2[q] + 2.0 * k3[q] + k4[q]);
     }
 }
