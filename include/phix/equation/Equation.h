@@ -29,6 +29,11 @@ public:
     // The expression as set (read by the Solver planner; not in the reference).
     const RHSExpr& rhsExpr() const { return rhs_expr_; }
 
+    // computeRHS without the final host wait: the work is enqueued in order on
+    // the null stream (used by the Solver's steps, which wait once per call;
+    // not in the reference).
+    void enqueueRHS(ScalarField& rhs) const;
+
 private:
     RHSExpr rhs_expr_;
     mutable ScratchPool scratch_pool_;

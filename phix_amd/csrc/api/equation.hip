@@ -300,15 +300,22 @@ void Equation::setRHS(const RHSExpr& e) { rhs_expr_ = e; }
 void Equation::setRHS(const Term& t) { rhs_expr_ = RHSExpr(t); }
 
 void Equation::computeRHS(ScalarField& rhs) const {
+    enqueueRHS(rhs);
+    detail::hipCheck(hipDeviceSynchronize(), "Equation::computeRHS");
+}
+
+void Equation::enqueueRHS(ScalarField& rhs) const {
     if (!rhs.deviceAllocated()) throw std::runtime_error("Equation::computeRHS: rhs device memory not allocated");
     if (rhs_expr_.terms.empty()) throw std::runtime_error("Equation::computeRHS: RHS not set (call setRHS first)");
-    detail::hipCheck(hipMemset(rhs.d_curr, 0, rhs.storedSize * sizeof(double)), "Equation::computeRHS memset");
+    detail::hipCheck(hipMemsetAsync(rhs.d_curr, 0, rhs.storedSize * sizeof(double), nullptr),
+                     "Equation::computeRHS memset");
     scratch_pool_.reset();
     for (const auto& t : rhs_expr_.terms) {
         if (!t.gpu_launcher) throw std::runtime_error("Equation::computeRHS: a Term has no GPU launcher.");
         t.gpu_launcher(rhs.d_curr, t.coeff, scratch_pool_);
     }
-    detail::hipCheck(hipDeviceSynchronize(), "Equation::computeRHS");
+    // Launch errors surface here, faults at the caller's next wait.
+    detail::hipCheck(hipGetLastError(), "Equation::computeRHS");
 }
 
 void Equation::computeRHSCPU(ScalarField& rhs) const {

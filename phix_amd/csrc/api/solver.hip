@@ -187,9 +187,8 @@ static void applyBCs(const std::vector<BoundaryCondition*>& bcs, ScalarField& f)
 
 static void genericEuler(Equation& eq, const std::vector<BoundaryCondition*>& bcs, ScalarField& rhs, double dt) {
     applyBCs(bcs, eq.unknown);
-    eq.computeRHS(rhs);
+    eq.enqueueRHS(rhs);
     detail::throwPhx(phx_axpy_f64(eq.unknown.d_curr, rhs.d_curr, dt, eq.unknown.storedSize, nullptr), "Euler update");
-    hipOk(hipDeviceSynchronize(), "Solver::advance");
     eq.unknown.advanceTimeLevelGPU();
 }
 
@@ -199,14 +198,14 @@ static void genericRK4(Equation& eq, const std::vector<BoundaryCondition*>& bcs,
     const std::size_t n = phi.storedSize;
     const double shift[3] = {dt * 0.5, dt * 0.5, dt};
     applyBCs(bcs, phi);
-    eq.computeRHS(*k[0]);
+    eq.enqueueRHS(*k[0]);
     for (int s = 1; s < 4; ++s) {
         detail::throwPhx(phx_rk4_tmp_f64(tmp.d_curr, phi.d_curr, k[s - 1]->d_curr, shift[s - 1], n, nullptr), "RK4 tmp");
         applyBCs(bcs, tmp);
         double* saved = phi.d_curr;
         phi.d_curr = tmp.d_curr;
         try {
-            eq.computeRHS(*k[s]);
+            eq.enqueueRHS(*k[s]);
         } catch (...) {
             phi.d_curr = saved;
             throw;
@@ -216,7 +215,6 @@ static void genericRK4(Equation& eq, const std::vector<BoundaryCondition*>& bcs,
     detail::throwPhx(phx_rk4_update_f64(phi.d_curr, k[0]->d_curr, k[1]->d_curr, k[2]->d_curr, k[3]->d_curr, dt / 6.0, n,
                                         nullptr),
                      "RK4 update");
-    hipOk(hipDeviceSynchronize(), "Solver::advance");
     phi.advanceTimeLevelGPU();
 }
 
@@ -236,18 +234,18 @@ void Solver::advanceN(int n) {
                     auto& s = steps_[q];
                     applyBCs(s.bcs, *s.sourceField);
                     if (s.type == EquationType::STEADY) {
-                        s.equation->computeRHS(s.equation->unknown);
+                        s.equation->enqueueRHS(s.equation->unknown);
                     } else {
-                        s.equation->computeRHS(*I.stepScratch[q]);
+                        s.equation->enqueueRHS(*I.stepScratch[q]);
                         detail::throwPhx(phx_axpy_f64(s.equation->unknown.d_curr, I.stepScratch[q]->d_curr, dt,
                                                       s.equation->unknown.storedSize, nullptr),
                                          "multi-step update");
-                        hipOk(hipDeviceSynchronize(), "Solver::advance");
                     }
                 }
                 for (auto& s : steps_)
                     if (s.type == EquationType::TRANSIENT) s.equation->unknown.advanceTimeLevelGPU();
             }
+            hipOk(hipDeviceSynchronize(), "Solver::advance");
             return;
         }
         if (!I.rhs) {
@@ -268,6 +266,9 @@ void Solver::advanceN(int n) {
                 genericEuler(equation_, bcs_, *I.rhs, dt);
             }
         }
+        // the steps run in order on the null stream; one host wait per call
+        // (the reference waits after every update)
+        hipOk(hipDeviceSynchronize(), "Solver::advance");
         return;
     }
 
