@@ -498,7 +498,7 @@ def bench_gfa(a, with_cpu):
                          f"on {cpu_model()}"}
     cfg = {"workload": "gfa2d_feb_ex1_600sq_noflux", "grid": [NX, NX], "dx": DX, "bc": "noflux",
            "scheme": "euler, six multi-steps per step", "dt": 1.0e-6, "T": 1000.0, "bulk": "f64",
-           "kernel": "k_gfa_step 32x8", "parallelism": "single-gpu",
+           "kernel": "k_gfa_step 32x16", "parallelism": "single-gpu",
            "note": "three 2.9 MB fields stay cache resident between steps"}
     return line("cell-updates/s, Fe-B glass-formation (GFA_2D) Euler step", cells * steps / (ms / 1e3),
                 "cell-updates/s", steps, a.warmup, ms, "f64",

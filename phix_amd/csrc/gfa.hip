@@ -27,7 +27,7 @@
 namespace phx {
 namespace {
 
-constexpr int GTX = 32, GTY = 8, GNT = GTX * GTY;   // tile and threads
+constexpr int GTX = 32, GTY = 16, GNT = GTX * GTY;  // tile and threads
 constexpr int GHX = GTX + 2, GHY = GTY + 2;         // tile plus one-cell ring
 
 // Constants of one run: the settings' values and what depends only on T
