@@ -359,11 +359,11 @@ class HybridSolver:
             if not self.packed:
                 self.net.forward(cur)   # packs the BF16 weights once
                 self.packed = True
-            # prev: input of the block's last jump (kept for the judge)
+            # prev: input of the block's last jump (kept for the solver check)
             prev = self.net.rollout(cur, k - 1, self.bufs).clone() if k > 1 else cur.clone()
             cur = self.net.forward_to(prev, self.bufs[2])
             ssteps += k
-            # judge: one surrogate step against `every` solver steps from its input
+            # solver check: one surrogate step against `every` solver steps from its input
             S.set(prev[0])
             S.steps(self.every)
             vsteps += self.every
