@@ -76,10 +76,11 @@ __device__ __forceinline__ double gfa_fL(const GfaConst& k, double c) {
 }
 
 // Source cell of ghost layer l (1-based) beyond the low (hi = false) or high
-// face of an axis of n cells, for the periodic and zero-flux fills.
+// face of an axis of n cells: periodic wraps, zero-flux copies the face cell
+// into every layer (src/boundary/Boundary.cu:276-329).
 __device__ __forceinline__ int ghost_src(int l, bool hi, int n, int kind) {
     if (kind == PHX_BC_PERIODIC) return hi ? l - 1 : n - l;
-    return hi ? n - l : l - 1;
+    return hi ? n - 1 : 0;
 }
 
 __global__ __launch_bounds__(GNT) void k_gfa_step(GfaArgs A) {
