@@ -111,7 +111,11 @@ int phx_bc_apply(const phx_grid* grid, int dtype, void* data,
 /* Applies grid->bc of the active axes in the order X, Y, Z, in place -- the
  * order of a {bc_x, bc_y, bc_z} list handed to the reference Solver
  * (Solver.cu:149-151). A periodic axis is one PeriodicBC object, any other
- * axis one object per face; faces set to PHX_BC_NONE are left alone. */
+ * axis one object per face; faces set to PHX_BC_NONE are left alone. When
+ * every active face has a condition and ghost <= n on every active axis, all
+ * faces are filled in one launch with the same values (each ghost cell takes
+ * what the face fills leave, corners included); otherwise one launch per
+ * face. */
 int phx_bc_fill(const phx_grid* grid, int dtype, void* data, phx_stream stream);
 
 /* ------------------------------------------ generic DSL term kernels ---- */

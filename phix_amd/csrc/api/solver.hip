@@ -181,7 +181,15 @@ void Solver::setBulkPrecision(Precision p) {
 }
 
 // ------------------------------------------------------------- generic ----
+// The list applied object by object, or -- when it is a table of standard
+// conditions in axis order (what bcTable accepts) -- as one phx_bc_fill,
+// which leaves the same values in one launch.
 static void applyBCs(const std::vector<BoundaryCondition*>& bcs, ScalarField& f) {
+    phx_grid g = detail::gridOf(f);
+    if (!bcs.empty() && detail::bcTable(bcs, f.mesh, g)) {
+        detail::throwPhx(phx_bc_fill(&g, PHX_F64, f.d_curr, nullptr), "Solver: BCs");
+        return;
+    }
     for (auto* bc : bcs) bc->applyOnGPU(f);
 }
 

@@ -116,7 +116,104 @@ static int bc_apply_t(const phx_grid* G, T* data, int axis, int kind, int side, 
     return PHX_OK;
 }
 
+// All faces in one launch: every ghost cell of the active axes gets the value
+// the face fills leave when applied axis by axis (x, then y, then z), each
+// over the full stored extent of the other axes. Walking the active axes from
+// the last to the first, the first one on which the cell is a ghost decides:
+// a FIXED face gives its value; otherwise the coordinate moves to its source
+// (periodic: wrap; zero-flux: the face cell) and the earlier axes are looked
+// at in turn. Sources have every active coordinate physical, and no such cell
+// is written here, so the one pass equals the sequence. Threads enumerate the
+// ghost cells once: the x slabs over the full (y, z) extent, the y slabs over
+// physical x, the z slabs over physical x and y. Needs ghost <= n on every
+// active axis and a BC on every active face.
+struct FillGeom {
+    Layout L;
+    int kind[3][2];
+    double value[3][2];
+    long long nx_slab, ny_slab, nz_slab;   // ghost cells in each group
+};
+
+template <typename T>
+__global__ void k_bc_fill_all(T* __restrict__ data, FillGeom F) {
+    const Layout& L = F.L;
+    const int g = L.g, g2 = 2 * g;
+    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int c[3];
+    if (q < F.nx_slab) {                     // x ghosts: (2g) x sy x sz
+        const int gi = (int)(q % g2);
+        const long long r = q / g2;
+        c[0] = gi < g ? gi - g : L.n0 + gi - g;
+        c[1] = (int)(r % L.sy) - g;
+        c[2] = (int)(r / L.sy) - g;
+    } else if ((q -= F.nx_slab) < F.ny_slab) {   // y ghosts: n0 x (2g) x sz
+        c[0] = (int)(q % L.n0);
+        const long long r = q / L.n0;
+        const int gj = (int)(r % g2);
+        c[1] = gj < g ? gj - g : L.n1 + gj - g;
+        c[2] = (int)(r / g2) - g;
+    } else if ((q -= F.ny_slab) < F.nz_slab) {   // z ghosts: n0 x n1 x (2g)
+        c[0] = (int)(q % L.n0);
+        const long long r = q / L.n0;
+        c[1] = (int)(r % L.n1);
+        const int gk = (int)(r / L.n1);
+        c[2] = gk < g ? gk - g : L.n2 + gk - g;
+    } else {
+        return;
+    }
+    const long long dst = L.at(c[0], c[1], c[2]);
+    const int n[3] = {L.n0, L.n1, L.n2};
+    for (int a = L.dim - 1; a >= 0; --a) {
+        const bool lo = c[a] < 0, hi = c[a] >= n[a];
+        if (!lo && !hi) continue;
+        const int s = hi ? 1 : 0;
+        const int k = F.kind[a][s];
+        if (k == PHX_BC_FIXED) {
+            data[dst] = (T)F.value[a][s];
+            return;
+        }
+        if (k == PHX_BC_PERIODIC) c[a] += lo ? n[a] : -n[a];
+        else c[a] = lo ? 0 : n[a] - 1;
+    }
+    data[dst] = data[L.at(c[0], c[1], c[2])];
+}
+
+static bool fill_all_ok(const phx_grid* G) {
+    for (int a = 0; a < G->dim; ++a) {
+        if (G->ghost > G->n[a]) return false;
+        for (int s = 0; s < 2; ++s)
+            if (G->bc[a][s] != PHX_BC_PERIODIC && G->bc[a][s] != PHX_BC_NOFLUX && G->bc[a][s] != PHX_BC_FIXED)
+                return false;
+        if ((G->bc[a][0] == PHX_BC_PERIODIC) != (G->bc[a][1] == PHX_BC_PERIODIC)) return false;
+    }
+    return true;
+}
+
+template <typename T>
+static int bc_fill_all_t(const phx_grid* G, T* data, hipStream_t st) {
+    FillGeom F{};
+    F.L = make_layout(G);
+    for (int a = 0; a < 3; ++a)
+        for (int s = 0; s < 2; ++s) {
+            F.kind[a][s] = G->bc[a][s];
+            F.value[a][s] = G->bc_value[a][s];
+        }
+    const Layout& L = F.L;
+    const long long g2 = 2LL * L.g;
+    F.nx_slab = g2 * L.sy * L.sz;
+    F.ny_slab = G->dim >= 2 ? (long long)L.n0 * g2 * L.sz : 0;
+    F.nz_slab = G->dim >= 3 ? (long long)L.n0 * L.n1 * g2 : 0;
+    const long long total = F.nx_slab + F.ny_slab + F.nz_slab;
+    const long long blocks = (total + 255) / 256;
+    if (blocks > 0x7fffffffLL) return fail(PHX_ERR_UNSUPPORTED, "phx_bc_fill: field too large");
+    hipLaunchKernelGGL(k_bc_fill_all<T>, dim3((unsigned)blocks), dim3(256), 0, st, data, F);
+    PHX_LAUNCH_CHECK("phx_bc_fill");
+    return PHX_OK;
+}
+
 int bc_fill_impl(const phx_grid* G, int dtype, void* data, hipStream_t st) {
+    if (fill_all_ok(G))
+        return dtype == PHX_F64 ? bc_fill_all_t(G, (double*)data, st) : bc_fill_all_t(G, (float*)data, st);
     for (int a = 0; a < G->dim; ++a) {
         int klo = G->bc[a][0], khi = G->bc[a][1];
         int rc = 0;
