@@ -181,6 +181,13 @@ void Solver::setBulkPrecision(Precision p) {
 }
 
 // ------------------------------------------------------------- generic ----
+// prev <- curr, enqueued on the null stream (advanceTimeLevelGPU without its
+// host wait; the Solver waits once per call).
+static void advanceLevel(ScalarField& u) {
+    hipOk(hipMemcpyAsync(u.d_prev, u.d_curr, u.storedSize * sizeof(double), hipMemcpyDeviceToDevice, nullptr),
+          "Solver: time level");
+}
+
 // The list applied object by object, or -- when it is a table of standard
 // conditions in axis order (what bcTable accepts) -- as one phx_bc_fill,
 // which leaves the same values in one launch.
@@ -197,7 +204,7 @@ static void genericEuler(Equation& eq, const std::vector<BoundaryCondition*>& bc
     applyBCs(bcs, eq.unknown);
     eq.enqueueRHS(rhs);
     detail::throwPhx(phx_axpy_f64(eq.unknown.d_curr, rhs.d_curr, dt, eq.unknown.storedSize, nullptr), "Euler update");
-    eq.unknown.advanceTimeLevelGPU();
+    advanceLevel(eq.unknown);
 }
 
 static void genericRK4(Equation& eq, const std::vector<BoundaryCondition*>& bcs, ScalarField* k[4], ScalarField& tmp,
@@ -223,7 +230,7 @@ static void genericRK4(Equation& eq, const std::vector<BoundaryCondition*>& bcs,
     detail::throwPhx(phx_rk4_update_f64(phi.d_curr, k[0]->d_curr, k[1]->d_curr, k[2]->d_curr, k[3]->d_curr, dt / 6.0, n,
                                         nullptr),
                      "RK4 update");
-    phi.advanceTimeLevelGPU();
+    advanceLevel(phi);
 }
 
 // -------------------------------------------------------------- advance ----
@@ -251,7 +258,7 @@ void Solver::advanceN(int n) {
                     }
                 }
                 for (auto& s : steps_)
-                    if (s.type == EquationType::TRANSIENT) s.equation->unknown.advanceTimeLevelGPU();
+                    if (s.type == EquationType::TRANSIENT) advanceLevel(s.equation->unknown);
             }
             hipOk(hipDeviceSynchronize(), "Solver::advance");
             return;
