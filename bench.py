@@ -2,32 +2,41 @@
 
 Default workload (the headline configuration that fits one GPU): 3D
 Cahn-Hilliard, 256^3 cells, dx=1, zero-flux BCs on all faces,
-f = c^2(1-c)^2, kappa=1, M=1, dt=0.005, classic RK4, FP32 stencil bulk;
-c0 = 0.4 + U(-0.02, 0.02) from SplitMix64 seed 2. One step = one RK4 time
-step over the whole grid, with the launch (one-stage or two-stage kernel,
-tile, z-chunk) chosen by the library's autotuner unless --tile/--zc are given.
-Inputs are resident in HBM before the timed region.
+f = c^2(1-c)^2, kappa=1, M=1, dt=0.005, classic RK4, FP64 (the reference's
+precision); c0 = 0.4 + U(-0.02, 0.02) from SplitMix64 seed 2. One step = one
+RK4 time step over the whole grid with the library's default launch for this
+shape (--autotune runs the launch autotuner first; --tile/--zc fix a launch).
+Inputs are resident in HBM before the timed region. The same workload in FP32
+bulk is reported beside it as `companion` (--no-companion skips it).
+
+Timing: W warm-up steps, then --reps windows (default 5) of exactly K steps,
+each bracketed by device synchronisation; `value` and `ms_per_step` come from
+the median window, every window is listed under `timing`.
 
 Other workloads (--workload), each a configuration of the same path:
   euler128  128^2 periodic CH, multi-step Euler, FP64, kappa=0.5, dt=0.01
-  ch2d1024  1024^2 periodic CH RK4, FP32 (same physics as the default)
+  ch2d1024  1024^2 periodic CH RK4, FP64 line + FP32 companion (same physics)
   ac1024    1024^2 periodic Allen-Cahn, 8 order parameters, RK4, FP32
   ens512    512 periodic CH runs, sides 64 + 32*floor(15u), RK4, FP32,
             one persistent launch per stage, hipGraph-captured step loop
+  gfa600    the Fe-B glass-formation program's 600^2 case, FP64
+  copycal   plain 2 GiB device copy (calibrates the PMC byte counters)
 
 Prints ONE JSON line. `roofline` is computed for the dominant kernel from its
-algorithmic bytes per launch divided by its average launch duration, measured
-with HIP events bracketing every launch on the launch stream over a second run
-of the same steps (the events would add ~9% to the timed region itself).
-`cpu_baseline` times the FP64 CPU oracle (a restatement of the reference's
-per-term CPU path) on one thread on a bounded sample of the same workload; the
-default workload adds the same passes on all host cores (`all_cores`).
+algorithmic bytes per launch divided by its average launch duration inside
+the median timed window: the same steps run again with HIP events bracketing
+every launch on the launch stream give each launch's share, scaled so that
+launches x average never exceed the un-instrumented window (the events
+themselves add ~9%). `cpu_baseline` times the FP64 CPU oracle (a restatement
+of the reference's per-term CPU path) on one thread on a bounded sample of the
+same workload, and the same passes on all host cores (`all_cores`).
 """
 from __future__ import annotations
 
 import argparse
 import ctypes as C
 import json
+import math
 import os
 import platform
 import sys
@@ -43,6 +52,8 @@ HBM_PEAK_GBS = 8000.0
 TILE_NAMES = ["64x16", "64x8", "128x8", "32x16", "64x4", "two-stage 64x8", "two-stage 32x16", "two-stage 32x8",
               "two-stage 64x4", "zreg 64x16/256", "zreg 128x8/256", "zreg 128x16/512", "zreg 256x8/512",
               "zreg 256x16/1024", "zreg 256x4/256", "zreg 256x8/1024", "zreg 128x16/1024", "zreg 128x8/512"]
+# 2D launch tile indices: the one-stage kernel, then the whole-step kernel tiles
+TILE2_NAMES = ["64x16 one-stage", "whole-step 64x16", "whole-step 64x32", "whole-step 128x16", "whole-step 32x32"]
 
 
 def cpu_model() -> str:
@@ -85,6 +96,14 @@ def timed(fn, stream):
     return ev0.elapsed_time(ev1)
 
 
+def timed_reps(run, steps, reps, stream):
+    """`reps` timed windows of exactly `steps` steps each (synchronised on both
+    sides); returns (median window ms, all window ms)."""
+    import statistics
+    ms = [timed(lambda: run(steps), stream) for _ in range(reps)]
+    return statistics.median(ms), [round(v, 4) for v in ms]
+
+
 def kernel_times(run, steps):
     """Per-kind launch durations of the CH RK4 kernels: the same `steps` steps
     as the timed region, run again with every launch bracketed by HIP events on
@@ -103,6 +122,16 @@ def kernel_times(run, steps):
     return kinds
 
 
+def launch_shares(kinds, window_ms):
+    """Per-kind average launch duration inside the un-instrumented timed
+    window: the event-bracketed durations scaled by window / their sum (never
+    up), so that launches x average <= the window. Returns {kind: (count,
+    avg_ms, event_avg_ms)}."""
+    tot = sum(t for _, t in kinds.values())
+    f = min(1.0, window_ms / tot) if tot > 0 else 1.0
+    return {k: (n, t * f / n, t / n) for k, (n, t) in kinds.items()}
+
+
 # Algorithmic bytes follow one fixed accounting per workload: the bytes the
 # classic stage-fused RK4 moves per unit of work with mu kept on chip, 16
 # field passes per step (read c_n; S1 writes u2 and acc; S2, S3 read u, c_n,
@@ -111,7 +140,7 @@ def kernel_times(run, steps):
 # credited with the units of work it completes (a one-stage launch a quarter
 # step, a two-stage launch half a step, a whole-step launch one step), so
 # kernels that move fewer bytes for the same work (temporal blocking, the 3D
-# FP32 stage-state form) can exceed the one-stage bound. The kernel's own
+# stage-state form) can exceed the one-stage bound. The kernel's own
 # bytes and the HBM traffic rocprofv3 measured are reported beside it.
 def roofline(bytes_per_launch, launch_s, kernel, traffic_name, launch=None, kernel_bytes=None):
     ach = bytes_per_launch / launch_s / 1e9
@@ -132,147 +161,156 @@ def line(metric, value, unit, steps, warmup, ms, dtype, data, config, roof, cpu)
             "host": {"nproc": os.cpu_count(), "cpu": cpu_model()}}
 
 
+def fp64_form_name():
+    from phix_amd import capi
+    return "exact (reference order)" if capi.lib().phx_ch_get_fp64_exact() else "fma, stage states"
+
+
 # ------------------------------------------------------------ workloads ----
-def bench_ch3d(a, with_cpu):
+# CH RK4 launch kinds (phx_ch_timing): 0 one-stage, 1/2 two-stage (1,2)/(3,4),
+# 3 2D whole step. Algorithmic passes per launch (accounting above):
+# one-stage 16/4, two-stage 16/2, whole step 16.
+CH_ALG = {0: 16 / 4, 1: 16 / 2, 2: 16 / 2, 3: 16}
+
+
+def ch_rk4_run(a, dtype_name, dim):
+    """Times the fused CH RK4 path of one precision: the 256^3 zero-flux
+    problem (dim 3) or the 1024^2 periodic one (dim 2); returns the workload
+    fields of the JSON line."""
     import torch
     from phix_amd import capi, init
-    NX = 256
     dev = torch.device("cuda:0")
-    dtype = capi.F32 if a.dtype == "f32" else capi.F64
+    dtype = capi.F32 if dtype_name == "f32" else capi.F64
     tdt, esz = (torch.float32, 4) if dtype == capi.F32 else (torch.float64, 8)
-    g = capi.make_grid((NX, NX, NX), bc=capi.NOFLUX)
+    n = (256,) * 3 if dim == 3 else (1024, 1024)
+    bc = capi.NOFLUX if dim == 3 else capi.PERIODIC
+    g = capi.make_grid(n, bc=bc)
     p = capi.CHParams(1.0, 0.0, 1.0, 1.0, 1.0, 0.005)
-    c = torch.from_numpy(init.to_stored(init.ch_field((NX,) * 3, 2, 0.4, 0.02), g)).to(tdt).to(dev)
+    c = torch.from_numpy(init.to_stored(init.ch_field(n, 2, 0.4, 0.02), g)).to(tdt).to(dev)
     work = torch.empty(capi.lib().phx_ch_rk4_workspace_bytes(C.byref(g), dtype) // esz, dtype=tdt, device=dev)
     stream = torch.cuda.current_stream()
     sh = capi.stream_handle(stream)
     zc, tune_ms = C.c_int(0), C.c_double(0)
-    if a.tile is None and a.zc is None:
+    if a.autotune:
         scratch = c.clone()
         capi.call("phx_ch_rk4_autotune", C.byref(g), dtype, capi.ptr(scratch), capi.ptr(work), C.byref(p), 10,
                   C.byref(zc), C.byref(tune_ms), sh)
         del scratch
-    else:
-        if a.tile is not None:
-            capi.call("phx_ch_rk4_set_tile", C.byref(g), dtype, a.tile)
-        if a.zc is not None:
-            capi.call("phx_ch_rk4_set_zchunk", C.byref(g), dtype, a.zc)
+    if a.tile is not None:
+        capi.call("phx_ch_rk4_set_tile", C.byref(g), dtype, a.tile)
+    if a.zc is not None and dim == 3:
+        capi.call("phx_ch_rk4_set_zchunk", C.byref(g), dtype, a.zc)
     tile = C.c_int(0)
     capi.call("phx_ch_rk4_get_launch", C.byref(g), dtype, C.byref(zc), C.byref(tile))
 
-    def run(n):
-        capi.call("phx_ch_rk4_steps", C.byref(g), dtype, capi.ptr(c), capi.ptr(work), C.byref(p), n, sh)
+    def run(k):
+        capi.call("phx_ch_rk4_steps", C.byref(g), dtype, capi.ptr(c), capi.ptr(work), C.byref(p), k, sh)
 
     run(a.warmup)
-    ms = timed(lambda: run(a.steps), stream)
-    kinds = kernel_times(run, a.steps)
-    cells = NX ** 3
-    # algorithmic passes per launch (accounting above): one-stage 16/4,
-    # two-stage 16/2. The kernels' own passes: one-stage 13/4 (FP32 stage-state
-    # form) or 17/4 (FP64: the reference's order re-reads c_n in stage 4);
-    # two-stage (1,2): read c_n, write the stage-3 state and the accumulator
-    # or u2; (3,4): read the stage-3 state, c_n and the accumulator or u2,
-    # write c_{n+1}
-    alg = {0: 16 / 4, 1: 16 / 2, 2: 16 / 2}
-    own = {0: (13 if dtype == capi.F32 else 17) / 4, 1: 3, 2: 4}
-    names = {0: "k_ch_stage_3dz" if tile.value >= 9 else "k_ch_stage_3d", 1: "k_ch_pair_3d (stages 1+2)",
-             2: "k_ch_pair_3d (stages 3+4)"}
-    tags = {0: "ch3d_rk4", 1: "ch3d_pair12", 2: "ch3d_pair34"}
-    dom = max(kinds, key=lambda k: kinds[k][1])
-    roof = roofline(alg[dom] * esz * cells, kinds[dom][1] / 1e3 / kinds[dom][0], names[dom],
-                    f"{tags[dom]}_{a.dtype}_256", TILE_NAMES[tile.value], own[dom] * esz * cells)
-    roof["launches"] = {names[k]: {"count": v[0], "avg_us": round(v[1] * 1e3 / v[0], 2),
-                                   "alg_bytes": alg[k] * esz * cells, "kernel_bytes": own[k] * esz * cells}
+    ms, reps = timed_reps(run, a.steps, a.reps, stream)
+    kinds = launch_shares(kernel_times(run, a.steps), ms)
+    cells = math.prod(n)
+    if dim == 3:
+        # the kernels' own passes: one-stage 13/4 (stage-state form: FP32 and
+        # the default FP64 form) or 17/4 (FP64 exact form: the reference's
+        # k1+2k2+2k3+k4 order re-reads c_n in stage 4); two-stage (1,2): read
+        # c_n, write the stage-3 state and acc/u2; (3,4): read the stage-3
+        # state, c_n and acc/u2, write c_{n+1}
+        exact = dtype == capi.F64 and capi.lib().phx_ch_get_fp64_exact()
+        own = {0: (17 if exact else 13) / 4, 1: 3, 2: 4}
+        names = {0: "k_ch_stage_3dz" if tile.value >= 9 else "k_ch_stage_3d", 1: "k_ch_pair_3d (stages 1+2)",
+                 2: "k_ch_pair_3d (stages 3+4)"}
+        tags = {0: "ch3d_rk4", 1: "ch3d_pair12", 2: "ch3d_pair34"}
+        launch = TILE_NAMES[tile.value]
+        traffic_tag = f"{tags[max(kinds, key=lambda k: kinds[k][0] * kinds[k][1])]}_{dtype_name}_256"
+    else:
+        own = {0: 4, 3: 2}
+        names = {0: "k_ch_stage_2d", 3: "k_ch_step_2d"}
+        launch = TILE2_NAMES[tile.value]
+        dk = max(kinds, key=lambda k: kinds[k][0] * kinds[k][1])
+        traffic_tag = f"{'ch2d_rk4' if dk == 0 else 'ch2d_step'}_{dtype_name}_1024"
+    dom = max(kinds, key=lambda k: kinds[k][0] * kinds[k][1])
+    roof = roofline(CH_ALG[dom] * esz * cells, kinds[dom][1] / 1e3, names[dom], traffic_tag, launch,
+                    own[dom] * esz * cells)
+    roof["avg_launch_method"] = ("launch durations inside the median timed window: HIP-event durations of the "
+                                 "same steps run again, scaled to the window (launches x average <= window)")
+    roof["launches"] = {names[k]: {"count": v[0], "avg_us": round(v[1] * 1e3, 2),
+                                   "event_avg_us": round(v[2] * 1e3, 2),
+                                   "alg_bytes": CH_ALG[k] * esz * cells, "kernel_bytes": own[k] * esz * cells}
                         for k, v in kinds.items()}
-    cpu = None
-    if with_cpu:
-        import numpy as np
-        from oracle import oracle as O
-        O.set_threads(1)   # the reference's CPU path is serial
-        steps = 10
-        cc = init.to_stored(init.ch_field((NX,) * 3, 2, 0.4, 0.02), g)
-        w = np.zeros((6,) + cc.shape)
-        t0 = time.perf_counter()
-        O.ch_rk4_steps(g, p, cc, steps, w)
-        dt = time.perf_counter() - t0
-        cpu = {"value": cells * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-               "sample": f"{steps} RK4 steps of the same 256^3 problem, FP64, reference-structured per-term passes "
-                         f"(oracle/phx_oracle.c orc_ch_rk4_steps), {dt:.1f} s on {cpu_model()}"}
-        # the same passes on every host core (OpenMP over k, j; same results)
-        nt = O.set_threads(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
-        cc = init.to_stored(init.ch_field((NX,) * 3, 2, 0.4, 0.02), g)
-        t0 = time.perf_counter()
-        O.ch_rk4_steps(g, p, cc, steps, w)
-        dta = time.perf_counter() - t0
-        O.set_threads(1)
-        cpu["all_cores"] = {"value": cells * steps / dta, "cores": nt, "sample": f"the same {steps} steps, {dta:.1f} s"}
-    cfg = {"workload": "ch3d_rk4_256cube_noflux", "grid": [NX] * 3, "bc": "noflux", "scheme": "rk4", "dt": 0.005,
-           "kappa": 1.0, "M": 1.0, "f": "c^2(1-c)^2", "bulk": a.dtype, "parallelism": "single-gpu",
-           "zchunk": zc.value, "tile": TILE_NAMES[tile.value]}
-    return line("cell-updates/s, Cahn-Hilliard RK4 (fused stencil, %HBM roofline in 'roofline')",
-                cells * a.steps / (ms / 1e3), "cell-updates/s", a.steps, a.warmup, ms, a.dtype,
-                "synthetic (c0 = 0.4 + U(-0.02, 0.02), SplitMix64 seed 2)", cfg, roof, cpu)
+    form = fp64_form_name() if dtype == capi.F64 else "fma, stage states" if dim == 3 else "fma, accumulator"
+    return {"value": cells * a.steps / (ms / 1e3), "ms": ms, "reps_ms": reps, "roofline": roof,
+            "launch": launch, "zchunk": zc.value, "form": form, "grid": g, "params": p}
+
+
+def ch_cpu_baseline(g, p, n, steps):
+    """The FP64 oracle (a restatement of the reference's per-term CPU path) on
+    one host thread, then on every host core, over `steps` RK4 steps."""
+    import numpy as np
+    from phix_amd import init
+    from oracle import oracle as O
+    cells = math.prod(n)
+    O.set_threads(1)   # the reference's CPU path is serial
+    cc = init.to_stored(init.ch_field(n, 2, 0.4, 0.02), g)
+    w = np.zeros((6,) + cc.shape)
+    t0 = time.perf_counter()
+    O.ch_rk4_steps(g, p, cc, steps, w)
+    dt = time.perf_counter() - t0
+    shape = "x".join(map(str, n))
+    cpu = {"value": cells * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+           "sample": f"{steps} RK4 steps of the same {shape} problem, FP64, reference-structured per-term passes "
+                     f"(oracle/phx_oracle.c orc_ch_rk4_steps), {dt:.1f} s on {cpu_model()}"}
+    # the same passes on every host core (OpenMP over k, j; same results)
+    nt = O.set_threads(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    cc = init.to_stored(init.ch_field(n, 2, 0.4, 0.02), g)
+    t0 = time.perf_counter()
+    O.ch_rk4_steps(g, p, cc, steps, w)
+    dta = time.perf_counter() - t0
+    O.set_threads(1)
+    cpu["all_cores"] = {"value": cells * steps / dta, "cores": nt, "sample": f"the same {steps} steps, {dta:.1f} s"}
+    return cpu
+
+
+def bench_ch_rk4(a, with_cpu, dim):
+    """The line of one precision (a.dtype) and, unless --no-companion, the
+    same workload in the other precision as a companion object."""
+    main = ch_rk4_run(a, a.dtype, dim)
+    n = (256,) * 3 if dim == 3 else (1024, 1024)
+    comp = None
+    if not a.no_companion:
+        other = "f32" if a.dtype == "f64" else "f64"
+        r = ch_rk4_run(a, other, dim)
+        comp = {"dtype": other, "value": round(r["value"], 1), "unit": "cell-updates/s",
+                "ms_per_step": round(r["ms"] / a.steps, 5), "reps_ms": r["reps_ms"], "form": r["form"],
+                "launch": r["launch"], "roofline": r["roofline"]}
+        if dim == 3:
+            comp["zchunk"] = r["zchunk"]
+    cpu = ch_cpu_baseline(main["grid"], main["params"], n, 10 if dim == 3 else 20) if with_cpu else None
+    if dim == 3:
+        cfg = {"workload": "ch3d_rk4_256cube_noflux", "grid": list(n), "bc": "noflux", "scheme": "rk4", "dt": 0.005,
+               "kappa": 1.0, "M": 1.0, "f": "c^2(1-c)^2", "bulk": a.dtype, "form": main["form"],
+               "parallelism": "single-gpu", "zchunk": main["zchunk"], "tile": main["launch"]}
+        metric = "cell-updates/s, Cahn-Hilliard RK4 (fused stencil, %HBM roofline in 'roofline')"
+    else:
+        cfg = {"workload": "ch2d_rk4_1024sq_periodic", "grid": list(n), "bc": "periodic", "scheme": "rk4",
+               "dt": 0.005, "kappa": 1.0, "M": 1.0, "f": "c^2(1-c)^2", "bulk": a.dtype, "form": main["form"],
+               "parallelism": "single-gpu", "tile": main["launch"]}
+        metric = "cell-updates/s, Cahn-Hilliard RK4 2D (fused stencil)"
+    out = line(metric, main["value"], "cell-updates/s", a.steps, a.warmup, main["ms"], a.dtype,
+               "synthetic (c0 = 0.4 + U(-0.02, 0.02), SplitMix64 seed 2)", cfg, main["roofline"], cpu)
+    out["timing"] = {"reps": a.reps, "steps_per_rep": a.steps, "reps_ms": main["reps_ms"],
+                     "value_from": "median window"}
+    if comp is not None:
+        out["companion"] = comp
+    return out
+
+
+def bench_ch3d(a, with_cpu):
+    return bench_ch_rk4(a, with_cpu, 3)
 
 
 def bench_ch2d(a, with_cpu):
-    import torch
-    from phix_amd import capi, init
-    NX = 1024
-    dev = torch.device("cuda:0")
-    g = capi.make_grid((NX, NX), bc=capi.PERIODIC)
-    p = capi.CHParams(1.0, 0.0, 1.0, 1.0, 1.0, 0.005)
-    c = torch.from_numpy(init.to_stored(init.ch_field((NX, NX), 2, 0.4, 0.02), g)).float().to(dev)
-    work = torch.empty(capi.lib().phx_ch_rk4_workspace_bytes(C.byref(g), capi.F32) // 4, dtype=torch.float32,
-                       device=dev)
-    stream = torch.cuda.current_stream()
-    sh = capi.stream_handle(stream)
-    if a.tile is None:
-        scratch = c.clone()
-        zc, tune_ms = C.c_int(0), C.c_double(0)
-        capi.call("phx_ch_rk4_autotune", C.byref(g), capi.F32, capi.ptr(scratch), capi.ptr(work), C.byref(p), 20,
-                  C.byref(zc), C.byref(tune_ms), sh)
-        del scratch
-    else:
-        capi.call("phx_ch_rk4_set_tile", C.byref(g), capi.F32, a.tile)
-    zc, tile = C.c_int(0), C.c_int(0)
-    capi.call("phx_ch_rk4_get_launch", C.byref(g), capi.F32, C.byref(zc), C.byref(tile))
-
-    def run(n):
-        capi.call("phx_ch_rk4_steps", C.byref(g), capi.F32, capi.ptr(c), capi.ptr(work), C.byref(p), n, sh)
-
-    run(a.warmup)
-    ms = timed(lambda: run(a.steps), stream)
-    kinds = kernel_times(run, a.steps)
-    cells = NX * NX
-    # algorithmic passes per launch (accounting above): one-stage 16/4, whole
-    # step 16; the whole-step kernel itself reads c_n and writes c_{n+1}
-    alg = {0: 16 / 4, 3: 16}
-    own = {0: 4, 3: 2}
-    names = {0: "k_ch_stage_2d", 3: "k_ch_step_2d"}
-    dom = max(kinds, key=lambda k: kinds[k][1])
-    tile_name = ["64x16 one-stage", "whole-step 64x16", "whole-step 64x32", "whole-step 128x16",
-                 "whole-step 32x32"][tile.value]
-    roof = roofline(alg[dom] * 4 * cells, kinds[dom][1] / 1e3 / kinds[dom][0], names[dom],
-                    f"{'ch2d_rk4' if dom == 0 else 'ch2d_step'}_f32_1024", tile_name, own[dom] * 4 * cells)
-    roof["launches"] = {names[k]: {"count": v[0], "avg_us": round(v[1] * 1e3 / v[0], 2),
-                                   "alg_bytes": alg[k] * 4 * cells, "kernel_bytes": own[k] * 4 * cells}
-                        for k, v in kinds.items()}
-    cpu = None
-    if with_cpu:
-        from oracle import oracle as O
-        O.set_threads(1)   # the reference's CPU path is serial
-        steps = 20
-        cc = init.to_stored(init.ch_field((NX, NX), 2, 0.4, 0.02), g)
-        t0 = time.perf_counter()
-        O.ch_rk4_steps(g, p, cc, steps)
-        dt = time.perf_counter() - t0
-        cpu = {"value": cells * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-               "sample": f"{steps} RK4 steps of the same 1024^2 problem, FP64 oracle, {dt:.1f} s on {cpu_model()}"}
-    cfg = {"workload": "ch2d_rk4_1024sq_periodic", "grid": [NX, NX], "bc": "periodic", "scheme": "rk4", "dt": 0.005,
-           "kappa": 1.0, "M": 1.0, "bulk": "f32", "parallelism": "single-gpu", "tile": tile_name}
-    return line("cell-updates/s, Cahn-Hilliard RK4 2D (fused stencil)", cells * a.steps / (ms / 1e3),
-                "cell-updates/s", a.steps, a.warmup, ms, "f32",
-                "synthetic (c0 = 0.4 + U(-0.02, 0.02), SplitMix64 seed 2)", cfg, roof, cpu)
+    return bench_ch_rk4(a, with_cpu, 2)
 
 
 def bench_ac(a, with_cpu):
@@ -524,59 +562,8 @@ def bench_copy(a, with_cpu):
                 ms, "f32", "constant buffer", cfg, roof, None)
 
 
-def bench_hbmprobe(a, with_cpu):
-    """Bandwidth that simple access patterns reach on this GPU (phx_hbm_probe):
-    read, write and one-float4-per-thread copy of 1 GiB; the 5 streams of an
-    RK4 stage (3 reads, 2 writes) over 256^3 FP32 fields; and the 3D stage
-    kernels' tile access pattern over the 256^3 problem without compute, per
-    tile shape and z-chunk count. Each figure is bytes moved / event-timed
-    duration, the mean of 20 launches after 3 warm-up launches."""
-    import torch
-    from phix_amd import capi
-    stream = torch.cuda.current_stream()
-    sh = capi.stream_handle(stream)
-    big = (1 << 30) // 4
-    x = torch.zeros(big, dtype=torch.float32, device="cuda:0")
-    y = torch.zeros(big, dtype=torch.float32, device="cuda:0")
-    n3 = 256 ** 3
-    f = [torch.zeros(n3, dtype=torch.float32, device="cuda:0") for _ in range(5)]
-    s3 = 258 ** 3
-    t = [torch.zeros(s3, dtype=torch.float32, device="cuda:0") for _ in range(4)]
-    P = capi.ptr
-
-    def probe(kind, bufs, n, side=0, tile=0, blocks=0, reps=20):
-        args = [P(b) if b is not None else None for b in bufs] + [None] * (5 - len(bufs))
-        call = lambda: capi.call("phx_hbm_probe", kind, *args, n, side, tile, blocks, sh)
-        for _ in range(3):
-            call()
-        ms = timed(lambda: [call() for _ in range(reps)], stream)
-        return ms / 1e3 / reps
-
-    res = {}
-    for blocks in (2048, 8192):
-        res[f"read 1GiB, {blocks} workgroups"] = 4 * big / probe(0, [x, y], big, blocks=blocks) / 1e9
-        res[f"write 1GiB, {blocks} workgroups"] = 4 * big / probe(1, [x], big, blocks=blocks) / 1e9
-    res["copy 1GiB, one float4 per thread"] = 8 * big / probe(2, [x, y], big) / 1e9
-    for blocks in (2048, 16384):
-        res[f"RK4-stage streams (3 reads, 2 writes) 256^3, {blocks} workgroups"] = \
-            20 * n3 / probe(3, f, n3, blocks=blocks) / 1e9
-    alg = 20 * n3   # the 5 passes over the interior cells
-    for tile, name in ((0, "64x16"), (1, "128x8"), (2, "256x4")):
-        for chunks in (16, 64):
-            for kind, what in ((4, "with halo"), (5, "cells only")):
-                res[f"stage tile pattern {name}, {chunks} z-chunks, {what}"] = \
-                    alg / probe(kind, t, s3, side=256, tile=tile, blocks=chunks) / 1e9
-    res = {k: round(v, 1) for k, v in res.items()}
-    key = "RK4-stage streams (3 reads, 2 writes) 256^3, 16384 workgroups"
-    roof = {"bound": "hbm", "achieved": res[key], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(res[key] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_probe_stage"}
-    cfg = {"workload": "hbm_probe", "patterns_gbs": res, "parallelism": "single-gpu"}
-    return line("GB/s, HBM access-pattern probes (bytes moved / time)", res[key], "GB/s", 20, 3, 0.0, "f32",
-                "zero-filled buffers", cfg, roof, None)
-
-
 WORKLOADS = {"copycal": bench_copy, "ch3d": bench_ch3d, "ch2d1024": bench_ch2d, "ac1024": bench_ac, "ens512": bench_ens,
-             "euler128": bench_euler, "gfa600": bench_gfa, "hbmprobe": bench_hbmprobe}
+             "euler128": bench_euler, "gfa600": bench_gfa}
 
 
 def main(argv=None) -> int:
@@ -584,16 +571,26 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--dtype", choices=["f32", "f64"], default="f32")
+    ap.add_argument("--dtype", choices=["f32", "f64"], default="f64",
+                    help="CH RK4 precision of the line (the other one is reported as 'companion')")
+    ap.add_argument("--reps", type=int, default=5, help="timed windows of --steps steps; the median is reported")
+    ap.add_argument("--no-companion", action="store_true", help="skip the other precision's companion run")
+    ap.add_argument("--autotune", action="store_true", help="CH RK4: run the launch autotuner first")
+    ap.add_argument("--fp64-exact", action="store_true",
+                    help="CH RK4 FP64 in the reference's operation order (bit-equal to the oracle)")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="ch3d")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--tile", type=int, default=None, help="3D stage tile index (skips the autotuner)")
-    ap.add_argument("--zc", type=int, default=None, help="3D stage z-chunk (skips the autotuner)")
+    ap.add_argument("--tile", type=int, default=None, help="CH RK4 launch tile index (after any autotuning)")
+    ap.add_argument("--zc", type=int, default=None, help="3D stage z-chunk (after any autotuning)")
     a = ap.parse_args(argv)
+    a.reps = max(1, a.reps)
     if a.gpus != 1:
         print(f"bench.py: this benchmark runs on one GPU only (asked for --gpus {a.gpus}); "
               "the stencil path shards by slab with halo exchange but that is not built here.", file=sys.stderr)
         return 2
+    if a.fp64_exact:
+        from phix_amd import capi
+        capi.call("phx_ch_set_fp64_exact", 1)
     out = WORKLOADS[a.workload](a, not a.no_cpu_baseline)
     print(json.dumps(out), flush=True)
     return 0

@@ -233,6 +233,17 @@ int phx_ch_rk4_get_launch(const phx_grid* grid, int dtype, int* zc, int* tile);
 /* XCD-aware workgroup order of the 3D stage kernel (on by default): each of
  * the 8 XCDs gets a contiguous range of tiles so halo re-reads hit its L2. */
 int phx_ch_set_xcd_remap(int on);
+/* Arithmetic form of FP64 RK4 (no reference counterpart; process-wide, host
+ * setting). 0, the default: explicit FMAs with M folded into lap(mu)'s
+ * coefficients and, in 3D, the step finished from the stage states
+ * u2, u3, u4 (13 field passes per step instead of 17); within a few ulp of
+ * the reference's expression per step. 1: the reference's operation order
+ * without contraction and its k1+2k2+2k3+k4 accumulator
+ * (src/solver/Solver.cu:61-71, src/equation/Equation.cu:164-180), equal to
+ * the FP64 oracle bit for bit. Every kernel of one form returns the same bits.
+ * phx_ch_get_fp64_exact returns the current form. */
+int phx_ch_set_fp64_exact(int on);
+int phx_ch_get_fp64_exact(void);
 int phx_ch_rk4_autotune(const phx_grid* grid, int dtype, void* c, void* work,
                         const phx_ch_params* p, int steps, int* best_zc,
                         double* best_ms_per_step, phx_stream stream);
@@ -249,22 +260,6 @@ int phx_ch_rk4_autotune(const phx_grid* grid, int dtype, void* c, void* work,
  * thread at a time. */
 int phx_ch_timing(int on);
 int phx_ch_timing_read(int kind, int* launches, double* total_ms);
-
-/* HBM streaming probes (no reference counterpart: measurement only). One
- * launch of probe `kind` on `stream` over caller-owned FP32 device buffers of
- * n floats each (the bandwidth simple access patterns reach, beside the
- * stencil kernels' roofline):
- *   0 read a (b: sink, never written in practice); 1 write a;
- *   2 copy a -> b, one float4 per thread; 3 d = b + a/2, e = c + a/4
- *   (3 reads, 2 writes: an RK4 stage's streams); kinds 0, 1, 3 use `blocks`
- *   grid-stride workgroups (0: 8192);
- *   4 the 3D stage kernels' access pattern without compute: TX x TY columns
- *   (tile 0: 64x16, 1: 128x8, 2: 256x4) of `blocks` z-chunks of a side^3
- *   grid stored with one ghost layer (n >= (side+2)^3), per plane the
- *   (TX+4) x (TY+4) halo of a, the cells of b and c read, of d and c written;
- *   5 as 4 reading only the cells of a. */
-int phx_hbm_probe(int kind, float* a, float* b, float* c, float* d, float* e, long long n, int side, int tile,
-                  int blocks, phx_stream stream);
 
 /* ------------------------------------------ FP64 diagnostics ---- */
 /* Workspace in bytes for phx_mass_energy. */

@@ -130,13 +130,14 @@ SIG = {
     "phx_ch_rk4_steps": (C.c_int, [GP, C.c_int, VP, VP, C.POINTER(CHParams), C.c_int, VP]),
     "phx_ch_rk4_set_zchunk": (C.c_int, [GP, C.c_int, C.c_int]),
     "phx_ch_set_xcd_remap": (C.c_int, [C.c_int]),
+    "phx_ch_set_fp64_exact": (C.c_int, [C.c_int]),
+    "phx_ch_get_fp64_exact": (C.c_int, []),
     "phx_ch_rk4_get_launch": (C.c_int, [GP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "phx_ch_rk4_set_tile": (C.c_int, [GP, C.c_int, C.c_int]),
     "phx_ch_rk4_autotune": (C.c_int, [GP, C.c_int, VP, VP, C.POINTER(CHParams), C.c_int, C.POINTER(C.c_int),
                                       C.POINTER(C.c_double), VP]),
     "phx_ch_timing": (C.c_int, [C.c_int]),
     "phx_ch_timing_read": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
-    "phx_hbm_probe": (C.c_int, [C.c_int, VP, VP, VP, VP, VP, C.c_longlong, C.c_int, C.c_int, C.c_int, VP]),
     "phx_reduce_workspace_bytes": (C.c_size_t, [GP]),
     "phx_mass_energy": (C.c_int, [GP, C.c_int, VP, C.POINTER(CHParams), VP, VP, VP]),
     "phx_ac_rk4_workspace_bytes": (C.c_size_t, [GP, C.c_int, C.c_int]),

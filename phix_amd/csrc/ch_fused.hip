@@ -1782,6 +1782,13 @@ static FastCoef<T> fast_coef(const phx_grid* G, const phx_ch_params* p) {
 
 static int g_xcd_remap = 1;
 static int g_euler_fused = 1;
+// FP64 RK4 arithmetic form. 0 (default): the FP32 form evaluated in double --
+// explicit FMAs, M folded into lap(mu)'s coefficients and, in 3D, the step
+// finished from the stage states (13 field passes per step instead of 17);
+// within a few ulp of the oracle per step. 1: the reference's operation order
+// without contraction and its k1+2k2+2k3+k4 accumulator, bit-equal to the
+// oracle. Multi-step Euler always uses the reference's order.
+static int g_fp64_exact = 0;
 
 // Multi-step Euler, one launch per step (1D and 2D): the one-stage stencil
 // kernel in its exact form evaluates mu on a ring around each 32x8 tile (the
@@ -2389,13 +2396,13 @@ int phx_ch_rk4_set_tile(const phx_grid* G, int dtype, int tile) {
 
 }  // extern "C"
 
-// Schedule by the launch's tile index; `pure` runs nsteps (even) two-stage or
-// whole-step steps only (autotuner timing).
-static int run_rk4(const phx_grid* G, int dtype, void* c, void* work, const phx_ch_params* p, int nsteps,
-                   hipStream_t st, Launch3 l, int pure) {
+// The schedule of one FP64 form (X64: exact) and of FP32.
+template <bool X64>
+static int run_rk4_t(const phx_grid* G, int dtype, void* c, void* work, const phx_ch_params* p, int nsteps,
+                     hipStream_t st, Launch3 l, int pure) {
     if (G->dim == 2 && l.tile >= 1 && step2d_ok(G))
-        return dtype == PHX_F64 ? rk4_step2d_steps_t<double, true>(G, (double*)c, (double*)work, p, nsteps, st,
-                                                                   l.tile - 1, pure)
+        return dtype == PHX_F64 ? rk4_step2d_steps_t<double, X64>(G, (double*)c, (double*)work, p, nsteps, st,
+                                                                  l.tile - 1, pure)
                                 : rk4_step2d_steps_t<float, false>(G, (float*)c, (float*)work, p, nsteps, st,
                                                                    l.tile - 1, pure);
     if (G->dim != 3) l.tile = 0;
@@ -2403,13 +2410,22 @@ static int run_rk4(const phx_grid* G, int dtype, void* c, void* work, const phx_
     if (G->dim == 3 && is_pair_tile(l.tile)) {
         if (!pair_ok(G)) l.tile = dtype == PHX_F32 ? 0 : 1;
         else
-            return dtype == PHX_F64 ? rk4_pair_steps_t<double, true>(G, (double*)c, (double*)work, p, nsteps, st,
-                                                                     l.zc, l.tile - kNumTiles3, pure)
+            return dtype == PHX_F64 ? rk4_pair_steps_t<double, X64>(G, (double*)c, (double*)work, p, nsteps, st,
+                                                                    l.zc, l.tile - kNumTiles3, pure)
                                     : rk4_pair_steps_t<float, false>(G, (float*)c, (float*)work, p, nsteps, st,
                                                                      l.zc, l.tile - kNumTiles3, pure);
     }
-    return dtype == PHX_F64 ? rk4_steps_t<double, true>(G, (double*)c, (double*)work, p, nsteps, st, l.zc, l.tile)
+    return dtype == PHX_F64 ? rk4_steps_t<double, X64>(G, (double*)c, (double*)work, p, nsteps, st, l.zc, l.tile)
                             : rk4_steps_t<float, false>(G, (float*)c, (float*)work, p, nsteps, st, l.zc, l.tile);
+}
+
+// Schedule by the launch's tile index; `pure` runs nsteps (even) two-stage or
+// whole-step steps only (autotuner timing). FP64 takes the form g_fp64_exact
+// selects.
+static int run_rk4(const phx_grid* G, int dtype, void* c, void* work, const phx_ch_params* p, int nsteps,
+                   hipStream_t st, Launch3 l, int pure) {
+    return g_fp64_exact ? run_rk4_t<true>(G, dtype, c, work, p, nsteps, st, l, pure)
+                        : run_rk4_t<false>(G, dtype, c, work, p, nsteps, st, l, pure);
 }
 
 extern "C" {
@@ -2459,7 +2475,8 @@ int phx_ch_rk4_autotune(const phx_grid* G, int dtype, void* c, void* work, const
     if (G->dim == 3) {
         const Tile3 tl = shape3(tile);
         const long long ncols = (long long)cdiv(G->n[0], tl.tx) * cdiv(G->n[1], tl.ty);
-        const int bpc = dtype == PHX_F64 ? blocks_per_cu<double, true>(tile) : blocks_per_cu<float, false>(tile);
+        const int bpc = dtype == PHX_F32 ? blocks_per_cu<float, false>(tile)
+                        : (g_fp64_exact ? blocks_per_cu<double, true>(tile) : blocks_per_cu<double, false>(tile));
         for (int k = 1; k <= 3 && bpc > 0; ++k) {
             const long long nch = std::max(1LL, (long long)k * bpc * ncu / ncols);
             const int zc = (int)cdiv(G->n[2], (int)std::min<long long>(nch, G->n[2]));
@@ -2494,6 +2511,13 @@ int phx_ch_rk4_autotune(const phx_grid* G, int dtype, void* c, void* work, const
     if (best_ms_per_step) *best_ms_per_step = best;
     return PHX_OK;
 }
+
+int phx_ch_set_fp64_exact(int on) {
+    g_fp64_exact = on ? 1 : 0;
+    return PHX_OK;
+}
+
+int phx_ch_get_fp64_exact(void) { return g_fp64_exact; }
 
 int phx_ch_timing(int on) {
     g_lt.on = on ? 1 : 0;

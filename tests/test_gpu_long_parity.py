@@ -1,0 +1,199 @@
+"""Long and full-size parity of the fused RK4 paths against the FP64 oracle.
+
+* FP64 CH RK4 over 1000 steps, compared with the oracle after every step
+  (<= 1e-12 max-abs), in the default arithmetic form (explicit FMAs, stage
+  states) and, bit for bit, in the reference-order form.
+* Config 2 at its full size (256^3 zero-flux, 500 steps): the FP64 path against
+  the oracle after each of the first 5 steps and every 50 steps, and the FP32
+  bulk's error stated against the oracle at the same checkpoints.
+* Config 3 at its full size (1024^2 x 8 Allen-Cahn fields): 10 steps against
+  the oracle (FP64 <= 1e-12, FP32 error stated), then the 2000-step FP32 run
+  checked through properties: the discrete free energy falls between
+  checkpoints, the fields stay bounded, and the final state is close to the
+  FP64 path's.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from phix_amd import capi, init
+from oracle import oracle as O
+
+from helpers import ac_params, ch_params, to_dev, to_np, stream
+
+pytestmark = pytest.mark.gpu
+P, N = capi.PERIODIC, capi.NOFLUX
+TOL64 = 1e-12
+
+
+def all_cores():
+    return O.set_threads(min(16, os.cpu_count() or 1))
+
+
+def fp64_form(exact):
+    capi.call("phx_ch_set_fp64_exact", 1 if exact else 0)
+
+
+def rk4(g, dtype, t, work, p, k):
+    capi.call("phx_ch_rk4_steps", C.byref(g), dtype, capi.ptr(t), capi.ptr(work), C.byref(p), k, stream())
+
+
+def ch_work(g, dtype, dev):
+    import torch
+    tdt, esz = (torch.float64, 8) if dtype == capi.F64 else (torch.float32, 4)
+    return torch.zeros(capi.lib().phx_ch_rk4_workspace_bytes(C.byref(g), dtype) // esz, dtype=tdt, device=dev)
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("exact", [False, True])
+@pytest.mark.parametrize("n,bc", [((64, 64, 64), N), ((128, 128), P)])
+def test_rk4_fp64_1000_steps_every_step(dev, n, bc, exact):
+    """Config-2 physics (c0 = 0.4 + U(-0.02, 0.02), kappa = 1, M = 1, dt =
+    0.005), 1000 RK4 steps, every stored cell of the active region compared
+    with the oracle after every step."""
+    g = capi.make_grid(n, bc=bc)
+    p = ch_params(rho=1.0, ca=0.0, cb=1.0, kappa=1.0, M=1.0, dt=0.005)
+    c = init.to_stored(init.ch_field(n, 2, 0.4, 0.02), g)
+    tc = to_dev(c, dev)
+    work = ch_work(g, capi.F64, dev)
+    ow = np.zeros((6,) + c.shape)
+    all_cores()
+    fp64_form(exact)
+    worst = 0.0
+    try:
+        for s in range(1000):
+            rk4(g, capi.F64, tc, work, p, 1)
+            O.ch_rk4_steps(g, p, c, 1, ow)
+            err = float(np.abs(capi.active(to_np(tc), g) - capi.active(c, g)).max())
+            worst = max(worst, err)
+            assert err <= (0.0 if exact else TOL64), (s, err)
+    finally:
+        fp64_form(False)
+        O.set_threads(1)
+    print(f"rk4 fp64 {'exact' if exact else 'fma'} form {n}: worst per-step max-abs over 1000 steps {worst:.3e}",
+          flush=True)
+
+
+@pytest.mark.timeout(600)
+def test_config2_full_size_against_oracle(dev):
+    """Config 2: 256^3, zero-flux, c0 = 0.4 + U(-0.02, 0.02) seed 2, kappa = 1,
+    M = 1, dt = 0.005, RK4, 500 steps. FP64 (default form) and FP32 bulk run
+    beside the FP64 oracle; both are compared with it after steps 1-5 and
+    every 50 steps. FP64: <= 1e-12 at every checkpoint. FP32: max-abs and the
+    final relative L2 stated and bounded. Mass: FP64 compensated reduction."""
+    import torch
+    n = (256, 256, 256)
+    g = capi.make_grid(n, bc=N)
+    p = ch_params(rho=1.0, ca=0.0, cb=1.0, kappa=1.0, M=1.0, dt=0.005)
+    c = init.to_stored(init.ch_field(n, 2, 0.4, 0.02), g)
+    t64 = to_dev(c, dev)
+    t32 = to_dev(c, dev, torch.float32)
+    w64, w32 = ch_work(g, capi.F64, dev), ch_work(g, capi.F32, dev)
+    ow = np.zeros((6,) + c.shape)
+    red = torch.empty(capi.lib().phx_reduce_workspace_bytes(C.byref(g)) // 8, dtype=torch.float64, device=dev)
+    out = torch.empty(2, dtype=torch.float64, device=dev)
+
+    def mass(t, dtype):
+        capi.call("phx_mass_energy", C.byref(g), dtype, capi.ptr(t), C.byref(p), capi.ptr(red), capi.ptr(out),
+                  stream())
+        return out.tolist()
+
+    m0, e0 = mass(t64, capi.F64)
+    all_cores()
+    checks = [1, 2, 3, 4, 5] + list(range(50, 501, 50))
+    done, w64max, w32max = 0, 0.0, 0.0
+    try:
+        for s in checks:
+            k = s - done
+            rk4(g, capi.F64, t64, w64, p, k)
+            rk4(g, capi.F32, t32, w32, p, k)
+            O.ch_rk4_steps(g, p, c, k, ow)
+            done = s
+            ref = capi.physical(c, g)
+            e64 = float(np.abs(capi.physical(to_np(t64), g) - ref).max())
+            e32 = float(np.abs(capi.physical(to_np(t32), g) - ref).max())
+            print(f"config2 step {s}: fp64 max-abs {e64:.3e}, fp32 max-abs {e32:.3e}", flush=True)
+            w64max, w32max = max(w64max, e64), max(w32max, e32)
+            assert e64 <= TOL64, (s, e64)
+    finally:
+        O.set_threads(1)
+    ref = capi.physical(c, g)
+    rel32 = float(np.linalg.norm(capi.physical(to_np(t32), g) - ref) / np.linalg.norm(ref))
+    m1, e1 = mass(t64, capi.F64)
+    m32, _ = mass(t32, capi.F32)
+    mo, eo = O.mass_energy(g, p, c)
+    print(f"config2 500 steps vs oracle: fp64 worst {w64max:.3e}; fp32 worst {w32max:.3e}, final rel-L2 {rel32:.3e}; "
+          f"mass drift fp64 {abs(m1 - m0) / m0:.3e} fp32 {abs(m32 - m0) / m0:.3e}; energy {e0:.6e} -> {e1:.6e} "
+          f"(oracle {eo:.6e})", flush=True)
+    assert w32max <= 5e-6 and rel32 <= 5e-6
+    assert abs(m1 - m0) <= 1e-12 * abs(m0) and abs(m1 - mo) <= 1e-12 * abs(mo)
+    assert abs(e1 - eo) <= 1e-10 * abs(eo)
+    assert e1 < e0
+
+
+def ac_run(g, p, ts, work, k):
+    arr = (C.c_void_p * len(ts))(*[capi.ptr(t) for t in ts])
+    capi.call("phx_ac_rk4_steps", C.byref(g), capi.F64 if ts[0].element_size() == 8 else capi.F32, len(ts), arr,
+              capi.ptr(work), C.byref(p), k, stream())
+
+
+def ac_energy(e, kappa, gamma):
+    """Discrete free energy of periodic fields e (nf, ny, nx), FP64:
+    sum over cells of sum_i(-e_i^2/2 + e_i^4/4) + gamma sum_{i<j} e_i^2 e_j^2
+    + kappa/2 sum_i |forward differences of e_i|^2 (dx = 1), the energy whose
+    gradient flow the 5-point semi-discrete Allen-Cahn system is."""
+    e = e.astype(np.float64)
+    sq = e * e
+    bulk = (-sq / 2 + sq * sq / 4).sum()
+    s = sq.sum(axis=0)
+    pair = gamma * 0.5 * (s * s - (sq * sq).sum(axis=0)).sum()
+    grad = sum(((np.roll(e, -1, axis=ax) - e) ** 2).sum() for ax in (1, 2))
+    return float(bulk + pair + 0.5 * kappa * grad)
+
+
+@pytest.mark.timeout(600)
+def test_config3_full_size(dev):
+    """Config 3: 1024^2 periodic, 8 order parameters one-hot on floor(8u) plus
+    U(-0.001, 0.001) (seed 3), gamma = 1.5, kappa = 2, L = 1, dt = 0.01, RK4,
+    2000 steps, FP32 bulk."""
+    import torch
+    n, nf = (1024, 1024), 8
+    g = capi.make_grid(n, bc=P)
+    p = ac_params(L=1.0, kappa=2.0, gamma=1.5, dt=0.01)
+    a = init.ac_fields(n, nf, 3)
+    etas = [init.to_stored(a[f], g) for f in range(nf)]
+    t64 = [to_dev(e, dev) for e in etas]
+    t32 = [to_dev(e, dev, torch.float32) for e in etas]
+    w64 = torch.zeros(capi.lib().phx_ac_rk4_workspace_bytes(C.byref(g), capi.F64, nf) // 8, dtype=torch.float64,
+                      device=dev)
+    w32 = torch.zeros(capi.lib().phx_ac_rk4_workspace_bytes(C.byref(g), capi.F32, nf) // 4, dtype=torch.float32,
+                      device=dev)
+    # 10 steps against the oracle at full size
+    ac_run(g, p, t64, w64, 10)
+    ac_run(g, p, t32, w32, 10)
+    O.ac_rk4_steps(g, p, etas, 10)
+    e64 = max(float(np.abs(capi.physical(to_np(t64[f]), g) - capi.physical(etas[f], g)).max()) for f in range(nf))
+    e32 = max(float(np.abs(capi.physical(to_np(t32[f]), g) - capi.physical(etas[f], g)).max()) for f in range(nf))
+    print(f"config3 1024^2 x 8, 10 steps vs oracle: fp64 max-abs {e64:.3e}, fp32 max-abs {e32:.3e}", flush=True)
+    assert e64 <= TOL64 and e32 <= 1e-5
+    del etas
+
+    def phys(ts):
+        return np.stack([capi.physical(to_np(t), g)[0] for t in ts])
+
+    energies = [ac_energy(phys(t32), p.kappa, p.gamma)]
+    for chunk in range(10):            # 10 x 199 more steps: 2000 in all
+        k = 199
+        ac_run(g, p, t32, w32, k)
+        ac_run(g, p, t64, w64, k)
+        x = phys(t32)
+        assert np.isfinite(x).all() and x.min() > -0.2 and x.max() < 1.2, (chunk, x.min(), x.max())
+        energies.append(ac_energy(x, p.kappa, p.gamma))
+    x32, x64 = phys(t32), phys(t64)
+    rel = float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64))
+    print(f"config3 2000 steps: energy {energies[0]:.6e} -> {energies[-1]:.6e} at 10 checkpoints; "
+          f"fp32 vs fp64 rel-L2 {rel:.3e}, max-abs {np.abs(x32 - x64).max():.3e}", flush=True)
+    assert all(b < a for a, b in zip(energies, energies[1:])), energies
+    assert rel <= 1e-3

@@ -125,23 +125,6 @@ def test_timing_record_without_launches():
     capi.call("phx_ch_timing", 0)
 
 
-def test_hbm_probe_rejects_bad_arguments():
-    """Argument checks of the HBM probes run on the host, before any launch."""
-    lib = capi.lib()
-    fake = C.c_void_p(0x1000)   # never dereferenced: every call below fails its checks first
-    assert lib.phx_hbm_probe(6, fake, fake, fake, fake, fake, 1024, 0, 0, 0, None) == capi.PHX_ERR_INVALID
-    assert lib.phx_hbm_probe(-1, fake, fake, fake, fake, fake, 1024, 0, 0, 0, None) == capi.PHX_ERR_INVALID
-    assert lib.phx_hbm_probe(1, fake, None, None, None, None, 1022, 0, 0, 0, None) == capi.PHX_ERR_INVALID
-    assert lib.phx_hbm_probe(2, fake, None, None, None, None, 1024, 0, 0, 0, None) == capi.PHX_ERR_INVALID
-    assert lib.phx_hbm_probe(3, fake, fake, fake, None, fake, 1024, 0, 0, 0, None) == capi.PHX_ERR_INVALID
-    assert lib.phx_hbm_probe(0, fake, fake, None, None, None, 1024, 0, 0, 8192, None) == capi.PHX_ERR_INVALID
-    # tile pattern: side must be a multiple of the tile, buffers hold (side+2)^3
-    assert lib.phx_hbm_probe(4, fake, fake, fake, fake, None, 66 ** 3, 64, 3, 4, None) == capi.PHX_ERR_INVALID
-    assert lib.phx_hbm_probe(4, fake, fake, fake, fake, None, 66 ** 3, 64, 2, 4, None) == capi.PHX_ERR_INVALID
-    assert lib.phx_hbm_probe(4, fake, fake, fake, fake, None, 66 ** 3 - 1, 64, 0, 4, None) == capi.PHX_ERR_INVALID
-    assert lib.phx_hbm_probe(5, fake, fake, fake, fake, None, 66 ** 3, 64, 0, 65, None) == capi.PHX_ERR_INVALID
-
-
 def test_fused_paths_reject_planes_beyond_32bit_offsets():
     """A stored x-y plane of 2^31 cells or more is refused before any launch;
     below that, z-register tiles are accepted for FP64 and FP32 alike."""
