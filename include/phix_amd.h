@@ -203,6 +203,21 @@ size_t phx_ch_rk4_workspace_bytes(const phx_grid* grid, int dtype);
 int phx_ch_rk4_steps(const phx_grid* grid, int dtype, void* c, void* work,
                      const phx_ch_params* p, int nsteps, phx_stream stream);
 
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). phx_ch_rk4_steps with the RK4 workspace held by the library:
+ * a static device array of PHX_CH_RK4_NOALLOC_BYTES per GPU (64 MiB: e.g.
+ * 2048^2 FP32, 1024^2 FP64, 128^3 FP64 with one ghost layer), taken from the
+ * library's load until the process ends. Grids whose workspace
+ * (phx_ch_rk4_workspace_bytes) exceeds it return PHX_ERR_UNSUPPORTED. Calls no
+ * allocating routine; results equal phx_ch_rk4_steps's for the same launch.
+ * Every call on one GPU uses that GPU's one array: calls that overlap on a GPU
+ * (other streams, other host threads) race and must be serialised by the
+ * caller. Launches made through it are never timed by phx_ch_timing. */
+#define PHX_CH_RK4_NOALLOC_BYTES (64u << 20)
+int phx_ch_rk4_steps_noalloc(const phx_grid* grid, int dtype, void* c,
+                             const phx_ch_params* p, int nsteps, phx_stream stream);
+
 /* Launch shape of the 3D stage kernels: z-chunk zc (default 16 FP32, 32
  * FP64; default tiles: 3D 9 (z-register 64x16/256) FP32 and 11 (z-register
  * 128x16/512) FP64, 2D FP32 the whole-step tile 4 when n >= 8, else 0), per

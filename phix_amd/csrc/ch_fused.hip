@@ -1985,11 +1985,25 @@ struct LaunchTimer {
     std::vector<hipEvent_t> ev;
     std::vector<int> kind;
     size_t used = 0;
+    int pending = 0;   // begin() recorded an event that end() must close
 };
 static LaunchTimer g_lt;
 
+// Set while an entry point that must not allocate runs on this thread: its
+// launches are never timed (the timer creates events on demand).
+static thread_local int t_timer_off = 0;
+
+// A launch is timed only when the timer is on, this thread allows it and the
+// stream is not being captured into a graph (an event recorded into a graph
+// would time capture, not replay); end() follows what begin() decided.
 void launch_timer_begin(hipStream_t st) {
-    if (!g_lt.on) return;
+    g_lt.pending = 0;
+    if (!g_lt.on || t_timer_off) return;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
+        (void)hipGetLastError();
+        return;
+    }
     if (2 * g_lt.used + 2 > g_lt.ev.size()) {
         for (int i = 0; i < 2; ++i) {
             hipEvent_t e;
@@ -2002,10 +2016,12 @@ void launch_timer_begin(hipStream_t st) {
         g_lt.kind.push_back(0);
     }
     (void)hipEventRecord(g_lt.ev[2 * g_lt.used], st);
+    g_lt.pending = 1;
 }
 
 void launch_timer_end(int kind, hipStream_t st) {
-    if (!g_lt.on) return;
+    if (!g_lt.pending) return;
+    g_lt.pending = 0;
     (void)hipEventRecord(g_lt.ev[2 * g_lt.used + 1], st);
     g_lt.kind[g_lt.used] = kind;
     ++g_lt.used;
@@ -2440,6 +2456,64 @@ int phx_ch_rk4_steps(const phx_grid* G, int dtype, void* c, void* work, const ph
     if (nsteps < 0) return fail(PHX_ERR_INVALID, "phx_ch_rk4_steps: nsteps < 0");
     if (nsteps == 0) return PHX_OK;
     return run_rk4(G, dtype, c, work, p, nsteps, as_stream(s), launch_for(G, dtype), 0);
+}
+
+// ------------------------------------------- RK4 without allocation ----
+// phx_ch_rk4_steps_noalloc runs phx_ch_rk4_steps with its three-field
+// workspace in this file-scope device array instead of a caller buffer. Each
+// GPU holds its own copy (PHX_CH_RK4_NOALLOC_BYTES) from the library's load
+// until the process ends. Host code must not pass the array's name to a
+// kernel (on the host it is a stand-in, not a device address): its address
+// on the current device comes from hipGetSymbolAddress, taken once per device
+// on the first call there and kept in g_scratch_dev.
+static __device__ __align__(256) double scratch[PHX_CH_RK4_NOALLOC_BYTES / sizeof(double)];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[kMaxDevices];
+
+static int scratch_on_current_device(void** out) {
+    int dev = 0;
+    PHX_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        return fail(PHX_ERR_UNSUPPORTED, "phx_ch_rk4_steps_noalloc: device index beyond 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(PHX_ERR_HIP, std::string("phx_ch_rk4_steps_noalloc: hipGetSymbolAddress: ") +
+                                         hipGetErrorString(e));
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return PHX_OK;
+}
+
+extern "C" int phx_ch_rk4_steps_noalloc(const phx_grid* G, int dtype, void* c, const phx_ch_params* p,
+                                        int nsteps, phx_stream s) {
+    int rc = check_grid_fused(G, "phx_ch_rk4_steps_noalloc");
+    if (rc) return rc;
+    if ((rc = check_dtype(dtype, "phx_ch_rk4_steps_noalloc"))) return rc;
+    if ((rc = check_ch_params(p, "phx_ch_rk4_steps_noalloc"))) return rc;
+    if (!c) return fail(PHX_ERR_INVALID, "phx_ch_rk4_steps_noalloc: NULL field");
+    if (nsteps < 0) return fail(PHX_ERR_INVALID, "phx_ch_rk4_steps_noalloc: nsteps < 0");
+    const size_t need = 3 * (size_t)make_layout(G).total() * (dtype == PHX_F64 ? 8 : 4);
+    if (need > (size_t)PHX_CH_RK4_NOALLOC_BYTES)
+        return fail(PHX_ERR_UNSUPPORTED, "phx_ch_rk4_steps_noalloc: the RK4 workspace of this grid needs " +
+                                             std::to_string(need) + " bytes, more than the " +
+                                             std::to_string((size_t)PHX_CH_RK4_NOALLOC_BYTES) +
+                                             " the library holds per GPU; use phx_ch_rk4_steps");
+    if (nsteps == 0) return PHX_OK;
+    void* work = nullptr;
+    if ((rc = scratch_on_current_device(&work))) return rc;
+    // The stage kernels write every physical cell of a workspace field
+    // before reading it and never read its ghost cells, so the array's
+    // contents from earlier calls need no clearing.
+    t_timer_off = 1;
+    rc = run_rk4(G, dtype, c, work, p, nsteps, as_stream(s), launch_for(G, dtype), 0);
+    t_timer_off = 0;
+    return rc;
 }
 
 // Times every launch candidate on `steps` steps of the caller's state

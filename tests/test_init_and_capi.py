@@ -136,3 +136,16 @@ def test_fused_paths_reject_planes_beyond_32bit_offsets():
     ok = capi.make_grid((30000, 30000, 2), bc=capi.PERIODIC)          # 3.6 GB FP32 plane: z-register ok
     assert lib.phx_ch_rk4_set_tile(C.byref(ok), capi.F32, 11) == 0
     assert lib.phx_ch_rk4_set_tile(C.byref(ok), capi.F64, 11) == capi.PHX_ERR_UNSUPPORTED   # 7.2 GB FP64 plane
+
+
+def test_noalloc_rk4_rejects_oversized_grids():
+    """The no-allocation RK4 entry point refuses grids whose workspace exceeds
+    the array the library holds per GPU, before touching the device."""
+    lib = capi.lib()
+    p = capi.CHParams(1, 0, 1, 1.0, 1, 0.005)
+    fake = C.c_void_p(0x1000)   # never dereferenced: the size check fails first
+    big = capi.make_grid((256, 256, 256), bc=capi.NOFLUX)
+    assert lib.phx_ch_rk4_workspace_bytes(C.byref(big), capi.F64) > 64 << 20
+    assert lib.phx_ch_rk4_steps_noalloc(C.byref(big), capi.F64, fake, C.byref(p), 1, None) == capi.PHX_ERR_UNSUPPORTED
+    assert b"per GPU" in lib.phx_last_error()
+    assert lib.phx_ch_rk4_steps_noalloc(C.byref(big), capi.F64, None, C.byref(p), 1, None) == capi.PHX_ERR_INVALID
