@@ -220,7 +220,7 @@ int phx_ch_rk4_steps_noalloc(const phx_grid* grid, int dtype, void* c,
 
 /* Launch shape of the 3D stage kernels: z-chunk zc (default 16 FP32, 32
  * FP64; default tiles: 3D 9 (z-register 64x16/256) FP32 and 11 (z-register
- * 128x16/512) FP64, 2D FP32 the whole-step tile 4 when n >= 8, else 0), per
+ * 128x16/512) FP64, 2D the whole-step tile 4 when n >= 8, else 0), per
  * (dim, n, dtype); the planes are split into cdiv(n_z, zc) balanced chunks of
  * floor or ceil(n_z / chunks) planes, one workgroup column each. The
  * autotuner times every tile shape x z-chunk in {4,8,16,32,64} plus the

@@ -1475,6 +1475,142 @@ __global__ __launch_bounds__(256) void k_ch_pair_3d(PairArgs<T> A) {
     }
 }
 
+// The eight levels of the 2D whole-step kernel after c_n is in SC. Each
+// level is a fully unrolled loop over the thread's positions t = tid + 256 i
+// of the level's region, in groups of G positions whose LDS operands are all
+// read before any is used (independent chains for the scheduler); the last
+// partial pass reads a valid position and skips the store. GEN: the block
+// needs the ghost rules of NOFLUX / FIXED faces.
+template <typename T, int TX, int TY, bool EXACT, bool GEN>
+__device__ __forceinline__ void step2d_levels(const PairArgs<T>& A, const T* SC, T* SA, T* SB, T* SACC,
+                                              const T* BCV, int x0, int y0, long long zb) {
+    constexpr int WC = TX + 16;
+    constexpr int G = sizeof(T) == 8 ? 3 : 5;
+    const int tid = threadIdx.x;
+    // mu on ring R from the state level on ring R+1 (row width TX+2R+2)
+    auto eval_mu = [&](auto rr, const T* src) {
+        constexpr int R = decltype(rr)::value;
+        constexpr int WS = TX + 2 * R + 2, WD = TX + 2 * R, ND = WD * (TY + 2 * R);
+        constexpr int NIT = (ND + 255) / 256;
+#pragma unroll
+        for (int i0 = 0; i0 < NIT; i0 += G) {
+            T s0[G], s1[G], s2[G], s3[G], s4[G];
+            int dst[G], code[G];
+#pragma unroll
+            for (int j = 0; j < G; ++j) {
+                if (i0 + j >= NIT) break;
+                const int t = min(tid + 256 * (i0 + j), ND - 1);
+                const int ly = t / WD, lx = t - ly * WD;
+                const int si = (ly + 1) * WS + lx + 1;
+                dst[j] = tid + 256 * (i0 + j);
+                code[j] = GEN ? ghost_code_xy(x0 - R + lx, y0 - R + ly, A) : 0;
+                s0[j] = src[si];
+                s1[j] = src[si + 1];
+                s2[j] = src[si - 1];
+                s3[j] = src[si + WS];
+                s4[j] = src[si - WS];
+            }
+#pragma unroll
+            for (int j = 0; j < G; ++j) {
+                if (i0 + j >= NIT) break;
+                T xp = s1[j], xm = s2[j], yp = s3[j], ym = s4[j];
+                if (GEN) {
+                    if (code[j] & GC_SWX) { const T q = xp; xp = xm; xm = q; }
+                    if (code[j] & GC_SWY) { const T q = yp; yp = ym; ym = q; }
+                }
+                T v = mu_at<T, EXACT, 2>(A, s0[j], xp, xm, yp, ym, (T)0, (T)0);
+                if (GEN && (code[j] & GC_FIX)) v = ghost_value_xy(code[j], BCV);
+                if (ND % 256 == 0 || i0 + j < NIT - 1 || dst[j] < ND) SA[dst[j]] = v;
+            }
+        }
+    };
+    // stage state on ring R from the mu level on ring R+1: c_n + a*k, and the
+    // accumulator update of stage S on the tile cells
+    auto eval_state = [&](auto rr, auto ss, T a) {
+        constexpr int R = decltype(rr)::value;
+        constexpr int S = decltype(ss)::value;
+        constexpr int WS = TX + 2 * R + 2, WD = TX + 2 * R, ND = WD * (TY + 2 * R);
+        constexpr int NIT = (ND + 255) / 256;
+#pragma unroll
+        for (int i0 = 0; i0 < NIT; i0 += G) {
+            T s0[G], s1[G], s2[G], s3[G], s4[G], cc[G];
+            int dst[G], code[G], lxy[G];
+#pragma unroll
+            for (int j = 0; j < G; ++j) {
+                if (i0 + j >= NIT) break;
+                const int t = min(tid + 256 * (i0 + j), ND - 1);
+                const int ly = t / WD, lx = t - ly * WD;
+                const int si = (ly + 1) * WS + lx + 1;
+                dst[j] = tid + 256 * (i0 + j);
+                lxy[j] = ((ly - R) << 16) | (lx - R + 0x8000);
+                code[j] = GEN ? ghost_code_xy(x0 - R + lx, y0 - R + ly, A) : 0;
+                cc[j] = SC[(ly + 8 - R) * WC + lx + 8 - R];
+                s0[j] = SA[si];
+                s1[j] = SA[si + 1];
+                s2[j] = SA[si - 1];
+                s3[j] = SA[si + WS];
+                s4[j] = SA[si - WS];
+            }
+#pragma unroll
+            for (int j = 0; j < G; ++j) {
+                if (i0 + j >= NIT) break;
+                T xp = s1[j], xm = s2[j], yp = s3[j], ym = s4[j];
+                if (GEN) {
+                    if (code[j] & GC_SWX) { const T q = xp; xp = xm; xm = q; }
+                    if (code[j] & GC_SWY) { const T q = yp; yp = ym; ym = q; }
+                }
+                T k = k_at<T, EXACT, 2>(A, s0[j], xp, xm, yp, ym, (T)0, (T)0);
+                T v = axpy<T, EXACT>(a, k, cc[j]);
+                if (GEN && (code[j] & GC_FIX)) {
+                    v = ghost_value_xy(code[j], BCV);
+                    k = (T)0;
+                }
+                const bool act = ND % 256 == 0 || i0 + j < NIT - 1 || dst[j] < ND;
+                if (act) SB[dst[j]] = v;
+                const int ox = (lxy[j] & 0xffff) - 0x8000, oy = lxy[j] >> 16;
+                if (act && ox >= 0 && ox < TX && oy >= 0 && oy < TY) {
+                    T& ac = SACC[oy * TX + ox];
+                    if (S == 1) ac = EXACT ? k : axpy<T, EXACT>(A.dt6, k, cc[j]);
+                    else ac = EXACT ? ac + (T)2 * k : axpy<T, EXACT>(A.dt3, k, ac);
+                }
+            }
+        }
+    };
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using I3 = std::integral_constant<int, 3>;
+    const T half = A.a1, full = A.a2;
+    eval_mu(std::integral_constant<int, 7>{}, SC);
+    __syncthreads();
+    eval_state(std::integral_constant<int, 6>{}, I1{}, half);
+    __syncthreads();
+    eval_mu(std::integral_constant<int, 5>{}, SB);
+    __syncthreads();
+    eval_state(std::integral_constant<int, 4>{}, I2{}, half);
+    __syncthreads();
+    eval_mu(I3{}, SB);
+    __syncthreads();
+    eval_state(I2{}, I3{}, full);
+    __syncthreads();
+    eval_mu(I1{}, SB);
+    __syncthreads();
+    // stage 4 on the tile cells
+    constexpr int W1 = TX + 2;
+    constexpr int N0 = TX * TY, NIT = (N0 + 255) / 256;
+#pragma unroll
+    for (int i = 0; i < NIT; ++i) {
+        const int t = tid + 256 * i;
+        if (N0 % 256 != 0 && t >= N0) break;
+        const int oy = t / TX, ox = t - oy * TX;
+        const int x = x0 + ox, y = y0 + oy;
+        const int si = (oy + 1) * W1 + ox + 1;
+        const T m = SA[si];
+        const T k = k_at<T, EXACT, 2>(A, m, SA[si + 1], SA[si - 1], SA[si + W1], SA[si - W1], (T)0, (T)0);
+        const T r = EXACT ? SC[(oy + 8) * WC + ox + 8] + A.dt6 * (SACC[t] + k) : axpy<T, EXACT>(A.dt6, k, SACC[t]);
+        if (x < A.L.n0 && y < A.L.n1) A.out[zb + (long long)(y + A.L.g) * A.L.sx + (x + A.L.g)] = r;
+    }
+}
+
 // ------------------------------------------------ 2D, whole step fused --
 // All four RK4 stages of a 2D step in one launch: the block loads c_n on its
 // tile plus an 8-cell ring (HBM -> LDS once) and evaluates, in LDS, the eight
@@ -1556,89 +1692,14 @@ __global__ __launch_bounds__(256) void k_ch_step_2d(PairArgs<T> A) {
     }
     __syncthreads();
 
-    // mu on ring R from the state level on ring R+1 (row width TX+2R+2)
-    auto eval_mu = [&](auto rr, const T* src) {
-        constexpr int R = decltype(rr)::value;
-        constexpr int WS = TX + 2 * R + 2, WD = TX + 2 * R, ND = WD * (TY + 2 * R);
-        for (int t = tid; t < ND; t += 256) {
-            const int ly = t / WD, lx = t - ly * WD;
-            const int si = (ly + 1) * WS + lx + 1;
-            const int code = xy_in ? 0 : ghost_code_xy(x0 - R + lx, y0 - R + ly, A);
-            T v;
-            if (code & GC_FIX) {
-                v = ghost_value_xy(code, BCV);
-            } else {
-                const T c = src[si];
-                T xp = src[si + 1], xm = src[si - 1], yp = src[si + WS], ym = src[si - WS];
-                if (code & GC_SWX) { const T q = xp; xp = xm; xm = q; }
-                if (code & GC_SWY) { const T q = yp; yp = ym; ym = q; }
-                v = mu_at<T, EXACT, 2>(A, c, xp, xm, yp, ym, (T)0, (T)0);
-            }
-            SA[t] = v;
-        }
-    };
-    // stage state on ring R from the mu level on ring R+1: c_n + a*k, and the
-    // accumulator update of stage S on the tile cells
-    auto eval_state = [&](auto rr, auto ss, T a) {
-        constexpr int R = decltype(rr)::value;
-        constexpr int S = decltype(ss)::value;
-        constexpr int WS = TX + 2 * R + 2, WD = TX + 2 * R, ND = WD * (TY + 2 * R);
-        for (int t = tid; t < ND; t += 256) {
-            const int ly = t / WD, lx = t - ly * WD;
-            const int si = (ly + 1) * WS + lx + 1;
-            const int code = xy_in ? 0 : ghost_code_xy(x0 - R + lx, y0 - R + ly, A);
-            const T cc = SC[(ly + 8 - R) * WC + lx + 8 - R];
-            T v, k = (T)0;
-            if (code & GC_FIX) {
-                v = ghost_value_xy(code, BCV);
-            } else {
-                const T m = SA[si];
-                T xp = SA[si + 1], xm = SA[si - 1], yp = SA[si + WS], ym = SA[si - WS];
-                if (code & GC_SWX) { const T q = xp; xp = xm; xm = q; }
-                if (code & GC_SWY) { const T q = yp; yp = ym; ym = q; }
-                k = k_at<T, EXACT, 2>(A, m, xp, xm, yp, ym, (T)0, (T)0);
-                v = axpy<T, EXACT>(a, k, cc);
-            }
-            SB[t] = v;
-            const int ox = lx - R, oy = ly - R;
-            if (ox >= 0 && ox < TX && oy >= 0 && oy < TY) {
-                T& ac = SACC[oy * TX + ox];
-                if (S == 1) ac = EXACT ? k : axpy<T, EXACT>(A.dt6, k, cc);
-                else ac = EXACT ? ac + (T)2 * k : axpy<T, EXACT>(A.dt3, k, ac);
-            }
-        }
-    };
-    using I1 = std::integral_constant<int, 1>;
-    using I2 = std::integral_constant<int, 2>;
-    using I3 = std::integral_constant<int, 3>;
-    const T half = A.a1, full = A.a2;
-    eval_mu(std::integral_constant<int, 7>{}, SC);
-    __syncthreads();
-    eval_state(std::integral_constant<int, 6>{}, I1{}, half);
-    __syncthreads();
-    eval_mu(std::integral_constant<int, 5>{}, SB);
-    __syncthreads();
-    eval_state(std::integral_constant<int, 4>{}, I2{}, half);
-    __syncthreads();
-    eval_mu(I3{}, SB);
-    __syncthreads();
-    eval_state(I2{}, I3{}, full);
-    __syncthreads();
-    eval_mu(I1{}, SB);
-    __syncthreads();
-    // stage 4 on the tile cells
-    constexpr int W1 = TX + 2;
-    for (int t = tid; t < N0; t += 256) {
-        const int oy = t / TX, ox = t - oy * TX;
-        const int x = x0 + ox, y = y0 + oy;
-        if (x >= A.L.n0 || y >= A.L.n1) continue;
-        const int si = (oy + 1) * W1 + ox + 1;
-        const T m = SA[si];
-        const T k = k_at<T, EXACT, 2>(A, m, SA[si + 1], SA[si - 1], SA[si + W1], SA[si - W1], (T)0, (T)0);
-        const long long gi = zb + (long long)(y + g) * A.L.sx + (x + g);
-        if (EXACT) A.out[gi] = SC[(oy + 8) * WC + ox + 8] + A.dt6 * (SACC[t] + k);
-        else A.out[gi] = axpy<T, EXACT>(A.dt6, k, SACC[t]);
-    }
+    // Blocks whose 8-cell ring crosses a NOFLUX or FIXED face need the
+    // per-position ghost rules; every other block (all of them on a periodic
+    // grid: the wrapped load already holds the images) runs the same levels
+    // with those rules compiled out.
+    const bool per = A.bx.klo == PHX_BC_PERIODIC && A.bx.khi == PHX_BC_PERIODIC &&
+                     A.by.klo == PHX_BC_PERIODIC && A.by.khi == PHX_BC_PERIODIC;
+    if (xy_in || per) step2d_levels<T, TX, TY, EXACT, false>(A, SC, SA, SB, SACC, BCV, x0, y0, zb);
+    else step2d_levels<T, TX, TY, EXACT, true>(A, SC, SA, SB, SACC, BCV, x0, y0, zb);
 }
 
 // ---------------------------------------------------------- Euler pieces --
@@ -2356,10 +2417,10 @@ static Launch3 launch_for(const phx_grid* G, int dtype) {
     auto it = g_zc.find(std::make_tuple(G->dim, G->n[0], G->n[1], G->n[2], dtype));
     if (it != g_zc.end()) return it->second;
     // defaults: the launches measured fastest at 1024^2 and 256^3 on MI355X
-    // (FP32 2D: whole-step 32x32; 3D: z-register kernel with 64x16 columns
+    // (2D: whole-step 32x32; 3D: z-register kernel with 64x16 columns
     // per 256-thread workgroup at z-chunk 16 in FP32, 128x16 per 512 threads
     // at z-chunk 32 in FP64); every launch returns the same bits
-    if (G->dim == 2) return Launch3{1, dtype == PHX_F32 && step2d_ok(G) ? 4 : 0};
+    if (G->dim == 2) return Launch3{1, step2d_ok(G) ? 4 : 0};
     if (G->dim != 3) return Launch3{1, 0};
     return dtype == PHX_F32 ? Launch3{16, kFirstZ + 0} : Launch3{32, kFirstZ + 2};
 }
