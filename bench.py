@@ -53,7 +53,9 @@ TILE_NAMES = ["64x16", "64x8", "128x8", "32x16", "64x4", "two-stage 64x8", "two-
               "two-stage 64x4", "zreg 64x16/256", "zreg 128x8/256", "zreg 128x16/512", "zreg 256x8/512",
               "zreg 256x16/1024", "zreg 256x4/256", "zreg 256x8/1024", "zreg 128x16/1024", "zreg 128x8/512"]
 # 2D launch tile indices: the one-stage kernel, then the whole-step kernel tiles
-TILE2_NAMES = ["64x16 one-stage", "whole-step 64x16", "whole-step 64x32", "whole-step 128x16", "whole-step 32x32"]
+TILE2_NAMES = ["64x16 one-stage", "whole-step 64x16", "whole-step 64x32", "whole-step 128x16", "whole-step 32x32",
+               "whole-step 32x32/512", "whole-step 64x32/512", "whole-step 128x32/1024 (FP64: 64x32/512)",
+               "whole-step 32x64/512"]
 
 
 def cpu_model() -> str:

@@ -220,7 +220,8 @@ int phx_ch_rk4_steps_noalloc(const phx_grid* grid, int dtype, void* c,
 
 /* Launch shape of the 3D stage kernels: z-chunk zc (default 16 FP32, 32
  * FP64; default tiles: 3D 9 (z-register 64x16/256) FP32 and 11 (z-register
- * 128x16/512) FP64, 2D the whole-step tile 4 when n >= 8, else 0), per
+ * 128x16/512) FP64; 2D with n >= 8 the whole-step tiles 7 (FP32 on grids
+ * of 256^2 and more) or 5, else 0), per
  * (dim, n, dtype); the planes are split into cdiv(n_z, zc) balanced chunks of
  * floor or ceil(n_z / chunks) planes, one workgroup column each. The
  * autotuner times every tile shape x z-chunk in {4,8,16,32,64} plus the
@@ -240,9 +241,11 @@ int phx_ch_rk4_set_zchunk(const phx_grid* grid, int dtype, int zc);
  * column's z-neighbours in registers and only one plane of the stage state
  * and of mu in LDS, so wide tiles (long contiguous rows in HBM) keep their
  * occupancy; same results bit for bit; any n >= 2. In 2D: 0 is the
- * one-stage kernel (64x16), 1: 64x16, 2: 64x32, 3: 128x16, 4: 32x32 the
- * whole-step kernel (a full RK4 step per launch, 2 field passes; same results
- * bit for bit; needs n >= 8 on both axes). */
+ * one-stage kernel (64x16), 1: 64x16, 2: 64x32, 3: 128x16, 4: 32x32 (256
+ * threads), 5: 32x32, 6: 64x32 (512 threads), 7: 128x32 (1024 threads; FP64
+ * runs 6 instead), 8: 32x64 (512 threads) the whole-step kernel (a full RK4
+ * step per launch, 2 field passes; same results bit for bit; needs n >= 8 on
+ * both axes). */
 int phx_ch_rk4_set_tile(const phx_grid* grid, int dtype, int tile);
 int phx_ch_rk4_get_launch(const phx_grid* grid, int dtype, int* zc, int* tile);
 /* XCD-aware workgroup order of the 3D stage kernel (on by default): each of

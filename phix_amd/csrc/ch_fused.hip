@@ -1475,40 +1475,47 @@ __global__ __launch_bounds__(256) void k_ch_pair_3d(PairArgs<T> A) {
     }
 }
 
-// The eight levels of the 2D whole-step kernel after c_n is in SC. Each
-// level is a fully unrolled loop over the thread's positions t = tid + 256 i
-// of the level's region, in groups of G positions whose LDS operands are all
-// read before any is used (independent chains for the scheduler); the last
-// partial pass reads a valid position and skips the store. GEN: the block
-// needs the ghost rules of NOFLUX / FIXED faces.
-template <typename T, int TX, int TY, bool EXACT, bool GEN>
+// The eight levels of the 2D whole-step kernel after c_n is in SC. Every
+// level lives in one frame: position (x, y) of the block's tile plus its
+// 8-cell ring sits at index (y - y0 + 8) * P + (x - x0 + 8) in SC, SA, SB
+// and SACC alike (P = TX + 16). A level covers whole frame rows -- ring R
+// spans rows 8-R .. 8+TY+R-1, all P columns -- so a thread's positions are
+// f = F0 + tid + NT i and every LDS address is tid plus a constant: no index
+// arithmetic per position. The frame columns outside the ring (at most 16 of
+// P) are evaluated too: they read written, finite entries of the previous
+// level (its rows cover one more row on each side) and feed only other
+// out-of-ring entries, never a ring or tile cell. The last partial pass reads
+// a valid position and skips the store. Positions are handled in groups of
+// G whose LDS operands are read before any is used (independent chains for
+// the scheduler). GEN: the block needs the ghost rules of NOFLUX / FIXED
+// faces.
+template <typename T, int TX, int TY, int NT, bool EXACT, bool GEN>
 __device__ __forceinline__ void step2d_levels(const PairArgs<T>& A, const T* SC, T* SA, T* SB, T* SACC,
                                               const T* BCV, int x0, int y0, long long zb) {
-    constexpr int WC = TX + 16;
-    constexpr int G = sizeof(T) == 8 ? 3 : 5;
+    constexpr int P = TX + 16;
+    constexpr int G = sizeof(T) == 8 ? 3 : 4;
+    static_assert(NT % TX == 0, "tile rows must divide the workgroup");
     const int tid = threadIdx.x;
-    // mu on ring R from the state level on ring R+1 (row width TX+2R+2)
+    // mu on ring R from the state level on ring R+1
     auto eval_mu = [&](auto rr, const T* src) {
         constexpr int R = decltype(rr)::value;
-        constexpr int WS = TX + 2 * R + 2, WD = TX + 2 * R, ND = WD * (TY + 2 * R);
-        constexpr int NIT = (ND + 255) / 256;
+        constexpr int F0 = (8 - R) * P, ND = (TY + 2 * R) * P;
+        constexpr int NIT = (ND + NT - 1) / NT;
 #pragma unroll
         for (int i0 = 0; i0 < NIT; i0 += G) {
             T s0[G], s1[G], s2[G], s3[G], s4[G];
-            int dst[G], code[G];
+            int code[G];
 #pragma unroll
             for (int j = 0; j < G; ++j) {
                 if (i0 + j >= NIT) break;
-                const int t = min(tid + 256 * (i0 + j), ND - 1);
-                const int ly = t / WD, lx = t - ly * WD;
-                const int si = (ly + 1) * WS + lx + 1;
-                dst[j] = tid + 256 * (i0 + j);
-                code[j] = GEN ? ghost_code_xy(x0 - R + lx, y0 - R + ly, A) : 0;
-                s0[j] = src[si];
-                s1[j] = src[si + 1];
-                s2[j] = src[si - 1];
-                s3[j] = src[si + WS];
-                s4[j] = src[si - WS];
+                const int f = F0 + (ND % NT != 0 && i0 + j == NIT - 1 ? min(tid + NT * (i0 + j), ND - 1)
+                                                                           : tid + NT * (i0 + j));
+                code[j] = GEN ? ghost_code_xy(x0 - 8 + f % P, y0 - 8 + f / P, A) : 0;
+                s0[j] = src[f];
+                s1[j] = src[f + 1];
+                s2[j] = src[f - 1];
+                s3[j] = src[f + P];
+                s4[j] = src[f - P];
             }
 #pragma unroll
             for (int j = 0; j < G; ++j) {
@@ -1520,36 +1527,36 @@ __device__ __forceinline__ void step2d_levels(const PairArgs<T>& A, const T* SC,
                 }
                 T v = mu_at<T, EXACT, 2>(A, s0[j], xp, xm, yp, ym, (T)0, (T)0);
                 if (GEN && (code[j] & GC_FIX)) v = ghost_value_xy(code[j], BCV);
-                if (ND % 256 == 0 || i0 + j < NIT - 1 || dst[j] < ND) SA[dst[j]] = v;
+                const int t = tid + NT * (i0 + j);
+                if (ND % NT == 0 || i0 + j < NIT - 1 || t < ND) SA[F0 + t] = v;
             }
         }
     };
     // stage state on ring R from the mu level on ring R+1: c_n + a*k, and the
-    // accumulator update of stage S on the tile cells
+    // accumulator update of stage S (kept on every position of the ring: only
+    // the tile cells' entries are read)
     auto eval_state = [&](auto rr, auto ss, T a) {
         constexpr int R = decltype(rr)::value;
         constexpr int S = decltype(ss)::value;
-        constexpr int WS = TX + 2 * R + 2, WD = TX + 2 * R, ND = WD * (TY + 2 * R);
-        constexpr int NIT = (ND + 255) / 256;
+        constexpr int F0 = (8 - R) * P, ND = (TY + 2 * R) * P;
+        constexpr int NIT = (ND + NT - 1) / NT;
 #pragma unroll
         for (int i0 = 0; i0 < NIT; i0 += G) {
-            T s0[G], s1[G], s2[G], s3[G], s4[G], cc[G];
-            int dst[G], code[G], lxy[G];
+            T s0[G], s1[G], s2[G], s3[G], s4[G], cc[G], ac[G];
+            int code[G];
 #pragma unroll
             for (int j = 0; j < G; ++j) {
                 if (i0 + j >= NIT) break;
-                const int t = min(tid + 256 * (i0 + j), ND - 1);
-                const int ly = t / WD, lx = t - ly * WD;
-                const int si = (ly + 1) * WS + lx + 1;
-                dst[j] = tid + 256 * (i0 + j);
-                lxy[j] = ((ly - R) << 16) | (lx - R + 0x8000);
-                code[j] = GEN ? ghost_code_xy(x0 - R + lx, y0 - R + ly, A) : 0;
-                cc[j] = SC[(ly + 8 - R) * WC + lx + 8 - R];
-                s0[j] = SA[si];
-                s1[j] = SA[si + 1];
-                s2[j] = SA[si - 1];
-                s3[j] = SA[si + WS];
-                s4[j] = SA[si - WS];
+                const int f = F0 + (ND % NT != 0 && i0 + j == NIT - 1 ? min(tid + NT * (i0 + j), ND - 1)
+                                                                           : tid + NT * (i0 + j));
+                code[j] = GEN ? ghost_code_xy(x0 - 8 + f % P, y0 - 8 + f / P, A) : 0;
+                cc[j] = SC[f];
+                ac[j] = S == 1 ? (T)0 : SACC[f];
+                s0[j] = SA[f];
+                s1[j] = SA[f + 1];
+                s2[j] = SA[f - 1];
+                s3[j] = SA[f + P];
+                s4[j] = SA[f - P];
             }
 #pragma unroll
             for (int j = 0; j < G; ++j) {
@@ -1565,13 +1572,13 @@ __device__ __forceinline__ void step2d_levels(const PairArgs<T>& A, const T* SC,
                     v = ghost_value_xy(code[j], BCV);
                     k = (T)0;
                 }
-                const bool act = ND % 256 == 0 || i0 + j < NIT - 1 || dst[j] < ND;
-                if (act) SB[dst[j]] = v;
-                const int ox = (lxy[j] & 0xffff) - 0x8000, oy = lxy[j] >> 16;
-                if (act && ox >= 0 && ox < TX && oy >= 0 && oy < TY) {
-                    T& ac = SACC[oy * TX + ox];
-                    if (S == 1) ac = EXACT ? k : axpy<T, EXACT>(A.dt6, k, cc[j]);
-                    else ac = EXACT ? ac + (T)2 * k : axpy<T, EXACT>(A.dt3, k, ac);
+                T nac;
+                if (S == 1) nac = EXACT ? k : axpy<T, EXACT>(A.dt6, k, cc[j]);
+                else nac = EXACT ? ac[j] + (T)2 * k : axpy<T, EXACT>(A.dt3, k, ac[j]);
+                const int t = tid + NT * (i0 + j);
+                if (ND % NT == 0 || i0 + j < NIT - 1 || t < ND) {
+                    SB[F0 + t] = v;
+                    SACC[F0 + t] = nac;
                 }
             }
         }
@@ -1594,20 +1601,21 @@ __device__ __forceinline__ void step2d_levels(const PairArgs<T>& A, const T* SC,
     __syncthreads();
     eval_mu(I1{}, SB);
     __syncthreads();
-    // stage 4 on the tile cells
-    constexpr int W1 = TX + 2;
-    constexpr int N0 = TX * TY, NIT = (N0 + 255) / 256;
+    // stage 4 on the tile cells: thread tid owns column tid % TX of rows
+    // tid / TX + (NT / TX) j
+    constexpr int RS = NT / TX, NIT = TY / RS;
+    static_assert(TY % RS == 0, "tile rows must be a multiple of NT / TX");
+    const int ox = tid % TX, oy0 = tid / TX;
+    const int fb = (8 + oy0) * P + 8 + ox;
+    const int x = x0 + ox;
+    const long long gb = zb + (long long)(y0 + oy0 + A.L.g) * A.L.sx + (x + A.L.g);
 #pragma unroll
-    for (int i = 0; i < NIT; ++i) {
-        const int t = tid + 256 * i;
-        if (N0 % 256 != 0 && t >= N0) break;
-        const int oy = t / TX, ox = t - oy * TX;
-        const int x = x0 + ox, y = y0 + oy;
-        const int si = (oy + 1) * W1 + ox + 1;
-        const T m = SA[si];
-        const T k = k_at<T, EXACT, 2>(A, m, SA[si + 1], SA[si - 1], SA[si + W1], SA[si - W1], (T)0, (T)0);
-        const T r = EXACT ? SC[(oy + 8) * WC + ox + 8] + A.dt6 * (SACC[t] + k) : axpy<T, EXACT>(A.dt6, k, SACC[t]);
-        if (x < A.L.n0 && y < A.L.n1) A.out[zb + (long long)(y + A.L.g) * A.L.sx + (x + A.L.g)] = r;
+    for (int j = 0; j < NIT; ++j) {
+        const int f = fb + j * RS * P;
+        const T m = SA[f];
+        const T k = k_at<T, EXACT, 2>(A, m, SA[f + 1], SA[f - 1], SA[f + P], SA[f - P], (T)0, (T)0);
+        const T r = EXACT ? SC[f] + A.dt6 * (SACC[f] + k) : axpy<T, EXACT>(A.dt6, k, SACC[f]);
+        if (x < A.L.n0 && y0 + oy0 + j * RS < A.L.n1) A.out[gb + (long long)j * RS * A.L.sx] = r;
     }
 }
 
@@ -1627,16 +1635,14 @@ __device__ __forceinline__ void step2d_levels(const PairArgs<T>& A, const T* SC,
 // SB = stage states 2..4, SACC = accumulator. A level reads the other buffer
 // and is separated by a barrier from the level that last read the buffer it
 // writes.
-template <typename T, int TX, int TY, bool EXACT>
-__global__ __launch_bounds__(256) void k_ch_step_2d(PairArgs<T> A) {
+template <typename T, int TX, int TY, int NT, bool EXACT>
+__global__ __launch_bounds__(NT) void k_ch_step_2d(PairArgs<T> A) {
     constexpr int WC = TX + 16, NC = WC * (TY + 16);
-    constexpr int NA = (TX + 14) * (TY + 14);
-    constexpr int NB = (TX + 12) * (TY + 12);
-    constexpr int N0 = TX * TY;
+    // every level in the same frame as c_n (see step2d_levels)
     __shared__ T SC[NC];
-    __shared__ T SA[NA];
-    __shared__ T SB[NB];
-    __shared__ T SACC[N0];
+    __shared__ T SA[NC];
+    __shared__ T SB[NC];
+    __shared__ T SACC[NC];
     __shared__ T BCV[6];
     const int tid = threadIdx.x;
     stage_bc_values(BCV, A);
@@ -1656,29 +1662,40 @@ __global__ __launch_bounds__(256) void k_ch_step_2d(PairArgs<T> A) {
     // c_n on the tile and an 8-cell ring, x/y BC images applied: offsets
     // first, then all loads back to back (no branch between them, so their
     // latencies overlap); FIXED and surplus slots read a valid cell and are
-    // replaced at the store.
-    constexpr int NSC = (NC + 255) / 256;
+    // replaced at the store. Blocks whose ring lies inside the domain read
+    // at affine offsets (no BC maps).
+    constexpr int NSC = (NC + NT - 1) / NT;
     {
         const int cell0 = g * A.L.sx + g;
         int off[NSC];
         unsigned long long fm = 0;
+        if (xy_in) {
+            const int base = (y0 - 8 + g) * A.L.sx + (x0 - 8 + g);
 #pragma unroll
-        for (int i = 0; i < NSC; ++i) {
-            const int t = min(tid + 256 * i, NC - 1);
-            const int ly = t / WC, lx = t - ly * WC;
-            const int px = x0 - 8 + lx, py = y0 - 8 + ly;
-            const int qy = map_deep<T, 8>(py, A.by), qx = map_deep<T, 8>(px, A.bx);
-            off[i] = cell0;
-            if (qy < 0 || qx < 0) fm |= 1ull << i;
-            else off[i] = (qy + g) * A.L.sx + (qx + g);
+            for (int i = 0; i < NSC; ++i) {
+                const int t = min(tid + NT * i, NC - 1);
+                const int ly = t / WC, lx = t - ly * WC;
+                off[i] = base + ly * A.L.sx + lx;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < NSC; ++i) {
+                const int t = min(tid + NT * i, NC - 1);
+                const int ly = t / WC, lx = t - ly * WC;
+                const int px = x0 - 8 + lx, py = y0 - 8 + ly;
+                const int qy = map_deep<T, 8>(py, A.by), qx = map_deep<T, 8>(px, A.bx);
+                off[i] = cell0;
+                if (qy < 0 || qx < 0) fm |= 1ull << i;
+                else off[i] = (qy + g) * A.L.sx + (qx + g);
+            }
         }
         T r[NSC];
 #pragma unroll
         for (int i = 0; i < NSC; ++i) r[i] = A.u[zb + off[i]];
 #pragma unroll
         for (int i = 0; i < NSC; ++i) {
-            const int t = tid + 256 * i;
-            if (t < NC) {
+            const int t = tid + NT * i;
+            if (NC % NT == 0 || t < NC) {
                 T v = r[i];
                 if ((fm >> i) & 1ull) {
                     const int ly = t / WC, lx = t - ly * WC;
@@ -1698,8 +1715,8 @@ __global__ __launch_bounds__(256) void k_ch_step_2d(PairArgs<T> A) {
     // with those rules compiled out.
     const bool per = A.bx.klo == PHX_BC_PERIODIC && A.bx.khi == PHX_BC_PERIODIC &&
                      A.by.klo == PHX_BC_PERIODIC && A.by.khi == PHX_BC_PERIODIC;
-    if (xy_in || per) step2d_levels<T, TX, TY, EXACT, false>(A, SC, SA, SB, SACC, BCV, x0, y0, zb);
-    else step2d_levels<T, TX, TY, EXACT, true>(A, SC, SA, SB, SACC, BCV, x0, y0, zb);
+    if (xy_in || per) step2d_levels<T, TX, TY, NT, EXACT, false>(A, SC, SA, SB, SACC, BCV, x0, y0, zb);
+    else step2d_levels<T, TX, TY, NT, EXACT, true>(A, SC, SA, SB, SACC, BCV, x0, y0, zb);
 }
 
 // ---------------------------------------------------------- Euler pieces --
@@ -2019,18 +2036,31 @@ static bool pair_ok(const phx_grid* G) { return G->dim == 3 && G->n[0] >= 4 && G
 
 // 2D whole-step kernel tiles; launch tile indices 1.. select them in 2D
 // (0 is the one-stage kernel).
-static const Tile3 kTilesS2[] = {{64, 16}, {64, 32}, {128, 16}, {32, 32}};
+static const TileZ kTilesS2[] = {{64, 16, 256}, {64, 32, 256}, {128, 16, 256}, {32, 32, 256},
+                                 {32, 32, 512}, {64, 32, 512}, {128, 32, 1024}, {32, 64, 512}};
 constexpr int kNumTilesS2 = sizeof(kTilesS2) / sizeof(kTilesS2[0]);
 
 template <typename T, bool EXACT>
 static void launch_step2d(int tile, dim3 grid, const PairArgs<T>& P, hipStream_t st) {
     switch (tile) {
-        case 0: hipLaunchKernelGGL((k_ch_step_2d<T, 64, 16, EXACT>), grid, dim3(256), 0, st, P); break;
-        case 1: hipLaunchKernelGGL((k_ch_step_2d<T, 64, 32, EXACT>), grid, dim3(256), 0, st, P); break;
-        case 2: hipLaunchKernelGGL((k_ch_step_2d<T, 128, 16, EXACT>), grid, dim3(256), 0, st, P); break;
-        default: hipLaunchKernelGGL((k_ch_step_2d<T, 32, 32, EXACT>), grid, dim3(256), 0, st, P); break;
+        case 0: hipLaunchKernelGGL((k_ch_step_2d<T, 64, 16, 256, EXACT>), grid, dim3(256), 0, st, P); break;
+        case 1: hipLaunchKernelGGL((k_ch_step_2d<T, 64, 32, 256, EXACT>), grid, dim3(256), 0, st, P); break;
+        case 2: hipLaunchKernelGGL((k_ch_step_2d<T, 128, 16, 256, EXACT>), grid, dim3(256), 0, st, P); break;
+        case 3: hipLaunchKernelGGL((k_ch_step_2d<T, 32, 32, 256, EXACT>), grid, dim3(256), 0, st, P); break;
+        case 4: hipLaunchKernelGGL((k_ch_step_2d<T, 32, 32, 512, EXACT>), grid, dim3(512), 0, st, P); break;
+        case 5: hipLaunchKernelGGL((k_ch_step_2d<T, 64, 32, 512, EXACT>), grid, dim3(512), 0, st, P); break;
+        case 6:
+            // FP32 only: 128x32 in FP64 would need 221 KB of LDS (step2d_tile maps it to 5)
+            if constexpr (sizeof(T) == 4)
+                hipLaunchKernelGGL((k_ch_step_2d<T, 128, 32, 1024, EXACT>), grid, dim3(1024), 0, st, P);
+            break;
+        default: hipLaunchKernelGGL((k_ch_step_2d<T, 32, 64, 512, EXACT>), grid, dim3(512), 0, st, P); break;
     }
 }
+
+// Whole-step tile actually launched for tile index stile: 128x32 (6) needs
+// more LDS than a CU has in FP64 and runs as 64x32 (5) there.
+static int step2d_tile(int stile, size_t esz) { return stile == 6 && esz == 8 ? 5 : stile; }
 
 // The 2D whole-step kernel needs depth-8 BC images inside the domain.
 static bool step2d_ok(const phx_grid* G) { return G->dim == 2 && G->n[0] >= 8 && G->n[1] >= 8; }
@@ -2334,7 +2364,8 @@ static int rk4_step2d_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_par
     P.c = nullptr;
     P.acc = nullptr;
     P.zc = 1;
-    const Tile3 tl = kTilesS2[stile];
+    stile = step2d_tile(stile, sizeof(T));
+    const TileZ tl = kTilesS2[stile];
     const dim3 grid(cdiv(L.n0, tl.tx), cdiv(L.n1, tl.ty), 1);
     P.xcd_remap = g_xcd_remap && (grid.x * grid.y) % 8 == 0;
     auto step = [&](const T* x, T* y) {
@@ -2417,10 +2448,15 @@ static Launch3 launch_for(const phx_grid* G, int dtype) {
     auto it = g_zc.find(std::make_tuple(G->dim, G->n[0], G->n[1], G->n[2], dtype));
     if (it != g_zc.end()) return it->second;
     // defaults: the launches measured fastest at 1024^2 and 256^3 on MI355X
-    // (2D: whole-step 32x32; 3D: z-register kernel with 64x16 columns
-    // per 256-thread workgroup at z-chunk 16 in FP32, 128x16 per 512 threads
-    // at z-chunk 32 in FP64); every launch returns the same bits
-    if (G->dim == 2) return Launch3{1, step2d_ok(G) ? 4 : 0};
+    // (2D: whole-step 128x32 per 1024 threads in FP32 on grids of 256^2 and
+    // more, 32x32 per 512 threads otherwise and in FP64; 3D: z-register
+    // kernel with 64x16 columns per 256-thread workgroup at z-chunk 16 in
+    // FP32, 128x16 per 512 threads at z-chunk 32 in FP64); every launch
+    // returns the same bits
+    if (G->dim == 2) {
+        if (!step2d_ok(G)) return Launch3{1, 0};
+        return Launch3{1, dtype == PHX_F32 && G->n[0] >= 256 && G->n[1] >= 256 ? 7 : 5};
+    }
     if (G->dim != 3) return Launch3{1, 0};
     return dtype == PHX_F32 ? Launch3{16, kFirstZ + 0} : Launch3{32, kFirstZ + 2};
 }

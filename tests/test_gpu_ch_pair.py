@@ -178,7 +178,7 @@ def test_launch_timing_records_kinds(dev):
 
 
 # ------------------------------------------------------ 2D whole-step kernel
-STEP2D_TILES = (1, 2, 3, 4)
+STEP2D_TILES = (1, 2, 3, 4, 5, 6, 7, 8)
 CASES2 = [
     # n, d, ghost, bc, values
     ((64, 48), (1.0, 1.0), 1, P, 0.0),
@@ -246,7 +246,7 @@ def test_step2d_full_size_1024_fp32_equals_one_stage(dev):
 def test_step2d_tile_rejected_on_small_grid(dev):
     g = capi.make_grid((7, 64), bc=P)
     assert capi.lib().phx_ch_rk4_set_tile(C.byref(g), capi.F32, 1) == capi.PHX_ERR_UNSUPPORTED
-    assert capi.lib().phx_ch_rk4_set_tile(C.byref(g), capi.F32, 5) == capi.PHX_ERR_INVALID
+    assert capi.lib().phx_ch_rk4_set_tile(C.byref(g), capi.F32, 9) == capi.PHX_ERR_INVALID
     g1 = capi.make_grid((64,), bc=P)
     assert capi.lib().phx_ch_rk4_set_tile(C.byref(g1), capi.F32, 1) == capi.PHX_ERR_INVALID
 
