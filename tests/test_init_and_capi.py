@@ -83,18 +83,20 @@ def test_launch_table_defaults_and_ranges():
     assert _launch(g3, capi.F32) == (16, 9)     # z-register 64x16, 256 threads
     assert _launch(g3, capi.F64) == (32, 11)
     g2 = capi.make_grid((1024, 1024), bc=capi.PERIODIC)
-    assert _launch(g2, capi.F32) == (1, 4)      # whole-step 32x32
-    assert _launch(g2, capi.F64) == (1, 0)      # one-stage
+    assert _launch(g2, capi.F32) == (1, 7)      # whole-step 128x32, 1024 threads
+    assert _launch(g2, capi.F64) == (1, 5)      # whole-step 32x32, 512 threads
+    g2m = capi.make_grid((128, 200), bc=capi.PERIODIC)
+    assert _launch(g2m, capi.F32) == (1, 5)
     g2s = capi.make_grid((7, 64), bc=capi.PERIODIC)
     assert _launch(g2s, capi.F32) == (1, 0)     # too small for the whole-step kernel
     g1 = capi.make_grid((64,), bc=capi.PERIODIC)
     assert _launch(g1, capi.F32)[1] == 0
-    # ranges: 3D 0-17, 2D 0-4, 1D 0
+    # ranges: 3D 0-17, 2D 0-8, 1D 0
     for t in range(18):
         assert lib.phx_ch_rk4_set_tile(C.byref(g3), capi.F32, t) == 0
     assert lib.phx_ch_rk4_set_tile(C.byref(g3), capi.F32, 18) == capi.PHX_ERR_INVALID
     assert lib.phx_ch_rk4_set_tile(C.byref(g3), capi.F32, -1) == capi.PHX_ERR_INVALID
-    assert lib.phx_ch_rk4_set_tile(C.byref(g2), capi.F32, 5) == capi.PHX_ERR_INVALID
+    assert lib.phx_ch_rk4_set_tile(C.byref(g2), capi.F32, 9) == capi.PHX_ERR_INVALID
     assert lib.phx_ch_rk4_set_tile(C.byref(g1), capi.F32, 1) == capi.PHX_ERR_INVALID
     # the two-stage tiles need n >= 4 per axis, the whole-step tiles n >= 8
     small3 = capi.make_grid((3, 16, 16), bc=capi.NOFLUX)
