@@ -219,9 +219,9 @@ int phx_ch_rk4_steps_noalloc(const phx_grid* grid, int dtype, void* c,
                              const phx_ch_params* p, int nsteps, phx_stream stream);
 
 /* Launch shape of the 3D stage kernels: z-chunk zc (default 16 FP32, 32
- * FP64; default tiles: 3D 9 (z-register 64x16/256) FP32 and 11 (z-register
- * 128x16/512) FP64; 2D with n >= 8 the whole-step tiles 7 (FP32 on grids
- * of 256^2 and more) or 5, else 0), per
+ * FP64; default tiles: 3D 9 (z-register 64x16/256) FP32 and per stage
+ * class in FP64 (phx_ch_rk4_set_stage_launch); 2D with n >= 8 the
+ * whole-step tiles 7 (FP32 on grids of 256^2 and more) or 5, else 0), per
  * (dim, n, dtype); the planes are split into cdiv(n_z, zc) balanced chunks of
  * floor or ceil(n_z / chunks) planes, one workgroup column each. The
  * autotuner times every tile shape x z-chunk in {4,8,16,32,64} plus the
@@ -262,9 +262,21 @@ int phx_ch_set_xcd_remap(int on);
  * phx_ch_get_fp64_exact returns the current form. */
 int phx_ch_set_fp64_exact(int on);
 int phx_ch_get_fp64_exact(void);
+/* The autotuner also refines each stage class of a winning 3D one-stage
+ * launch (see phx_ch_rk4_set_stage_launch). */
 int phx_ch_rk4_autotune(const phx_grid* grid, int dtype, void* c, void* work,
                         const phx_ch_params* p, int steps, int* best_zc,
                         double* best_ms_per_step, phx_stream stream);
+/* Launch of one stage class of the 3D one-stage schedule (no reference
+ * counterpart): cls 0 = stage 1, 1 = stages 2 and 3, 2 = stage 4; tile a
+ * one-stage tile (0-4 or 9-17), z-chunk zc. Every one-stage kernel returns
+ * the same bits, so classes may mix launches. phx_ch_rk4_set_tile/_set_zchunk
+ * set all three classes; phx_ch_rk4_get_launch reports class 1, whose tile
+ * also chooses the schedule (a two-stage tile there runs every stage
+ * two-stage). Default in 3D FP32: tile 9 at zc 16 for every class; FP64:
+ * class 0 tile 9 at 12 z-chunks, classes 1 and 2 tile 17 at 8 and 4 z-chunks. */
+int phx_ch_rk4_set_stage_launch(const phx_grid* grid, int dtype, int cls, int tile, int zc);
+int phx_ch_rk4_get_stage_launch(const phx_grid* grid, int dtype, int cls, int* zc, int* tile);
 
 /* Benchmark timing (no reference counterpart): while on, each launch of the
  * fused RK4 kernels is bracketed by HIP events on its stream. phx_ch_timing(1)

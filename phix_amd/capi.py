@@ -135,6 +135,8 @@ SIG = {
     "phx_ch_get_fp64_exact": (C.c_int, []),
     "phx_ch_rk4_get_launch": (C.c_int, [GP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "phx_ch_rk4_set_tile": (C.c_int, [GP, C.c_int, C.c_int]),
+    "phx_ch_rk4_set_stage_launch": (C.c_int, [GP, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "phx_ch_rk4_get_stage_launch": (C.c_int, [GP, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "phx_ch_rk4_autotune": (C.c_int, [GP, C.c_int, VP, VP, C.POINTER(CHParams), C.c_int, C.POINTER(C.c_int),
                                       C.POINTER(C.c_double), VP]),
     "phx_ch_timing": (C.c_int, [C.c_int]),

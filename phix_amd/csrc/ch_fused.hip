@@ -2158,9 +2158,13 @@ static int blocks_per_cu(int tile) {
     return nb;
 }
 
+// One-stage RK4 steps. In 3D each stage class -- 0: stage 1, 1: stages 2
+// and 3, 2: stage 4 -- may run its own one-stage launch (tile, z-chunk):
+// every one-stage kernel reads and writes the same stage buffers and returns
+// the same bits, so any mix gives the results of a uniform launch.
 template <typename T, bool EXACT>
 static int rk4_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_params* p, int nsteps, hipStream_t st,
-                       int zc, int tile, int skip_fill = 0) {
+                       const int (&zc)[3], const int (&tile)[3], int skip_fill = 0) {
     CHArgs<T> A;
     A.L = make_layout(G);
     const long long N = A.L.total();
@@ -2180,7 +2184,7 @@ static int rk4_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_params* p,
     A.s2 = acc;
     A.s3 = w0;
     A.third = (T)(1.0 / 3.0);
-    A.zc = zc;
+    A.zc = zc[1];
     A.xcd_remap = 0;
     const T shift[4] = {(T)(dt * 0.5), (T)(dt * 0.5), (T)dt, (T)0};
     // FP64 and 2D: stage states in w0, w1, w0 and the accumulator in acc;
@@ -2188,15 +2192,17 @@ static int rk4_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_params* p,
     const bool states = G->dim == 3 && !EXACT;
     const T* uin[4] = {c, states ? acc : w0, states ? w0 : w1, states ? w1 : w0};
     T* uout[4] = {states ? acc : w0, states ? w0 : w1, states ? w1 : w0, c};
-    dim3 grid;
-    if (G->dim == 3) {
-        const Tile3 tl = shape3(tile);
-        grid = dim3(cdiv(A.L.n0, tl.tx), cdiv(A.L.n1, tl.ty), cdiv(A.L.n2, zc));
-        A.xcd_remap = g_xcd_remap && (grid.x * grid.y * grid.z) % 8 == 0;
-    }
-    else {
-        grid = dim3(cdiv(A.L.n0, T2X), cdiv(A.L.n1, T2Y), 1);
-        A.xcd_remap = g_xcd_remap && (grid.x * grid.y) % 8 == 0;
+    dim3 grid[3];
+    int remap[3];
+    for (int k = 0; k < 3; ++k) {
+        if (G->dim == 3) {
+            const Tile3 tl = shape3(tile[k]);
+            grid[k] = dim3(cdiv(A.L.n0, tl.tx), cdiv(A.L.n1, tl.ty), cdiv(A.L.n2, zc[k]));
+            remap[k] = g_xcd_remap && (grid[k].x * grid[k].y * grid[k].z) % 8 == 0;
+        } else {
+            grid[k] = dim3(cdiv(A.L.n0, T2X), cdiv(A.L.n1, T2Y), 1);
+            remap[k] = g_xcd_remap && (grid[k].x * grid[k].y) % 8 == 0;
+        }
     }
     for (int s = 0; s < nsteps; ++s) {
         if (s == nsteps - 1 && !skip_fill) {
@@ -2204,19 +2210,31 @@ static int rk4_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_params* p,
             if (rc) return rc;
         }
         for (int stg = 0; stg < 4; ++stg) {
+            const int k = stg == 0 ? 0 : (stg == 3 ? 2 : 1);
+            const dim3 gr = grid[k];
             A.stage = stg + 1;
             A.u = uin[stg];
             A.out = uout[stg];
             A.a = shift[stg];
+            A.zc = zc[k];
+            A.xcd_remap = remap[k];
             lt_begin(st);
-            if (G->dim == 3) launch3<T, EXACT>(tile, grid, A, st);
-            else if (G->dim == 2) hipLaunchKernelGGL((k_ch_stage_2d<T, 2, T2X, T2Y, EXACT>), grid, dim3(256), 0, st, A);
-            else hipLaunchKernelGGL((k_ch_stage_2d<T, 1, T2X, T2Y, EXACT>), grid, dim3(256), 0, st, A);
+            if (G->dim == 3) launch3<T, EXACT>(tile[k], gr, A, st);
+            else if (G->dim == 2) hipLaunchKernelGGL((k_ch_stage_2d<T, 2, T2X, T2Y, EXACT>), gr, dim3(256), 0, st, A);
+            else hipLaunchKernelGGL((k_ch_stage_2d<T, 1, T2X, T2Y, EXACT>), gr, dim3(256), 0, st, A);
             lt_end(0, st);
         }
     }
     PHX_LAUNCH_CHECK("phx_ch_rk4_steps");
     return PHX_OK;
+}
+
+// The same with one launch for every stage.
+template <typename T, bool EXACT>
+static int rk4_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_params* p, int nsteps, hipStream_t st,
+                       int zc, int tile, int skip_fill = 0) {
+    const int z3[3] = {zc, zc, zc}, t3[3] = {tile, tile, tile};
+    return rk4_steps_t<T, EXACT>(G, c, work, p, nsteps, st, z3, t3, skip_fill);
 }
 
 // RK4 steps with the two-stage kernel (3D, tile index ptile into kTilesP).
@@ -2438,8 +2456,13 @@ size_t phx_ch_rk4_workspace_bytes(const phx_grid* G, int dtype) {
 // planes streamed per workgroup. Set by phx_ch_rk4_set_tile/_set_zchunk or the
 // autotuner; defaults below.
 struct Launch3 {
-    int zc, tile;
+    int zc, tile;          // stages 2 and 3; the schedule (two-stage, whole step, 2D) follows this tile
+    int zc1, tile1;        // stage 1 (3D one-stage schedules only)
+    int zc4, tile4;        // stage 4 (3D one-stage schedules only)
 };
+static Launch3 uniform_launch(int zc, int tile) { return Launch3{zc, tile, zc, tile, zc, tile}; }
+// one-stage 3D tiles (LDS ring or z-register): the ones a stage class may mix
+static bool one_stage_tile(int tile) { return tile < kNumTiles3 || tile >= kFirstZ; }
 static std::mutex g_zc_mu;
 static std::map<std::tuple<int, int, int, int, int>, Launch3> g_zc;
 
@@ -2454,11 +2477,21 @@ static Launch3 launch_for(const phx_grid* G, int dtype) {
     // FP32, 128x16 per 512 threads at z-chunk 32 in FP64); every launch
     // returns the same bits
     if (G->dim == 2) {
-        if (!step2d_ok(G)) return Launch3{1, 0};
-        return Launch3{1, dtype == PHX_F32 && G->n[0] >= 256 && G->n[1] >= 256 ? 7 : 5};
+        if (!step2d_ok(G)) return uniform_launch(1, 0);
+        return uniform_launch(1, dtype == PHX_F32 && G->n[0] >= 256 && G->n[1] >= 256 ? 7 : 5);
     }
-    if (G->dim != 3) return Launch3{1, 0};
-    return dtype == PHX_F32 ? Launch3{16, kFirstZ + 0} : Launch3{32, kFirstZ + 2};
+    if (G->dim != 3) return uniform_launch(1, 0);
+    if (dtype == PHX_F32) return uniform_launch(16, kFirstZ + 0);
+    // FP64, per stage class (256^3 measurement): stage 1 64x16/256 at 12
+    // chunks, stages 2-3 128x8/512 at 8 chunks, stage 4 128x8/512 at 4 chunks
+    Launch3 l;
+    l.tile1 = kFirstZ + 0;
+    l.zc1 = (int)cdiv(G->n[2], 12);
+    l.tile = kFirstZ + 8;
+    l.zc = (int)cdiv(G->n[2], 8);
+    l.tile4 = kFirstZ + 8;
+    l.zc4 = (int)cdiv(G->n[2], 4);
+    return l;
 }
 
 static void set_launch(const phx_grid* G, int dtype, Launch3 l) {
@@ -2476,7 +2509,7 @@ int phx_ch_rk4_set_zchunk(const phx_grid* G, int dtype, int zc) {
     if (rc) return rc;
     if (zc < 1 || zc > 4096) return fail(PHX_ERR_INVALID, "phx_ch_rk4_set_zchunk: zc must be in [1, 4096]");
     Launch3 l = launch_for(G, dtype);
-    l.zc = zc;
+    l.zc = l.zc1 = l.zc4 = zc;
     set_launch(G, dtype, l);
     return PHX_OK;
 }
@@ -2502,7 +2535,7 @@ int phx_ch_rk4_set_tile(const phx_grid* G, int dtype, int tile) {
     if (G->dim == 2 && tile >= 1 && !step2d_ok(G))
         return fail(PHX_ERR_UNSUPPORTED, "phx_ch_rk4_set_tile: whole-step 2D tiles need n >= 8 on both axes");
     Launch3 l = launch_for(G, dtype);
-    l.tile = tile;
+    l.tile = l.tile1 = l.tile4 = tile;
     set_launch(G, dtype, l);
     return PHX_OK;
 }
@@ -2528,8 +2561,16 @@ static int run_rk4_t(const phx_grid* G, int dtype, void* c, void* work, const ph
                                     : rk4_pair_steps_t<float, false>(G, (float*)c, (float*)work, p, nsteps, st,
                                                                      l.zc, l.tile - kNumTiles3, pure);
     }
-    return dtype == PHX_F64 ? rk4_steps_t<double, X64>(G, (double*)c, (double*)work, p, nsteps, st, l.zc, l.tile)
-                            : rk4_steps_t<float, false>(G, (float*)c, (float*)work, p, nsteps, st, l.zc, l.tile);
+    int z3[3] = {l.zc1, l.zc, l.zc4}, t3[3] = {l.tile1, l.tile, l.tile4};
+    for (int k = 0; k < 3; ++k) {
+        // a class keeps its own launch only in a 3D one-stage schedule
+        if (G->dim != 3 || !one_stage_tile(t3[k]) || (t3[k] >= kFirstZ && !zreg_ok(G, dtype))) {
+            z3[k] = l.zc;
+            t3[k] = l.tile;
+        }
+    }
+    return dtype == PHX_F64 ? rk4_steps_t<double, X64>(G, (double*)c, (double*)work, p, nsteps, st, z3, t3)
+                            : rk4_steps_t<float, false>(G, (float*)c, (float*)work, p, nsteps, st, z3, t3);
 }
 
 // Schedule by the launch's tile index; `pure` runs nsteps (even) two-stage or
@@ -2618,8 +2659,10 @@ extern "C" int phx_ch_rk4_steps_noalloc(const phx_grid* G, int dtype, void* c, c
 // one-stage kernel (5 tiles), the two-stage kernel (4 tiles) and the
 // z-register kernel (9 tiles) at z-chunks {4, 8, 16, 32, 64} plus the chunk
 // counts that fill whole waves of resident workgroups; in 2D the one-stage
-// kernel and the whole-step kernel (4 tiles). Two-stage and whole-step
-// candidates run 2*ceil(steps/2) steps.
+// kernel and the whole-step kernels. Two-stage and whole-step candidates run
+// 2*ceil(steps/2) steps. When a 3D one-stage launch wins, each stage class
+// (stage 1, stages 2-3, stage 4) is then refined in turn over the one-stage
+// candidates with the other classes held.
 int phx_ch_rk4_autotune(const phx_grid* G, int dtype, void* c, void* work, const phx_ch_params* p, int steps,
                         int* best_zc, double* best_ms_per_step, phx_stream s) {
     int rc = check_grid_fused(G, "phx_ch_rk4_autotune");
@@ -2627,23 +2670,23 @@ int phx_ch_rk4_autotune(const phx_grid* G, int dtype, void* c, void* work, const
     if (steps < 1) return fail(PHX_ERR_INVALID, "phx_ch_rk4_autotune: steps < 1");
     hipStream_t st = as_stream(s);
     const int cand[] = {4, 8, 16, 32, 64};
-    float best = 1e30f;
-    int bz = 16, bt = 0;
-    hipEvent_t e0, e1;
-    PHX_HIP(hipEventCreate(&e0));
-    PHX_HIP(hipEventCreate(&e1));
     const int ntiles = G->dim == 3 ? kFirstZ + kNumTilesZ : (G->dim == 2 && step2d_ok(G) ? 1 + kNumTilesS2 : 1);
     const int psteps = 2 * ((steps + 1) / 2);
     int dev = 0, ncu = 0;
     PHX_HIP(hipGetDevice(&dev));
     PHX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    // candidates (zc, tile)
+    std::vector<std::pair<int, int>> cands;
     for (int tile = 0; tile < ntiles; ++tile) {
-    if (G->dim == 3 && is_pair_tile(tile) && !pair_ok(G)) continue;
-    if (G->dim == 3 && tile >= kFirstZ && !zreg_ok(G, dtype)) continue;
-    // z-chunks: fixed sizes, plus chunk counts that fill whole waves of the
-    // resident workgroup slots (no partly filled last wave)
-    std::vector<int> zcs(cand, cand + sizeof(cand) / sizeof(cand[0]));
-    if (G->dim == 3) {
+        if (G->dim == 3 && is_pair_tile(tile) && !pair_ok(G)) continue;
+        if (G->dim == 3 && tile >= kFirstZ && !zreg_ok(G, dtype)) continue;
+        if (G->dim != 3) {
+            cands.emplace_back(1, tile);
+            continue;
+        }
+        // z-chunks: fixed sizes, plus chunk counts that fill whole waves of
+        // the resident workgroup slots (no partly filled last wave)
+        std::vector<int> zcs(cand, cand + sizeof(cand) / sizeof(cand[0]));
         const Tile3 tl = shape3(tile);
         const long long ncols = (long long)cdiv(G->n[0], tl.tx) * cdiv(G->n[1], tl.ty);
         const int bpc = dtype == PHX_F32 ? blocks_per_cu<float, false>(tile)
@@ -2653,33 +2696,89 @@ int phx_ch_rk4_autotune(const phx_grid* G, int dtype, void* c, void* work, const
             const int zc = (int)cdiv(G->n[2], (int)std::min<long long>(nch, G->n[2]));
             if (std::find(zcs.begin(), zcs.end(), zc) == zcs.end()) zcs.push_back(zc);
         }
+        for (int zc : zcs)
+            if (zc <= G->n[2]) cands.emplace_back(zc, tile);
     }
-    for (int zc : zcs) {
-        if (G->dim == 3 && zc > G->n[2]) continue;
-        if (G->dim != 3 && zc != cand[0]) continue;
-        const Launch3 l{G->dim == 3 ? zc : 1, tile};
-        const int pure = G->dim == 3 ? is_pair_tile(tile) : tile >= 1;
+    hipEvent_t e0, e1;
+    PHX_HIP(hipEventCreate(&e0));
+    PHX_HIP(hipEventCreate(&e1));
+    auto time_launch = [&](const Launch3& l, float& ms) -> int {
+        const int pure = G->dim == 3 ? is_pair_tile(l.tile) : l.tile >= 1;
         const int n = pure ? psteps : steps;
-        if ((rc = run_rk4(G, dtype, c, work, p, pure ? 2 : 1, st, l, pure))) return rc;
+        int r = run_rk4(G, dtype, c, work, p, pure ? 2 : 1, st, l, pure);
+        if (r) return r;
         PHX_HIP(hipEventRecord(e0, st));
-        if ((rc = run_rk4(G, dtype, c, work, p, n, st, l, pure))) return rc;
+        if ((r = run_rk4(G, dtype, c, work, p, n, st, l, pure))) return r;
         PHX_HIP(hipEventRecord(e1, st));
         PHX_HIP(hipEventSynchronize(e1));
-        float ms = 0.f;
         PHX_HIP(hipEventElapsedTime(&ms, e0, e1));
         ms /= n;
+        return PHX_OK;
+    };
+    float best = 1e30f;
+    Launch3 bl = uniform_launch(16, 0);
+    for (const auto& cz : cands) {
+        const Launch3 l = uniform_launch(cz.first, cz.second);
+        float ms = 0.f;
+        if ((rc = time_launch(l, ms))) return rc;
         if (ms < best) {
             best = ms;
-            bz = l.zc;
-            bt = tile;
+            bl = l;
         }
     }
+    if (G->dim == 3 && one_stage_tile(bl.tile)) {
+        for (int cls = 0; cls < 3; ++cls) {
+            for (const auto& cz : cands) {
+                if (!one_stage_tile(cz.second)) continue;
+                Launch3 l = bl;
+                int& zc = cls == 0 ? l.zc1 : (cls == 1 ? l.zc : l.zc4);
+                int& tile = cls == 0 ? l.tile1 : (cls == 1 ? l.tile : l.tile4);
+                if (zc == cz.first && tile == cz.second) continue;
+                zc = cz.first;
+                tile = cz.second;
+                float ms = 0.f;
+                if ((rc = time_launch(l, ms))) return rc;
+                if (ms < best) {
+                    best = ms;
+                    bl = l;
+                }
+            }
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    set_launch(G, dtype, Launch3{bz, bt});
-    if (best_zc) *best_zc = bz;
+    set_launch(G, dtype, bl);
+    if (best_zc) *best_zc = bl.zc;
     if (best_ms_per_step) *best_ms_per_step = best;
+    return PHX_OK;
+}
+
+// Launch of one stage class (0: stage 1, 1: stages 2 and 3, 2: stage 4) of
+// the 3D one-stage schedule.
+int phx_ch_rk4_set_stage_launch(const phx_grid* G, int dtype, int cls, int tile, int zc) {
+    int rc = check_grid(G, "phx_ch_rk4_set_stage_launch");
+    if (rc) return rc;
+    if (cls < 0 || cls > 2) return fail(PHX_ERR_INVALID, "phx_ch_rk4_set_stage_launch: class must be 0, 1 or 2");
+    if (G->dim != 3) return fail(PHX_ERR_UNSUPPORTED, "phx_ch_rk4_set_stage_launch: 3D grids only");
+    if (tile < 0 || tile >= kFirstZ + kNumTilesZ || !one_stage_tile(tile))
+        return fail(PHX_ERR_INVALID, "phx_ch_rk4_set_stage_launch: not a one-stage 3D tile");
+    if (tile >= kFirstZ && !zreg_ok(G, dtype))
+        return fail(PHX_ERR_UNSUPPORTED, "phx_ch_rk4_set_stage_launch: z-register tiles need a stored plane below 4 GiB");
+    if (zc < 1 || zc > 4096) return fail(PHX_ERR_INVALID, "phx_ch_rk4_set_stage_launch: zc must be in [1, 4096]");
+    Launch3 l = launch_for(G, dtype);
+    (cls == 0 ? l.zc1 : (cls == 1 ? l.zc : l.zc4)) = zc;
+    (cls == 0 ? l.tile1 : (cls == 1 ? l.tile : l.tile4)) = tile;
+    set_launch(G, dtype, l);
+    return PHX_OK;
+}
+
+int phx_ch_rk4_get_stage_launch(const phx_grid* G, int dtype, int cls, int* zc, int* tile) {
+    int rc = check_grid(G, "phx_ch_rk4_get_stage_launch");
+    if (rc) return rc;
+    if (cls < 0 || cls > 2) return fail(PHX_ERR_INVALID, "phx_ch_rk4_get_stage_launch: class must be 0, 1 or 2");
+    const Launch3 l = launch_for(G, dtype);
+    if (zc) *zc = cls == 0 ? l.zc1 : (cls == 1 ? l.zc : l.zc4);
+    if (tile) *tile = cls == 0 ? l.tile1 : (cls == 1 ? l.tile : l.tile4);
     return PHX_OK;
 }
 

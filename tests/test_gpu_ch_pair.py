@@ -45,6 +45,11 @@ class launch:
         zc, tile = C.c_int(), C.c_int()
         capi.call("phx_ch_rk4_get_launch", C.byref(self.g), self.dtype, C.byref(zc), C.byref(tile))
         self.saved = (zc.value, tile.value)
+        self.classes = []
+        if self.g.dim == 3:
+            for k in range(3):
+                capi.call("phx_ch_rk4_get_stage_launch", C.byref(self.g), self.dtype, k, C.byref(zc), C.byref(tile))
+                self.classes.append((zc.value, tile.value))
         capi.call("phx_ch_rk4_set_tile", C.byref(self.g), self.dtype, self.tile)
         capi.call("phx_ch_rk4_set_zchunk", C.byref(self.g), self.dtype, self.zc)
         return self
@@ -52,6 +57,9 @@ class launch:
     def __exit__(self, *exc):
         capi.call("phx_ch_rk4_set_tile", C.byref(self.g), self.dtype, self.saved[1])
         capi.call("phx_ch_rk4_set_zchunk", C.byref(self.g), self.dtype, self.saved[0])
+        for k, (zc, tile) in enumerate(self.classes):
+            if tile < 5 or tile >= 9:   # one-stage tiles only are per class
+                capi.call("phx_ch_rk4_set_stage_launch", C.byref(self.g), self.dtype, k, tile, zc)
 
 
 def run_gpu(g, dtype, c, p, nsteps, dev, tile, zc):
@@ -124,8 +132,8 @@ def test_pair_tile_rejected_on_small_or_2d_grid(dev):
     assert capi.lib().phx_ch_rk4_set_tile(C.byref(g), capi.F64, 5) == capi.PHX_ERR_UNSUPPORTED
     assert capi.lib().phx_ch_rk4_set_tile(C.byref(g), capi.F64, 9) == 0      # z-register: any n >= 2
     assert capi.lib().phx_ch_rk4_set_tile(C.byref(g), capi.F64, 18) == capi.PHX_ERR_INVALID
-    g2 = capi.make_grid((64, 64), bc=P)   # 2D has tiles 0-4 only
-    assert capi.lib().phx_ch_rk4_set_tile(C.byref(g2), capi.F32, 7) == capi.PHX_ERR_INVALID
+    g2 = capi.make_grid((64, 64), bc=P)   # 2D has tiles 0-8 only
+    assert capi.lib().phx_ch_rk4_set_tile(C.byref(g2), capi.F32, 9) == capi.PHX_ERR_INVALID
 
 
 def test_autotune_with_pair_candidates_keeps_parity(dev):

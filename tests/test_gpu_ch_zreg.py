@@ -91,3 +91,34 @@ def test_zreg_full_size_256_fp32_equals_one_stage(dev):
         print(f"256^3 tile={tile}", flush=True)
         got = run_gpu(g, capi.F32, c0, p, 5, dev, tile, 16)
         assert torch.equal(got, one), tile
+
+
+@pytest.mark.parametrize("dtype", [capi.F64, capi.F32])
+@pytest.mark.parametrize("case", [0, 1, 3, 7, 8])
+def test_mixed_stage_class_launches_equal_uniform(dev, dtype, case):
+    """Each stage class on its own one-stage launch (z-register and LDS-ring
+    tiles mixed, different z-chunks) returns the uniform launch's bits."""
+    import torch
+    n, d, ghost, bc, vals = CASES[case]
+    g = capi.make_grid(n, d=d, ghost=ghost, bc=bc, bc_value=vals)
+    p = ch_params(rho=1.0, ca=0.0, cb=1.0, kappa=1.0, M=1.0, dt=0.005)
+    c0 = init.to_stored(init.ch_field(n, 41 + case, 0.4, 0.02), g)
+    one = run_gpu(g, dtype, c0, p, 5, dev, ONE_STAGE[dtype], 16)
+    for mix in ([(9, 5), (17, 8), (1, 3)], [(0, 2), (12, 16), (17, 64)], [(14, 1), (1, 7), (10, 4)]):
+        with launch(g, dtype, ONE_STAGE[dtype], 16):
+            for k, (tile, zc) in enumerate(mix):
+                capi.call("phx_ch_rk4_set_stage_launch", C.byref(g), dtype, k, tile, zc)
+            print(f"mixed launches case {case} dtype {dtype}: {mix}", flush=True)
+            got = run_gpu_current(g, dtype, c0, p, 5, dev)
+        assert torch.equal(got, one), mix
+
+
+def run_gpu_current(g, dtype, c, p, nsteps, dev):
+    import torch
+    from helpers import to_dev, stream
+    tdt, esz = (torch.float64, 8) if dtype == capi.F64 else (torch.float32, 4)
+    tc = to_dev(c, dev, tdt)
+    work = torch.zeros(capi.lib().phx_ch_rk4_workspace_bytes(C.byref(g), dtype) // esz, dtype=tdt, device=dev)
+    capi.call("phx_ch_rk4_steps", C.byref(g), dtype, capi.ptr(tc), capi.ptr(work), C.byref(p), nsteps, stream())
+    torch.cuda.synchronize()
+    return tc

@@ -81,7 +81,7 @@ def test_launch_table_defaults_and_ranges():
     lib = capi.lib()
     g3 = capi.make_grid((40, 36, 34), bc=capi.NOFLUX)
     assert _launch(g3, capi.F32) == (16, 9)     # z-register 64x16, 256 threads
-    assert _launch(g3, capi.F64) == (32, 11)
+    assert _launch(g3, capi.F64) == (5, 17)     # stages 2-3: z-register 128x8/512 at 8 z-chunks
     g2 = capi.make_grid((1024, 1024), bc=capi.PERIODIC)
     assert _launch(g2, capi.F32) == (1, 7)      # whole-step 128x32, 1024 threads
     assert _launch(g2, capi.F64) == (1, 5)      # whole-step 32x32, 512 threads
@@ -107,7 +107,7 @@ def test_launch_table_defaults_and_ranges():
     # settings are kept per (dim, shape, dtype)
     assert lib.phx_ch_rk4_set_zchunk(C.byref(g3), capi.F32, 22) == 0
     assert _launch(g3, capi.F32) == (22, 17)
-    assert _launch(g3, capi.F64) == (32, 11)
+    assert _launch(g3, capi.F64) == (5, 17)
     assert lib.phx_ch_rk4_set_zchunk(C.byref(g3), capi.F32, 0) == capi.PHX_ERR_INVALID
     capi.call("phx_ch_rk4_set_tile", C.byref(g3), capi.F32, 11)
     capi.call("phx_ch_rk4_set_zchunk", C.byref(g3), capi.F32, 16)
@@ -151,3 +151,30 @@ def test_noalloc_rk4_rejects_oversized_grids():
     assert lib.phx_ch_rk4_steps_noalloc(C.byref(big), capi.F64, fake, C.byref(p), 1, None) == capi.PHX_ERR_UNSUPPORTED
     assert b"per GPU" in lib.phx_last_error()
     assert lib.phx_ch_rk4_steps_noalloc(C.byref(big), capi.F64, None, C.byref(p), 1, None) == capi.PHX_ERR_INVALID
+
+
+def _stage(g, dtype, cls):
+    zc, tile = C.c_int(), C.c_int()
+    capi.call("phx_ch_rk4_get_stage_launch", C.byref(g), dtype, cls, C.byref(zc), C.byref(tile))
+    return zc.value, tile.value
+
+
+def test_stage_class_launches():
+    """Per stage class launches of the 3D one-stage schedule: FP64 defaults
+    per class, the setters, uniform setters overriding every class, and the
+    argument checks (host state only)."""
+    lib = capi.lib()
+    g = capi.make_grid((256, 256, 256), bc=capi.NOFLUX)
+    assert [_stage(g, capi.F64, k) for k in range(3)] == [(22, 9), (32, 17), (64, 17)]
+    assert [_stage(g, capi.F32, k) for k in range(3)] == [(16, 9)] * 3
+    g5 = capi.make_grid((64, 48, 40), bc=capi.NOFLUX)
+    capi.call("phx_ch_rk4_set_stage_launch", C.byref(g5), capi.F32, 2, 3, 11)
+    assert _stage(g5, capi.F32, 2) == (11, 3) and _stage(g5, capi.F32, 0) == (16, 9)
+    assert lib.phx_ch_rk4_set_stage_launch(C.byref(g5), capi.F32, 3, 9, 16) == capi.PHX_ERR_INVALID
+    assert lib.phx_ch_rk4_set_stage_launch(C.byref(g5), capi.F32, 0, 5, 16) == capi.PHX_ERR_INVALID  # two-stage
+    assert lib.phx_ch_rk4_set_stage_launch(C.byref(g5), capi.F32, 0, 18, 16) == capi.PHX_ERR_INVALID
+    assert lib.phx_ch_rk4_set_stage_launch(C.byref(g5), capi.F32, 0, 9, 0) == capi.PHX_ERR_INVALID
+    g2 = capi.make_grid((64, 64), bc=capi.PERIODIC)
+    assert lib.phx_ch_rk4_set_stage_launch(C.byref(g2), capi.F32, 0, 0, 1) == capi.PHX_ERR_UNSUPPORTED
+    capi.call("phx_ch_rk4_set_tile", C.byref(g5), capi.F32, 12)
+    assert [_stage(g5, capi.F32, k)[1] for k in range(3)] == [12] * 3
