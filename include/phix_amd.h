@@ -218,11 +218,9 @@ int phx_ch_rk4_steps(const phx_grid* grid, int dtype, void* c, void* work,
 int phx_ch_rk4_steps_noalloc(const phx_grid* grid, int dtype, void* c,
                              const phx_ch_params* p, int nsteps, phx_stream stream);
 
-/* Launch shape of the 3D stage kernels: z-chunk zc (default 16 FP32, 32
- * FP64; default tiles: 3D 9 (z-register 64x16/256) FP32 and per stage
- * class in FP64 (phx_ch_rk4_set_stage_launch); 2D with n >= 8 the
- * whole-step tiles 7 (FP32 on grids of 256^2 and more) or 5, else 0), per
- * (dim, n, dtype); the planes are split into cdiv(n_z, zc) balanced chunks of
+/* Launch shape of the RK4 kernels, per (dim, n, dtype). Defaults: 3D per
+ * stage class (phx_ch_rk4_set_stage_launch); 2D with n >= 8 the whole-step
+ * tiles 7 (FP32 on grids of 256^2 and more) or 5, else 0. In 3D the planes are split into cdiv(n_z, zc) balanced chunks of
  * floor or ceil(n_z / chunks) planes, one workgroup column each. The
  * autotuner times every tile shape x z-chunk in {4,8,16,32,64} plus the
  * chunk counts that fill 1, 2 or 3 whole waves of resident workgroups (from
@@ -273,8 +271,9 @@ int phx_ch_rk4_autotune(const phx_grid* grid, int dtype, void* c, void* work,
  * the same bits, so classes may mix launches. phx_ch_rk4_set_tile/_set_zchunk
  * set all three classes; phx_ch_rk4_get_launch reports class 1, whose tile
  * also chooses the schedule (a two-stage tile there runs every stage
- * two-stage). Default in 3D FP32: tile 9 at zc 16 for every class; FP64:
- * class 0 tile 9 at 12 z-chunks, classes 1 and 2 tile 17 at 8 and 4 z-chunks. */
+ * two-stage). Defaults in 3D FP32: classes 0 and 1 tile 11 at 24 z-chunks,
+ * class 2 tile 12 at 16; FP64: class 0 tile 9 at 12 z-chunks, classes 1 and
+ * 2 tile 17 at 8 and 4 z-chunks. */
 int phx_ch_rk4_set_stage_launch(const phx_grid* grid, int dtype, int cls, int tile, int zc);
 int phx_ch_rk4_get_stage_launch(const phx_grid* grid, int dtype, int cls, int* zc, int* tile);
 

@@ -2481,10 +2481,18 @@ static Launch3 launch_for(const phx_grid* G, int dtype) {
         return uniform_launch(1, dtype == PHX_F32 && G->n[0] >= 256 && G->n[1] >= 256 ? 7 : 5);
     }
     if (G->dim != 3) return uniform_launch(1, 0);
-    if (dtype == PHX_F32) return uniform_launch(16, kFirstZ + 0);
+    Launch3 l;
+    if (dtype == PHX_F32) {
+        // FP32, per stage class (256^3 measurement): stages 1-3 128x16/512 at
+        // 24 z-chunks, stage 4 256x8/512 at 16
+        l.tile1 = l.tile = kFirstZ + 2;
+        l.zc1 = l.zc = (int)cdiv(G->n[2], 24);
+        l.tile4 = kFirstZ + 3;
+        l.zc4 = (int)cdiv(G->n[2], 16);
+        return l;
+    }
     // FP64, per stage class (256^3 measurement): stage 1 64x16/256 at 12
     // chunks, stages 2-3 128x8/512 at 8 chunks, stage 4 128x8/512 at 4 chunks
-    Launch3 l;
     l.tile1 = kFirstZ + 0;
     l.zc1 = (int)cdiv(G->n[2], 12);
     l.tile = kFirstZ + 8;

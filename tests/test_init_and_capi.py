@@ -80,7 +80,7 @@ def test_launch_table_defaults_and_ranges():
     temporally blocked kernels and their error codes."""
     lib = capi.lib()
     g3 = capi.make_grid((40, 36, 34), bc=capi.NOFLUX)
-    assert _launch(g3, capi.F32) == (16, 9)     # z-register 64x16, 256 threads
+    assert _launch(g3, capi.F32) == (2, 11)     # stages 2-3: z-register 128x16/512 at 24 z-chunks
     assert _launch(g3, capi.F64) == (5, 17)     # stages 2-3: z-register 128x8/512 at 8 z-chunks
     g2 = capi.make_grid((1024, 1024), bc=capi.PERIODIC)
     assert _launch(g2, capi.F32) == (1, 7)      # whole-step 128x32, 1024 threads
@@ -166,10 +166,10 @@ def test_stage_class_launches():
     lib = capi.lib()
     g = capi.make_grid((256, 256, 256), bc=capi.NOFLUX)
     assert [_stage(g, capi.F64, k) for k in range(3)] == [(22, 9), (32, 17), (64, 17)]
-    assert [_stage(g, capi.F32, k) for k in range(3)] == [(16, 9)] * 3
+    assert [_stage(g, capi.F32, k) for k in range(3)] == [(11, 11), (11, 11), (16, 12)]
     g5 = capi.make_grid((64, 48, 40), bc=capi.NOFLUX)
     capi.call("phx_ch_rk4_set_stage_launch", C.byref(g5), capi.F32, 2, 3, 11)
-    assert _stage(g5, capi.F32, 2) == (11, 3) and _stage(g5, capi.F32, 0) == (16, 9)
+    assert _stage(g5, capi.F32, 2) == (11, 3) and _stage(g5, capi.F32, 0) == (2, 11)
     assert lib.phx_ch_rk4_set_stage_launch(C.byref(g5), capi.F32, 3, 9, 16) == capi.PHX_ERR_INVALID
     assert lib.phx_ch_rk4_set_stage_launch(C.byref(g5), capi.F32, 0, 5, 16) == capi.PHX_ERR_INVALID  # two-stage
     assert lib.phx_ch_rk4_set_stage_launch(C.byref(g5), capi.F32, 0, 18, 16) == capi.PHX_ERR_INVALID
