@@ -2355,16 +2355,18 @@ static int rk4_pair_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_param
 }
 
 // RK4 steps with the 2D whole-step kernel (tile index stile into kTilesS2).
-// Same schedule as the two-stage 3D path: the state alternates between c and
-// the work buffer w1, an odd count of steps before the last starts with one
-// one-stage step, and the last step runs one-stage in place (BC fill of c
-// first), so c ends with the state and the ghost layer the one-stage path
-// leaves.
+// The state alternates between two buffers X and Y, chosen so that the last
+// step reads the caller's c (X = c for an odd step count; for an even one c
+// is first copied to w1 and X = w1, Y = c). Before the last step c's ghost
+// layer gets the BC fill of that state; the last step writes w0 and its
+// physical cells are copied into c. c thus ends with the state and the ghost
+// layer the one-stage path leaves, and every step runs the whole-step kernel.
 template <typename T, bool EXACT>
 static int rk4_step2d_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_params* p, int nsteps, hipStream_t st,
                               int stile, int pure) {
     const Layout L = make_layout(G);
     const long long N = L.total();
+    T* w0 = work;
     T* w1 = work + N;
     PairArgs<T> P;
     P.L = L;
@@ -2393,23 +2395,31 @@ static int rk4_step2d_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_par
         launch_step2d<T, EXACT>(stile, grid, P, st);
         lt_end(3, st);
     };
-    int rest = pure ? nsteps : nsteps - 1;
-    if (!pure && rest % 2) {
-        int rc = rk4_steps_t<T, EXACT>(G, c, work, p, 1, st, 1, 0, /*skip_fill=*/1);
-        if (rc) return rc;
-        --rest;
-    }
-    T* x = c;
-    T* y = w1;
-    for (int s = 0; s < rest; ++s) {
-        step(x, y);
-        std::swap(x, y);
-    }
     if (pure) {
+        // timing helper: nsteps (even) whole steps, the state ends in c
+        T* x = c;
+        T* y = w1;
+        for (int s = 0; s < nsteps; ++s) {
+            step(x, y);
+            std::swap(x, y);
+        }
         PHX_LAUNCH_CHECK("phx_ch_rk4_steps");
         return PHX_OK;
     }
-    return rk4_steps_t<T, EXACT>(G, c, work, p, 1, st, 1, 0);
+    const bool even = nsteps % 2 == 0;
+    T* X = even ? w1 : c;
+    T* Y = even ? c : w1;
+    if (even) PHX_HIP(hipMemcpyAsync(w1, c, (size_t)N * sizeof(T), hipMemcpyDeviceToDevice, st));
+    for (int s = 0; s < nsteps - 1; ++s) step(s % 2 == 0 ? X : Y, s % 2 == 0 ? Y : X);
+    int rc = bc_fill_impl(G, sizeof(T) == 8 ? PHX_F64 : PHX_F32, c, st);
+    if (rc) return rc;
+    step(c, w0);
+    const size_t pitch = (size_t)L.sx * sizeof(T);
+    const long long o0 = L.at(0, 0, 0);
+    PHX_HIP(hipMemcpy2DAsync(c + o0, pitch, w0 + o0, pitch, (size_t)L.n0 * sizeof(T), L.n1, hipMemcpyDeviceToDevice,
+                             st));
+    PHX_LAUNCH_CHECK("phx_ch_rk4_steps");
+    return PHX_OK;
 }
 
 static int check_ch_params(const phx_ch_params* p, const char* who) {

@@ -260,8 +260,8 @@ def test_step2d_tile_rejected_on_small_grid(dev):
 
 
 def test_step2d_timing_kind(dev):
-    """A 4-step call with a whole-step tile: steps before the last = 3 (odd) ->
-    one one-stage step, 2 whole steps, then the one-stage last step."""
+    """Calls with a whole-step tile run every step as one whole-step launch,
+    for even and odd step counts (no one-stage launches)."""
     import torch
     n = (64, 64)
     g = capi.make_grid(n, bc=P)
@@ -269,13 +269,15 @@ def test_step2d_timing_kind(dev):
     tc = to_dev(init.to_stored(init.ch_field(n, 9, 0.4, 0.02), g), dev, torch.float32)
     work = torch.zeros(capi.lib().phx_ch_rk4_workspace_bytes(C.byref(g), capi.F32) // 4, dtype=torch.float32,
                        device=dev)
-    with launch(g, capi.F32, 2, 1):
-        capi.call("phx_ch_timing", 1)
-        capi.call("phx_ch_rk4_steps", C.byref(g), capi.F32, capi.ptr(tc), capi.ptr(work), C.byref(p), 4, stream())
-        counts = []
-        for kind in range(4):
-            k, ms = C.c_int(), C.c_double()
-            capi.call("phx_ch_timing_read", kind, C.byref(k), C.byref(ms))
-            counts.append(k.value)
-        capi.call("phx_ch_timing", 0)
-    assert counts == [8, 0, 0, 2], counts
+    for nsteps in (4, 5):
+        with launch(g, capi.F32, 2, 1):
+            capi.call("phx_ch_timing", 1)
+            capi.call("phx_ch_rk4_steps", C.byref(g), capi.F32, capi.ptr(tc), capi.ptr(work), C.byref(p), nsteps,
+                      stream())
+            counts = []
+            for kind in range(4):
+                k, ms = C.c_int(), C.c_double()
+                capi.call("phx_ch_timing_read", kind, C.byref(k), C.byref(ms))
+                counts.append(k.value)
+            capi.call("phx_ch_timing", 0)
+        assert counts == [0, 0, 0, nsteps], counts
