@@ -201,15 +201,32 @@ def grad(f, axis, d, bc):
     return (p[2:, 1:-1] - p[:-2, 1:-1]) * inv_2d
 
 
+def grad_zero_ghosts(f, axis, d):
+    """grad of a field whose ghost layer holds zeros (same central difference
+    as grad)."""
+    p = np.pad(f, 1, mode="constant", constant_values=0.0)
+    inv_2d = 0.5 / d
+    if axis == 0:
+        return (p[1:-1, 2:] - p[1:-1, :-2]) * inv_2d
+    return (p[2:, 1:-1] - p[:-2, 1:-1]) * inv_2d
+
+
 def c_operator(D, mu, dx, dy, bc):
-    """D lap(mu) + dD/dx dmu/dx + dD/dy dmu/dy (GFA.cu:160-165)."""
-    return D * lap(mu, dx, dy, bc) + grad(D, 0, dx, bc) * grad(mu, 0, dx, bc) + grad(D, 1, dy, bc) * grad(mu, 1, dy, bc)
+    """D lap(mu) + dD/dx dmu/dx + dD/dy dmu/dy (GFA.cu:160-165). mu's ghost
+    layer is the BC fill (multi-step 2 refreshes it, Solver.cu:322). D's ghost
+    layer is zero, whatever the BC: multi-step 3 fills D's ghosts and then
+    recomputes D as a STEADY unknown, and computeRHS clears the whole stored
+    array before the pointwise term writes the physical cells
+    (Solver.cu:322-326, Equation.cu:265, TermPW.inl:28-36)."""
+    return (D * lap(mu, dx, dy, bc) + grad_zero_ghosts(D, 0, dx) * grad(mu, 0, dx, bc)
+            + grad_zero_ghosts(D, 1, dy) * grad(mu, 1, dy, bc))
 
 
 def steps(m: Model, c, phi, eta, dx, dy, bc, dt, nsteps):
-    """nsteps multi-step Euler steps in the reference's order: mu; D (twice,
-    the second only refreshing D's ghosts); c; phi reading the new c; eta
-    reading the new phi. Returns new (c, phi, eta, mu, D)."""
+    """nsteps multi-step Euler steps in the reference's order: mu; D (twice:
+    the second fills D's ghosts, then clears them again by recomputing D, see
+    c_operator); c; phi reading the new c; eta reading the new phi. Returns
+    new (c, phi, eta, mu, D)."""
     c, phi, eta = (np.array(a, dtype=np.float64, copy=True) for a in (c, phi, eta))
     mu = D = None
     k = m.k

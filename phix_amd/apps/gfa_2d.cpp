@@ -8,7 +8,8 @@
 // One Euler step is six multi-steps, in the reference's order (GFA.cu:192-201):
 //   1. BCs(c);   mu  = (1 - h(phi)) dG_L/dc(c, T) / Vm_L                  STEADY
 //   2. BCs(mu);  D   = M_c(phi, eta) c (1 - c)                             STEADY
-//   3. BCs(D);   D   recomputed (so D's ghosts are filled for step 4)      STEADY
+//   3. BCs(D);   D   recomputed (which clears D's ghosts again: step 4    STEADY
+//                reads D = 0 beyond the faces, as the reference does)
 //   4. BCs(c);   c  += dt [D lap(mu) + dD/dx dmu/dx + dD/dy dmu/dy]        TRANSIENT
 //   5. BCs(phi); phi += dt [-M_phi dF/dphi + M_phi eps^2 lap(phi)]         TRANSIENT
 //   6. BCs(eta); eta += dt [-M_eta dF/deta + M_eta beta^2 lap(eta)]        TRANSIENT

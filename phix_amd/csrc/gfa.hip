@@ -3,12 +3,16 @@
 // 2D/GFA.cu:143-201; its six multi-steps run in Solver.cu:317-338 order).
 //
 // Why one pass is exact: every multi-step reads its inputs either pointwise at
-// the same cell or through a radius-1 stencil of fields BC-filled at the start
-// of the step. mu and D are pointwise in (c, phi, eta), and a BC fill of them
-// (periodic wrap, zero-flux mirror) equals evaluating them at the BC image of
-// c, phi, eta. So a workgroup loads c, phi, eta on its tile plus a one-cell
-// ring through the BC index maps, evaluates mu and D there into LDS, and
-// updates c (D lap mu + grad D . grad mu), then phi reading the new c, then eta
+// the same cell or through a radius-1 stencil of fields whose ghost layer is
+// set at the start of the step. mu is pointwise in (c, phi), and its BC fill
+// (periodic wrap, zero-flux copy) equals evaluating it at the BC image of c
+// and phi. D's ghost layer is zero for every BC: multi-step 3 fills it and
+// then recomputes D as a STEADY unknown, and computeRHS clears the whole
+// stored array before the pointwise term writes the physical cells
+// (Solver.cu:322-326, Equation.cu:265, TermPW.inl:28-36). So a workgroup
+// loads c, phi, eta on its tile plus a one-cell ring through the BC index
+// maps, evaluates mu (and, inside the domain, D) there into LDS, and updates
+// c (D lap mu + grad D . grad mu), then phi reading the new c, then eta
 // reading the new phi, all at the same cell. HBM per cell-step: read c, phi,
 // eta, write them (48 B in FP64); mu and D never leave the chip.
 //
@@ -91,8 +95,10 @@ __global__ __launch_bounds__(GNT) void k_gfa_step(GfaArgs A) {
         const int hx = q % GHX, hy = q / GHX;
         const long long o = A.L.at(map_in(i0 + hx - 1, A.bx), map_in(j0 + hy - 1, A.by), 0);
         const double c = A.c[o], p = A.phi[o], e = A.eta[o];
+        const int x = i0 + hx - 1, y = j0 + hy - 1;
+        const bool inside = x >= 0 && x < A.L.n0 && y >= 0 && y < A.L.n1;
         s_mu[hy][hx] = gfa_mu(k, c, p);
-        s_D[hy][hx] = gfa_D(k, c, p, e);
+        s_D[hy][hx] = inside ? gfa_D(k, c, p, e) : 0.0;   // D's ghost layer is zero
         s_p[hy][hx] = p;
         s_e[hy][hx] = e;
     }

@@ -175,8 +175,30 @@ def test_c_operator_converges_to_div_D_grad_mu():
         lap_mu = -4 * np.cos(2 * X) - np.sin(Y)
         exact = D * lap_mu + (0.5 * np.cos(X) * np.cos(Y)) * (-2 * np.sin(2 * X)) + \
             (-0.5 * np.sin(X) * np.sin(Y)) * np.cos(Y)
-        errs.append(np.abs(G.c_operator(D, mu, d, d, ("Periodic", "Periodic")) - exact).max())
+        # interior cells: D's zero ghost layer enters only the face cells
+        errs.append(np.abs(G.c_operator(D, mu, d, d, ("Periodic", "Periodic")) - exact)[1:-1, 1:-1].max())
     assert errs[0] / errs[1] > 3.8 and errs[1] / errs[2] > 3.9, errs
+
+
+@pytest.mark.parametrize("bc", ["NoFlux", "Periodic"])
+def test_c_operator_reads_zero_D_beyond_the_faces(bc):
+    """The reference's multi-step 3 leaves D's ghost layer at zero for every
+    BC, so at a face cell grad D reads D = 0 outside while grad mu and lap mu
+    read mu's BC fill. Known answer for D = 1, mu = x along x (dx = 1, 5 x 4
+    cells): interior 0; cell 0: lap mu = (mu1 - 2 mu0 + ghost)/dx^2 and
+    dD/dx dmu/dx = (1 - 0)/2 * (mu1 - ghost)/2."""
+    nx, ny = 5, 4
+    mu = np.tile(np.arange(nx, dtype=float), (ny, 1))
+    D = np.ones((ny, nx))
+    got = G.c_operator(D, mu, 1.0, 1.0, (bc, bc))
+    g0 = mu[0, 0] if bc == "NoFlux" else mu[0, -1]       # mu's ghost at x = -1
+    g1 = mu[0, -1] if bc == "NoFlux" else mu[0, 0]       # mu's ghost at x = nx
+    lo = (mu[0, 1] - 2 * mu[0, 0] + g0) + 0.5 * (mu[0, 1] - g0) / 2
+    hi = (g1 - 2 * mu[0, -1] + mu[0, -2]) + (-0.5) * (g1 - mu[0, -2]) / 2
+    assert np.array_equal(got[:, 1:-1], np.zeros((ny, nx - 2)))
+    assert np.allclose(got[:, 0], lo, rtol=0, atol=1e-15) and np.allclose(got[:, -1], hi, rtol=0, atol=1e-15)
+    # y faces: grad D along y reads the zero ghosts, but mu is constant in y
+    assert np.array_equal(got[0, 1:-1], np.zeros(nx - 2))
 
 
 def test_model_header_matches_oracle_on_host():
