@@ -31,8 +31,10 @@ static void hipOk(hipError_t e, const char* what) {
 
 // Per-face BC table of a BC list applied in order (last writer wins per
 // face). Fails (returns false) on a kind the fused kernels do not know, a
-// face left without a condition, or a list not in X, Y, Z order (the order
-// fixes the corner ghosts, which the fused kernels fill X, Y, Z).
+// face left without a condition, a condition on an axis at or beyond the
+// mesh dimension (applied object by object it fills ghost planes the table
+// fill does not walk), or a list not in X, Y, Z order (the order fixes the
+// corner ghosts, which the fused kernels fill X, Y, Z).
 bool bcTable(const std::vector<BoundaryCondition*>& bcs, const Mesh& m, phx_grid& g) {
     for (int a = 0; a < 3; ++a)
         for (int s = 0; s < 2; ++s) {
@@ -43,7 +45,7 @@ bool bcTable(const std::vector<BoundaryCondition*>& bcs, const Mesh& m, phx_grid
     for (auto* bc : bcs) {
         if (!bc) return false;
         const int a = (int)bc->axis;
-        if (a < lastAxis) return false;
+        if (a < lastAxis || a >= m.dim) return false;
         lastAxis = a;
         const int k = bc->kind();
         if (k < PHX_BC_PERIODIC || k > PHX_BC_FIXED) return false;
