@@ -77,52 +77,3 @@ def read_field(path):
     nc = meta.get("nComponents", 1)
     data = np.frombuffer(raw[end:], dtype="<f8").reshape(nc, meta["nz"], meta["ny"], meta["nx"])
     return meta, data
-
-
-# ------------------------------------------------ composite DSL reference
-def composite_reference(mode):
-    """numpy restatement of the forms tests/cpp/composite.cpp evaluates, on its
-    24 x 20 mesh (dx = 0.5, dy = 0.25) with one ghost layer. A composite term
-    materialises its expression on the physical cells (ghosts zero), applies
-    the BCs (x, then y) and takes the stencil: gradient (s[+1] - s[-1]) / (2d),
-    Laplacian (s[+1] - 2 s + s[-1]) / d^2 (the reference's
-    src/equation/Equation.cu:87-96 and its lap kernel). Returns {name: (ny, nx)}."""
-    nx, ny, dx, dy = 24, 20, 0.5, 0.25
-    x = (np.arange(nx) + 0.5) * dx
-    y = (np.arange(ny) + 0.5) * dy
-    X, Y = np.meshgrid(x, y, indexing="xy")
-
-    def padded(phys):
-        s = np.zeros((ny + 2, nx + 2))
-        s[1:-1, 1:-1] = phys
-        if mode == "periodic":
-            s[:, 0], s[:, -1] = s[:, nx], s[:, 1]
-            s[0, :], s[-1, :] = s[ny, :], s[1, :]
-        else:
-            s[:, 0], s[:, -1] = s[:, 1], s[:, nx]
-            s[0, :], s[-1, :] = s[1, :], s[ny, :]
-        return s
-
-    def gx(s):
-        return (s[1:-1, 2:] - s[1:-1, :-2]) * (0.5 / dx)
-
-    def gy(s):
-        return (s[2:, 1:-1] - s[:-2, 1:-1]) * (0.5 / dy)
-
-    def lap(s):
-        c = s[1:-1, 1:-1]
-        return (s[1:-1, 2:] - 2.0 * c + s[1:-1, :-2]) / dx ** 2 + (s[2:, 1:-1] - 2.0 * c + s[:-2, 1:-1]) / dy ** 2
-
-    a = 1.5 + np.sin(X) * np.cos(2.0 * Y)
-    b = np.cos(0.7 * X - 1.3 * Y)
-    v0 = X * np.sin(Y)
-    v1 = np.exp(-0.1 * X) * Y
-    ab = a * b
-    return {
-        "gradx_sq": gx(padded(a * a)),
-        "grady_ab": gy(padded(ab)),
-        "lap_ab": lap(padded(ab)),
-        "divgrad_ab": gx(padded(gx(padded(ab)))) + gy(padded(gy(padded(ab)))),
-        "ab_lapb": ab * lap(padded(b)),
-        "mixed": 2.0 * gx(padded(a)) + (gx(padded(v0)) + gy(padded(v1))) - ab,
-    }

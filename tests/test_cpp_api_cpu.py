@@ -64,12 +64,13 @@ def test_composite_forms_host_path(tmp_path):
     path, against the numpy restatement; periodic and zero-flux BCs."""
     import sys
     sys.path.insert(0, str(ROOT / "tests"))
-    from helpers import composite_reference, read_field
+    from helpers import read_field
+    from oracle import dsl
     for mode in ("periodic", "noflux"):
         r = subprocess.run([str(BIN / "composite"), "cpu", str(tmp_path), mode], capture_output=True, text=True,
                            timeout=120)
         assert r.returncode == 0, r.stderr
-        ref = composite_reference(mode)
+        ref = dsl.composite_forms(*dsl.composite_inputs(), 0.5, 0.25, mode)
         for name, want in ref.items():
             _, got = read_field(tmp_path / f"{name}_{mode}_cpu.field")
             err = np.abs(got[0, 0] - want).max()

@@ -122,27 +122,7 @@ def test_ch_2d_fp32_bulk(dev, tmp_path):
     assert 0 < err <= 1e-5   # FP32 arithmetic, so not bit-equal to FP64
 
 
-def _lap_periodic(f):
-    return (np.roll(f, 1, 1) + np.roll(f, -1, 1) - 2.0 * f) + (np.roll(f, 1, 0) + np.roll(f, -1, 0) - 2.0 * f)
-
-
-def chac_reference(c, eta, k, dt, nsteps):
-    """FP64 numpy restatement of the CH_AC_2D equations (dx = dy = 1,
-    periodic), one multi-step Euler step = mu from c and eta; c += dt M lap mu;
-    eta += dt RHS(eta, updated c)."""
-    rho, ca, cb = k["rho"], k["ca"], k["cb"]
-    c, eta = c.copy(), eta.copy()
-    for _ in range(nsteps):
-        h = eta ** 3 * (6.0 * eta ** 2 - 15.0 * eta + 10.0)
-        mu = 2.0 * rho * rho * (c - ca) + 2.0 * rho * rho * (ca - cb) * h - k["kappa_c"] * _lap_periodic(c)
-        c = c + dt * (k["M"] * _lap_periodic(mu))
-        bulk = 30.0 * rho * rho * eta ** 2 * (1.0 - eta) ** 2 * (2.0 * c - ca - cb) * (ca - cb)
-        well = 2.0 * k["w"] * eta * (1.0 - eta) * (1.0 - 2.0 * eta)
-        eta = eta + dt * (-k["L"] * (bulk + well) + k["L"] * k["kappa_eta"] * _lap_periodic(eta))
-    return c, eta
-
-
-def test_ch_ac_2d_matches_numpy(dev, tmp_path):
+def test_ch_ac_2d_matches_oracle(dev, tmp_path):
     nx, ny, nsteps = 48, 40, 200
     c0 = init.ch_field((nx, ny), 13, 0.5, 0.05)
     eta0 = init.ch_field((nx, ny), 14, 0.5, 0.3)
@@ -151,11 +131,12 @@ def test_ch_ac_2d_matches_numpy(dev, tmp_path):
             {"c": c0, "eta": eta0, "mu": np.zeros_like(c0)})
     out = run("ch_ac_2d", work)
     assert "solver plan: generic" in out and "Done." in out
-    c_ref, eta_ref = chac_reference(c0[0], eta0[0], CHAC_CONST, 0.001, nsteps)
+    from oracle import dsl
+    c_ref, eta_ref, _ = dsl.chac_steps(c0[0], eta0[0], CHAC_CONST, 0.001, nsteps)
     _, cc = read_field(work / "output" / f"c_{nsteps}.field")
     _, ee = read_field(work / "output" / f"eta_{nsteps}.field")
     ec, ee_ = np.abs(cc[0, 0] - c_ref).max(), np.abs(ee[0, 0] - eta_ref).max()
-    print(f"ch_ac_2d: max |c - numpy| = {ec:.3e}, max |eta - numpy| = {ee_:.3e}")
+    print(f"ch_ac_2d: max |c - oracle| = {ec:.3e}, max |eta - oracle| = {ee_:.3e}")
     assert np.abs(eta_ref - eta0[0]).max() > 1e-3   # the run moved the fields
     assert ec <= 1e-11 and ee_ <= 1e-11
 

@@ -110,10 +110,10 @@ def test_ac_vector_solver(dev, tmp_path, nf, bcmode):
 def test_composite_forms_device_path(dev, tmp_path, mode):
     """Composite DSL forms on the device path (materialise, BC launch, stencil
     and product kernels) against the host path and the numpy restatement."""
-    from helpers import composite_reference
+    from oracle import dsl
     run([BIN / "composite", "cpu", tmp_path, mode])
     run([BIN / "composite", "gpu", tmp_path, mode])
-    for name, want in composite_reference(mode).items():
+    for name, want in dsl.composite_forms(*dsl.composite_inputs(), 0.5, 0.25, mode).items():
         _, h = read_field(tmp_path / f"{name}_{mode}_cpu.field")
         _, d = read_field(tmp_path / f"{name}_{mode}_gpu.field")
         scale = max(1.0, np.abs(want).max())
