@@ -203,6 +203,17 @@ size_t phx_ch_rk4_workspace_bytes(const phx_grid* grid, int dtype);
 int phx_ch_rk4_steps(const phx_grid* grid, int dtype, void* c, void* work,
                      const phx_ch_params* p, int nsteps, phx_stream stream);
 
+/* nbatch independent fields of one 2D grid (periodic, no-flux or fixed
+ * faces, n >= 8 on both axes), stored back to back in c (each
+ * phx_ch_rk4_workspace_bytes(grid)/3 bytes apart, i.e. its full stored
+ * size), advanced by nsteps RK4 steps together: the whole-step kernel of the
+ * grid's 2D launch runs all fields per launch (the batch on the launch's z
+ * axis). Each field ends exactly as phx_ch_rk4_steps would leave it, bit for
+ * bit. work: nbatch * phx_ch_rk4_workspace_bytes(grid) bytes. No reference
+ * counterpart (the reference advances one field per solver). */
+int phx_ch_rk4_steps_batch(const phx_grid* grid, int dtype, int nbatch, void* c, void* work,
+                           const phx_ch_params* p, int nsteps, phx_stream stream);
+
 /* No reference counterpart: for callers that cannot allocate device memory
  * (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc). phx_ch_rk4_steps with the RK4 workspace held by the library:
@@ -374,8 +385,13 @@ int phx_ensemble_autotune(phx_ensemble* ens, float* c, float* work, double dt, i
 int    phx_sur_param_count(void);
 size_t phx_sur_workspace_bytes(int B, int H, int W);
 /* flags: PHX_SUR_REUSE_WEIGHTS skips re-packing the BF16 weights (valid when
- * params are unchanged since the last call on this ws). */
+ * params are unchanged since the last call on this ws). PHX_SUR_FUSED
+ * evaluates the six layers of each 16x16 output tile in one launch with the
+ * activations in LDS (inference: the activations are not kept, so no
+ * phx_sur_backward may follow); same arithmetic as the layer-by-layer
+ * kernels. */
 #define PHX_SUR_REUSE_WEIGHTS 1
+#define PHX_SUR_FUSED 2
 int    phx_sur_forward(const float* params, void* ws, const float* c_in, float* c_out,
                        int B, int H, int W, int flags, phx_stream stream);
 /* Loss = mean((c_out - c_target)^2); grad (param layout) <- dLoss/dparams;

@@ -1068,6 +1068,7 @@ struct PairArgs {
     FastCoef<T> F; // FP32 arithmetic form
     int zc;
     int xcd_remap;
+    long long bstride = 0;   // 2D whole step: elements between the fields of a batch (blockIdx.z)
 };
 
 // BC image of position p for ghost depths up to D (needs n >= D); -1: FIXED.
@@ -1656,7 +1657,7 @@ __global__ __launch_bounds__(NT) void k_ch_step_2d(PairArgs<T> A) {
     }
     const int x0 = bx * TX, y0 = by * TY;
     const int g = A.L.g;
-    const long long zb = (long long)g * A.L.sxy;
+    const long long zb = (long long)g * A.L.sxy + (long long)blockIdx.z * A.bstride;
     const bool xy_in = x0 >= 8 && x0 + TX + 8 <= A.L.n0 && y0 >= 8 && y0 + TY + 8 <= A.L.n1;
 
     // c_n on the tile and an 8-cell ring, x/y BC images applied: offsets
@@ -2361,13 +2362,16 @@ static int rk4_pair_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_param
 // layer gets the BC fill of that state; the last step writes w0 and its
 // physical cells are copied into c. c thus ends with the state and the ghost
 // layer the one-stage path leaves, and every step runs the whole-step kernel.
+// nbatch > 1: nbatch fields of this grid back to back in c (and each work
+// buffer), all advanced by every launch (blockIdx.z = field).
 template <typename T, bool EXACT>
 static int rk4_step2d_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_params* p, int nsteps, hipStream_t st,
-                              int stile, int pure) {
+                              int stile, int pure, int nbatch = 1) {
     const Layout L = make_layout(G);
     const long long N = L.total();
+    const long long NB = N * nbatch;
     T* w0 = work;
-    T* w1 = work + N;
+    T* w1 = work + NB;
     PairArgs<T> P;
     P.L = L;
     P.bx = axis_bc<T>(G, 0);
@@ -2386,8 +2390,9 @@ static int rk4_step2d_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_par
     P.zc = 1;
     stile = step2d_tile(stile, sizeof(T));
     const TileZ tl = kTilesS2[stile];
-    const dim3 grid(cdiv(L.n0, tl.tx), cdiv(L.n1, tl.ty), 1);
+    const dim3 grid(cdiv(L.n0, tl.tx), cdiv(L.n1, tl.ty), nbatch);
     P.xcd_remap = g_xcd_remap && (grid.x * grid.y) % 8 == 0;
+    P.bstride = N;
     auto step = [&](const T* x, T* y) {
         P.u = x;
         P.out = y;
@@ -2409,15 +2414,18 @@ static int rk4_step2d_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_par
     const bool even = nsteps % 2 == 0;
     T* X = even ? w1 : c;
     T* Y = even ? c : w1;
-    if (even) PHX_HIP(hipMemcpyAsync(w1, c, (size_t)N * sizeof(T), hipMemcpyDeviceToDevice, st));
+    if (even) PHX_HIP(hipMemcpyAsync(w1, c, (size_t)NB * sizeof(T), hipMemcpyDeviceToDevice, st));
     for (int s = 0; s < nsteps - 1; ++s) step(s % 2 == 0 ? X : Y, s % 2 == 0 ? Y : X);
-    int rc = bc_fill_impl(G, sizeof(T) == 8 ? PHX_F64 : PHX_F32, c, st);
-    if (rc) return rc;
+    for (int b = 0; b < nbatch; ++b) {
+        int rc = bc_fill_impl(G, sizeof(T) == 8 ? PHX_F64 : PHX_F32, c + b * N, st);
+        if (rc) return rc;
+    }
     step(c, w0);
     const size_t pitch = (size_t)L.sx * sizeof(T);
     const long long o0 = L.at(0, 0, 0);
-    PHX_HIP(hipMemcpy2DAsync(c + o0, pitch, w0 + o0, pitch, (size_t)L.n0 * sizeof(T), L.n1, hipMemcpyDeviceToDevice,
-                             st));
+    for (int b = 0; b < nbatch; ++b)
+        PHX_HIP(hipMemcpy2DAsync(c + b * N + o0, pitch, w0 + b * N + o0, pitch, (size_t)L.n0 * sizeof(T), L.n1,
+                                 hipMemcpyDeviceToDevice, st));
     PHX_LAUNCH_CHECK("phx_ch_rk4_steps");
     return PHX_OK;
 }
@@ -2670,6 +2678,29 @@ extern "C" int phx_ch_rk4_steps_noalloc(const phx_grid* G, int dtype, void* c, c
     rc = run_rk4(G, dtype, c, work, p, nsteps, as_stream(s), launch_for(G, dtype), 0);
     t_timer_off = 0;
     return rc;
+}
+
+// nbatch independent fields of one 2D grid, stored back to back, advanced
+// together: the whole-step schedule with the batch on the launch's z axis.
+int phx_ch_rk4_steps_batch(const phx_grid* G, int dtype, int nbatch, void* c, void* work, const phx_ch_params* p,
+                           int nsteps, phx_stream s) {
+    int rc = check_grid_fused(G, "phx_ch_rk4_steps_batch");
+    if (rc) return rc;
+    if ((rc = check_dtype(dtype, "phx_ch_rk4_steps_batch"))) return rc;
+    if ((rc = check_ch_params(p, "phx_ch_rk4_steps_batch"))) return rc;
+    if (!c || !work) return fail(PHX_ERR_INVALID, "phx_ch_rk4_steps_batch: NULL buffer");
+    if (nsteps < 0) return fail(PHX_ERR_INVALID, "phx_ch_rk4_steps_batch: nsteps < 0");
+    if (nbatch < 1 || nbatch > 65535) return fail(PHX_ERR_INVALID, "phx_ch_rk4_steps_batch: nbatch must be in [1, 65535]");
+    if (!step2d_ok(G))
+        return fail(PHX_ERR_UNSUPPORTED, "phx_ch_rk4_steps_batch: 2D grids with n >= 8 on both axes only");
+    if (nsteps == 0) return PHX_OK;
+    Launch3 l = launch_for(G, dtype);
+    const int stile = l.tile >= 1 ? l.tile - 1 : step2d_tile(4, dtype == PHX_F64 ? 8 : 4);
+    hipStream_t st = as_stream(s);
+    if (dtype == PHX_F32)
+        return rk4_step2d_steps_t<float, false>(G, (float*)c, (float*)work, p, nsteps, st, stile, 0, nbatch);
+    return g_fp64_exact ? rk4_step2d_steps_t<double, true>(G, (double*)c, (double*)work, p, nsteps, st, stile, 0, nbatch)
+                        : rk4_step2d_steps_t<double, false>(G, (double*)c, (double*)work, p, nsteps, st, stile, 0, nbatch);
 }
 
 // Times every launch candidate on `steps` steps of the caller's state

@@ -501,6 +501,222 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
     p[t] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
 }
 
+// ------------------------------------------- fused inference forward -----
+// One launch evaluates layers 1-6 for a 16x16 output tile: x = c - 0.5 on the
+// tile plus a 6-pixel ring (28x28, circular wrap), a1 on 26x26, a2..a5 on
+// 24..18, d on the tile; each layer's activations stay in LDS (BF16, the NHWC
+// pixel pitch of k_conv_mid). With circular padding every layer is
+// translation invariant, so a ring pixel's activation equals the activation
+// at its wrapped position: the tile's d equals the layer-by-layer result.
+// Layers 1 and 6 do k_conv_in_fwd's and k_conv_out_fwd's operations in their
+// order; the 32->32 layers issue k_conv_mid's MFMAs with the two operands
+// exchanged (D^T = W^T A^T, the same products and taps), so each lane ends
+// with 4 consecutive output channels of one pixel and stores them in one
+// 8-byte LDS write. The region is walked as a flat list of pixels in blocks
+// of 16 (the MFMA's N), two blocks per wave at a time. Inference only: the
+// activations are not kept for a backward pass.
+constexpr int FT = 16, FNT = 512;
+constexpr int FR0 = FT + 12, FR1 = FT + 10;
+
+__global__ __launch_bounds__(FNT) void k_sur_fused_fwd(const float* __restrict__ c, const float* __restrict__ P,
+                                                       const __bf16* __restrict__ wpk, float* __restrict__ d,
+                                                       double* __restrict__ tpart, int H, int W) {
+    __shared__ float X[FR0 * FR0];
+    __shared__ float W1[CH * 10];   // layer 1: w [co][tap], then b
+    __shared__ __attribute__((aligned(16))) __bf16 A[2][FR1 * FR1 * PIXP];
+    __shared__ double RS[FT * FT], RC[FT * FT];
+    __shared__ __attribute__((aligned(16))) float W6T[9 * CH];   // layer 6: w [tap][ci]
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int x0 = blockIdx.x * FT, y0 = blockIdx.y * FT;
+    const long long img = (long long)blockIdx.z * H * W;
+    for (int t = tid; t < FR0 * FR0; t += FNT) {
+        const int yy = wrap(y0 - 6 + t / FR0, H), xx = wrap(x0 - 6 + t % FR0, W);
+        X[t] = c[img + (long long)yy * W + xx] - 0.5f;
+    }
+    for (int t = tid; t < CH * 10; t += FNT) W1[t] = P[P_W1 + t];
+    for (int t = tid; t < 9 * CH; t += FNT) W6T[t] = P[P_W6 + (t % CH) * 9 + t / CH];
+    __syncthreads();
+    // layer 1 (1 -> 32) on the 26x26 region: work item = (pixel, group of 8
+    // channels); FNT is a multiple of 4, so a thread's group is fixed and its
+    // 72 weights and 8 biases are read once
+    {
+        static_assert(FNT % 4 == 0, "channel group per thread");
+        const int grp = tid & 3;
+        float w[8][9], bb[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            bb[k] = W1[CH * 9 + 8 * grp + k];
+#pragma unroll
+            for (int t = 0; t < 9; ++t) w[k][t] = W1[(8 * grp + k) * 9 + t];
+        }
+        for (int it = tid; it < FR1 * FR1 * 4; it += FNT) {
+            const int p = it >> 2, py = p / FR1, px = p - py * FR1;
+            float v[9];
+#pragma unroll
+            for (int t = 0; t < 9; ++t) v[t] = X[(py + t / 3) * FR0 + px + t % 3];
+            __attribute__((aligned(16))) __bf16 o[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                float s = bb[k];
+#pragma unroll
+                for (int t = 0; t < 9; ++t) s += w[k][t] * v[t];
+                o[k] = f2bf(fmaxf(s, 0.f));
+            }
+            *reinterpret_cast<uint4*>(&A[0][p * PIXP + 8 * grp]) = *reinterpret_cast<const uint4*>(o);
+        }
+    }
+    __syncthreads();
+    // layers 2-5 (32 -> 32, MFMA): input region Rin, output region Rin - 2.
+    // A layer's weights are loaded while the layer before it computes.
+    // (with 1024 threads the registers for the prefetch are not there)
+    constexpr bool PREF = FNT <= 512;
+    bf16x8 bw[NTAP][2], bwn[NTAP][2];
+    auto load_w = [&](int l, bf16x8 (&r)[NTAP][2]) {
+        const __bf16* wl = wpk + l * MID_W;
+#pragma unroll
+        for (int tap = 0; tap < NTAP; ++tap)
+#pragma unroll
+            for (int mb = 0; mb < 2; ++mb)
+                r[tap][mb] = *reinterpret_cast<const bf16x8*>(wl + (tap * CH + (lane & 15) + 16 * mb) * CH + 8 * (lane >> 4));
+    };
+    load_w(0, bw);
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+        // unrolled: every layer's region size is a constant (cheap pixel decode)
+        const int rin = FR1 - 2 * l, rout = rin - 2, npix = rout * rout, nblk = (npix + 15) / 16;
+        const __bf16* in = A[l & 1];
+        __bf16* out = A[(l + 1) & 1];
+        const float* bl = P + P_MID + l * (MID_W + CH) + MID_W;
+        if (PREF) load_w(min(l + 1, 3), bwn);
+        else if (l > 0) load_w(l, bw);
+        float bias[2][4];
+#pragma unroll
+        for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) bias[mb][r] = bl[16 * mb + 4 * (lane >> 4) + r];
+        for (int nb0 = wv; nb0 < nblk; nb0 += 16) {
+            int pix[2], base[2];
+            bool val[2];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                pix[k] = (nb0 + 8 * k) * 16 + (lane & 15);
+                val[k] = nb0 + 8 * k < nblk && pix[k] < npix;
+                const int pc = min(pix[k], npix - 1);
+                const int py = pc / rout, px = pc - py * rout;
+                base[k] = (py * rin + px) * PIXP + 8 * (lane >> 4);
+            }
+            f32x4 acc[2][2];
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+#pragma unroll
+                for (int mb = 0; mb < 2; ++mb) acc[k][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int tap = 0; tap < NTAP; ++tap) {
+                const int off = ((tap / 3) * rin + tap % 3) * PIXP;
+                bf16x8 b[2];
+#pragma unroll
+                for (int k = 0; k < 2; ++k) b[k] = *reinterpret_cast<const bf16x8*>(&in[base[k] + off]);
+#pragma unroll
+                for (int k = 0; k < 2; ++k)
+#pragma unroll
+                    for (int mb = 0; mb < 2; ++mb)
+                        acc[k][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[tap][mb], b[k], acc[k][mb], 0, 0, 0);
+            }
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                if (!val[k]) continue;
+#pragma unroll
+                for (int mb = 0; mb < 2; ++mb) {
+                    __attribute__((aligned(8))) __bf16 o[4];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) o[r] = f2bf(fmaxf(acc[k][mb][r] + bias[mb][r], 0.f));
+                    *reinterpret_cast<uint2*>(&out[pix[k] * PIXP + 16 * mb + 4 * (lane >> 4)]) =
+                        *reinterpret_cast<const uint2*>(o);
+                }
+            }
+        }
+        if (PREF) {
+#pragma unroll
+            for (int tap = 0; tap < NTAP; ++tap)
+#pragma unroll
+                for (int mb = 0; mb < 2; ++mb) bw[tap][mb] = bwn[tap][mb];
+        }
+        __syncthreads();
+    }
+    // layer 6 (32 -> 1) on the tile, from a5 on 18x18 (in A[0]), the
+    // operations of k_conv_out_fwd in its order; the weights are read from
+    // LDS four at a time (every lane the same address)
+    if (tid < FT * FT) {
+        const int py = tid / FT, px = tid % FT;
+        float s = P[P_B6];
+#pragma unroll 3
+        for (int t = 0; t < 9; ++t) {
+            const uint4* src = reinterpret_cast<const uint4*>(&A[0][((py + t / 3) * (FT + 2) + px + t % 3) * PIXP]);
+            const float4* wt = reinterpret_cast<const float4*>(&W6T[t * CH]);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                uint4 u = src[q];
+                const float4 wa = wt[2 * q], wb = wt[2 * q + 1];
+                const float wv8[8] = {wa.x, wa.y, wa.z, wa.w, wb.x, wb.y, wb.z, wb.w};
+                const __bf16* e = reinterpret_cast<const __bf16*>(&u);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) s += wv8[j] * bf2f(e[j]);
+            }
+        }
+        d[img + (long long)(y0 + py) * W + x0 + px] = s;
+        RS[tid] = (double)s;
+        RC[tid] = 0.0;
+    }
+    __syncthreads();
+    // the tile's sum of d, compensated, in a fixed tree order
+    for (int h = FT * FT / 2; h > 0; h >>= 1) {
+        if (tid < h) {
+            const double a = RS[tid], b = RS[tid + h], t = a + b;
+            const double e = fabs(a) >= fabs(b) ? (a - t) + b : (b - t) + a;
+            RS[tid] = t;
+            RC[tid] += RC[tid + h] + e;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const long long tile = ((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        tpart[2 * tile] = RS[0];
+        tpart[2 * tile + 1] = RC[0];
+    }
+}
+
+// c' = c + d - mean(d) per sample, the mean from the fused kernel's per-tile
+// compensated sums (ntile of them per sample, summed in tile order by every
+// block alike).
+__global__ __launch_bounds__(256) void k_sur_project_tiles(const float* __restrict__ c, const float* __restrict__ d,
+                                                           const double* __restrict__ tpart, int ntile, int HW,
+                                                           long long n, float* __restrict__ out) {
+    __shared__ double S[256], Cc[256];
+    const long long p0 = blockIdx.x * 256LL;
+    const int smp = (int)(p0 / HW);   // HW is a multiple of 256: one sample per block
+    const double* tp = tpart + 2LL * smp * ntile;
+    double s = 0.0, cmp = 0.0;
+    for (int k = threadIdx.x; k < ntile; k += 256) {
+        neu_add(s, cmp, tp[2 * k]);
+        cmp += tp[2 * k + 1];
+    }
+    S[threadIdx.x] = s;
+    Cc[threadIdx.x] = cmp;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            const double a = S[threadIdx.x], b = S[threadIdx.x + h], t = a + b;
+            const double e = fabs(a) >= fabs(b) ? (a - t) + b : (b - t) + a;
+            S[threadIdx.x] = t;
+            Cc[threadIdx.x] += Cc[threadIdx.x + h] + e;
+        }
+        __syncthreads();
+    }
+    const double mean = (S[0] + Cc[0]) / HW;
+    const long long p = p0 + threadIdx.x;
+    if (p < n) out[p] = (float)((double)c[p] + ((double)d[p] - mean));
+}
+
 // --------------------------------------------------------------- host ----
 struct SurWS {
     // carved from the caller's workspace
@@ -512,6 +728,7 @@ struct SurWS {
     double* sums;     // per-sample sums (B)
     double* loss;     // 1
     double* spart;    // SUM_BLOCKS * 2 * B partial sums
+    double* tpart;    // fused forward: 2 per 16x16 tile
     float* part;      // WG_BLOCKS * PART
     __bf16* wfwd;     // 4 * MID_W
     __bf16* wbwd;     // 4 * MID_W
@@ -536,6 +753,7 @@ static size_t ws_layout(int B, int H, int W, SurWS* w, char* base) {
     s.sums = (double*)take((size_t)B * 8 + 8);
     s.loss = (double*)take(16);
     s.spart = (double*)take((size_t)SUM_BLOCKS * 2 * B * 8);
+    s.tpart = (double*)take((size_t)2 * B * (H / 16) * (W / 16) * 8);
     s.part = (float*)take((size_t)WG_BLOCKS * PART * 4);
     s.wfwd = (__bf16*)take(4 * MID_W * 2);
     s.wbwd = (__bf16*)take(4 * MID_W * 2);
@@ -557,11 +775,20 @@ static int sur_forward(const float* P, SurWS& S, const float* c, float* out, int
     const int nb = (int)((n + 255) / 256);
     if (!(flags & PHX_SUR_REUSE_WEIGHTS))
         hipLaunchKernelGGL(k_pack, dim3((4 * MID_W + 255) / 256), dim3(256), 0, st, P, S.wfwd, S.wbwd);
-    hipLaunchKernelGGL(k_conv_in_fwd, dim3(nb), dim3(256), 0, st, c, P, S.act[0], B, H, W);
-    for (int l = 0; l < 4; ++l)
-        hipLaunchKernelGGL(k_conv_mid, dim3(W / 16, H / 16, B), dim3(256), 0, st, S.act[l], S.wfwd + l * MID_W,
-                           P + P_MID + l * (MID_W + CH) + MID_W, (const __bf16*)nullptr, S.act[l + 1], H, W, 0);
-    hipLaunchKernelGGL(k_conv_out_fwd, dim3(nb), dim3(256), 0, st, S.act[4], P, S.d, B, H, W);
+    if (flags & PHX_SUR_FUSED) {
+        const int ntile = (H / FT) * (W / FT);
+        hipLaunchKernelGGL(k_sur_fused_fwd, dim3(W / FT, H / FT, B), dim3(FNT), 0, st, c, P, S.wfwd, S.d, S.tpart, H,
+                           W);
+        hipLaunchKernelGGL(k_sur_project_tiles, dim3(nb), dim3(256), 0, st, c, S.d, S.tpart, ntile, H * W, n, out);
+        PHX_LAUNCH_CHECK("phx_sur_forward");
+        return PHX_OK;
+    } else {
+        hipLaunchKernelGGL(k_conv_in_fwd, dim3(nb), dim3(256), 0, st, c, P, S.act[0], B, H, W);
+        for (int l = 0; l < 4; ++l)
+            hipLaunchKernelGGL(k_conv_mid, dim3(W / 16, H / 16, B), dim3(256), 0, st, S.act[l], S.wfwd + l * MID_W,
+                               P + P_MID + l * (MID_W + CH) + MID_W, (const __bf16*)nullptr, S.act[l + 1], H, W, 0);
+        hipLaunchKernelGGL(k_conv_out_fwd, dim3(nb), dim3(256), 0, st, S.act[4], P, S.d, B, H, W);
+    }
     seg_sums(S.d, (long long)H * W, B, S.spart, S.sums, st);
     hipLaunchKernelGGL(k_project, dim3(nb), dim3(256), 0, st, c, S.d, S.sums, H * W, n, out);
     PHX_LAUNCH_CHECK("phx_sur_forward");

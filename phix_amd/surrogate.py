@@ -156,21 +156,24 @@ class Surrogate:
                   self.n, self.n, 1 if reuse_weights else 0, capi.stream_handle())
         return self.out
 
-    def forward_to(self, c, out, reuse_weights=True):
-        """Forward into a caller buffer (B x n x n FP32, may be c itself: the
-        projection, the last kernel, is elementwise)."""
+    def forward_to(self, c, out, reuse_weights=True, fused=True):
+        """Inference forward into a caller buffer (B x n x n FP32, may be c
+        itself: the projection, the last kernel, is elementwise). fused: the
+        six layers per 16x16 tile in one launch (PHX_SUR_FUSED; no
+        activations are kept, so no backward may follow)."""
         assert c.shape == out.shape == (self.B, self.n, self.n) and out.is_contiguous()
+        flags = (1 if reuse_weights else 0) | (2 if fused else 0)
         capi.call("phx_sur_forward", capi.ptr(self.p), capi.ptr(self.ws), capi.ptr(c), capi.ptr(out), self.B,
-                  self.n, self.n, 1 if reuse_weights else 0, capi.stream_handle())
+                  self.n, self.n, flags, capi.stream_handle())
         return out
 
-    def rollout(self, c, jumps, bufs):
+    def rollout(self, c, jumps, bufs, fused=True):
         """`jumps` forwards from c, alternating between the two buffers of
         `bufs`; returns the buffer holding the result."""
         cur = c
         for j in range(jumps):
             nxt = bufs[j % 2]
-            self.forward_to(cur, nxt)
+            self.forward_to(cur, nxt, fused=fused)
             cur = nxt
         return cur
 
@@ -404,10 +407,11 @@ class PararealSolver:
 
     The run [0, T] is cut into `slices` slices of `slice_steps` solver steps.
     The surrogate is the coarse propagator G (slice_steps/every jumps per
-    slice), the fused FP32 RK4 solver the fine propagator F. F is applied to
-    every slice at once as one ensemble launch (the persistent ensemble kernel
-    with a hipGraph step loop) -- on one GPU a 256^2 run leaves most of the
-    chip idle, so all slices cost about what one does. Each iteration then
+    slice), the plain solver the fine propagator F. F is applied to every
+    slice at once (phx_ch_rk4_steps_batch: the plain solver's whole-step
+    launch with the slices on its z axis, each slice's result the plain
+    solver's bits) -- on one GPU a 256^2 run leaves most of the chip idle, so
+    all slices cost far less than as many runs. Each iteration then
     sweeps the slices in order:
         U[s+1] <- G(U_new[s]) + F(U_old[s]) - G(U_old[s]),
     which makes the first k slices exact after k iterations; the surrogate's
@@ -425,23 +429,18 @@ class PararealSolver:
         self.dev, self.n, self.dt, self.every, self.kappa = dev, n, dt, every, kappa
         self.S, self.steps = slices, slice_steps
         self.net = Surrogate(dev, 1, n=n, params=params)
-        h = C.c_void_p()
-        sides = (C.c_int * slices)(*([n] * slices))
-        kap = (C.c_double * slices)(*([kappa] * slices))
-        capi.call("phx_ensemble_create", slices, sides, kap, None, C.byref(h))
-        self.h = h
-        cells = capi.lib().phx_ensemble_cells(h)
-        self.arena = torch.empty((slices, n, n), dtype=torch.float32, device=dev)
-        self.work = torch.empty(3 * cells, dtype=torch.float32, device=dev)
+        # the fine propagator: the plain solver's own RK4 launch, all slices
+        # at once (phx_ch_rk4_steps_batch: one stored 2D field per slice)
         self.judge = PlainSolver(dev, n=n, kappa=kappa, dt=dt)
+        self.arena = torch.zeros((slices,) + tuple(self.judge.shape), dtype=torch.float32, device=dev)
+        self.work = torch.empty(slices * self.judge.work.numel(), dtype=torch.float32, device=dev)
+        self.h = None
         self.bufs = [torch.empty((1, n, n), dtype=torch.float32, device=dev) for _ in range(2)]
         self.graph = None
         self.lead = None
 
     def close(self):
-        if self.h:
-            capi.lib().phx_ensemble_destroy(self.h)
-            self.h = None
+        pass
 
     def G(self, u):
         if self.graph is None:
@@ -449,19 +448,27 @@ class PararealSolver:
         return self.graph(u).clone()
 
     def F_all(self, U):
-        for s in range(self.S):
-            self.arena[s].copy_(U[s][0])
-        per = min(self.steps, 100)
-        capi.call("phx_ensemble_graph_steps", self.h, capi.ptr(self.arena), capi.ptr(self.work), self.dt, per,
-                  self.steps // per, capi.stream_handle())
-        return self.arena.clone()
+        import torch
+        n, J = self.n, self.judge
+        phys = self.arena[:, 1, 1:n + 1, 1:n + 1]
+        phys.copy_(torch.cat(U[: self.S]))
+        capi.call("phx_ch_rk4_steps_batch", C.byref(J.g), J.dtype, self.S, capi.ptr(self.arena), capi.ptr(self.work),
+                  C.byref(J.p), self.steps, capi.stream_handle())
+        return phys.clone()
 
     def energies(self, U):
-        es = []
-        for u in U:
-            self.judge.set(u[0])
-            es.append(self.judge.mass_energy()[1])
-        return es
+        """Free energies of the states U, each reduced on the device into its
+        own slot; one readback for all of them."""
+        import torch
+        if getattr(self, "_eslots", None) is None or self._eslots.numel() < 2 * len(U):
+            self._eslots = torch.empty(2 * len(U), dtype=torch.float64, device=self.dev)
+        J = self.judge
+        base = capi.ptr(self._eslots)
+        for k, u in enumerate(U):
+            J.set(u[0])
+            capi.call("phx_mass_energy", C.byref(J.g), J.dtype, capi.ptr(J.c), C.byref(J.p), capi.ptr(J.red),
+                      C.c_void_p(base + 16 * k), capi.stream_handle())
+        return self._eslots[: 2 * len(U)].view(len(U), 2)[:, 1].tolist()
 
     def run(self, c0, iters, lead_steps=0):
         """Advances c0 (n x n FP32 device tensor) by lead_steps plain solver
@@ -477,7 +484,7 @@ class PararealSolver:
         if lead_steps:
             if self.lead is None:
                 self.lead = PlainSolver(self.dev, n=n, kappa=self.kappa, dt=self.dt)
-                self.lead.autotune()
+                self.lead.autotune()   # the launch is kept per shape: F uses it too
             self.lead.set(c0.reshape(n, n))
             self.lead.steps(lead_steps)
             c0 = self.lead.phys()

@@ -281,3 +281,33 @@ def test_step2d_timing_kind(dev):
                 counts.append(k.value)
             capi.call("phx_ch_timing", 0)
         assert counts == [0, 0, 0, nsteps], counts
+
+
+@pytest.mark.parametrize("dtype", [capi.F32, capi.F64])
+@pytest.mark.parametrize("n,bc,vals,nsteps", [
+    ((256, 256), P, 0.0, 7),
+    ((40, 33), [[F, N], [P, P]], [[0.45, 0], [0, 0]], 4),
+    ((64, 48), N, 0.0, 1),
+])
+def test_batch_equals_per_field(dev, dtype, n, bc, vals, nsteps):
+    """phx_ch_rk4_steps_batch on 5 fields leaves each one exactly as
+    phx_ch_rk4_steps on that field alone (stored ghosts included)."""
+    import torch
+    g = capi.make_grid(n, bc=bc, bc_value=vals)
+    p = ch_params(rho=1.0, kappa=0.5, dt=0.01)
+    tdt, esz = (torch.float64, 8) if dtype == capi.F64 else (torch.float32, 4)
+    nb = 5
+    fields = [init.to_stored(init.ch_field(n, 60 + k, 0.5, 0.05), g) for k in range(nb)]
+    batch = torch.stack([to_dev(f, dev, tdt) for f in fields]).contiguous()
+    wbytes = capi.lib().phx_ch_rk4_workspace_bytes(C.byref(g), dtype)
+    work = torch.zeros(nb * wbytes // esz, dtype=tdt, device=dev)
+    capi.call("phx_ch_rk4_steps_batch", C.byref(g), dtype, nb, capi.ptr(batch), capi.ptr(work), C.byref(p), nsteps,
+              stream())
+    for k in range(nb):
+        one = to_dev(fields[k], dev, tdt)
+        w1 = torch.zeros(wbytes // esz, dtype=tdt, device=dev)
+        capi.call("phx_ch_rk4_steps", C.byref(g), dtype, capi.ptr(one), capi.ptr(w1), C.byref(p), nsteps, stream())
+        torch.cuda.synchronize()
+        assert torch.equal(batch[k], one), k
+    assert capi.lib().phx_ch_rk4_steps_batch(C.byref(g), dtype, 0, capi.ptr(batch), capi.ptr(work), C.byref(p), 1,
+                                             None) == capi.PHX_ERR_INVALID

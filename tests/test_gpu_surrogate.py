@@ -326,3 +326,36 @@ def test_parareal_with_solver_lead_phase(dev):
     e = float(np.linalg.norm(out.cpu().numpy() - ref) / np.linalg.norm(ref))
     print(f"lead {lead} + 19 slices, one iteration, 6000 steps: rel-L2 vs FP64 {e:.2e} in {rep.seconds * 1e3:.1f} ms")
     assert e <= 5e-2
+
+
+@pytest.mark.parametrize("B,H,W", [(1, 256, 256), (3, 48, 32), (1, 16, 16), (2, 16, 64)])
+def test_fused_forward_equals_layer_by_layer(dev, B, H, W):
+    """PHX_SUR_FUSED (six layers per 16x16 tile in one launch, activations in
+    LDS, rings wrapped at the image edges) returns the layer-by-layer
+    forward's output: the network's d is the same bits; the mean of d is the
+    same compensated FP64 sum taken in another order (per tile), so the
+    projected output may differ in the last bit of a few cells."""
+    import torch
+    from phix_amd import surrogate as S
+    p = np.load(S.WEIGHTS_LATE)["params"]
+    net = S.Surrogate(dev, B, n=H, params=p) if H == W else None
+    npar = capi.lib().phx_sur_param_count()
+    tp = torch.from_numpy(np.ascontiguousarray(p, np.float32)).to(dev)
+    ws = torch.empty(capi.lib().phx_sur_workspace_bytes(B, H, W), dtype=torch.uint8, device=dev)
+    rng = np.random.default_rng(B * 1000 + H + W)
+    c = torch.from_numpy((0.5 + 0.45 * np.tanh(3 * rng.standard_normal((B, H, W)))).astype(np.float32)).to(dev)
+    a = torch.empty_like(c)
+    b = torch.empty_like(c)
+    capi.call("phx_sur_forward", capi.ptr(tp), capi.ptr(ws), capi.ptr(c), capi.ptr(a), B, H, W, 0, stream())
+    capi.call("phx_sur_forward", capi.ptr(tp), capi.ptr(ws), capi.ptr(c), capi.ptr(b), B, H, W, 3, stream())
+    torch.cuda.synchronize()
+    assert len(p) == npar
+    diff = (a - b).abs().max().item()
+    ndiff = int((a != b).sum().item())
+    print(f"fused vs layer-by-layer B={B} {H}x{W}: max |diff| = {diff:.3e} in {ndiff} cells", flush=True)
+    assert diff <= 2.0 ** -23 and ndiff <= max(1, a.numel() // 1000), (diff, ndiff)
+    if net is not None:
+        out = torch.empty_like(c)
+        net.forward(c)
+        net.forward_to(c, out)
+        assert torch.equal(out, b)
