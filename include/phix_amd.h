@@ -388,8 +388,10 @@ size_t phx_sur_workspace_bytes(int B, int H, int W);
  * params are unchanged since the last call on this ws). PHX_SUR_FUSED
  * evaluates the six layers of each 16x16 output tile in one launch with the
  * activations in LDS (inference: the activations are not kept, so no
- * phx_sur_backward may follow); same arithmetic as the layer-by-layer
- * kernels. */
+ * phx_sur_backward may follow). Every layer runs on the MFMA there, the
+ * first and last with their FP32 operands split into BF16 high and low
+ * parts (~16 significant bits): the output follows the layer-by-layer
+ * kernels to within the BF16 rounding of the activations, not bit for bit. */
 #define PHX_SUR_REUSE_WEIGHTS 1
 #define PHX_SUR_FUSED 2
 int    phx_sur_forward(const float* params, void* ws, const float* c_in, float* c_out,

@@ -507,38 +507,103 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 // 24..18, d on the tile; each layer's activations stay in LDS (BF16, the NHWC
 // pixel pitch of k_conv_mid). With circular padding every layer is
 // translation invariant, so a ring pixel's activation equals the activation
-// at its wrapped position: the tile's d equals the layer-by-layer result.
-// Layers 1 and 6 do k_conv_in_fwd's and k_conv_out_fwd's operations in their
-// order; the 32->32 layers issue k_conv_mid's MFMAs with the two operands
-// exchanged (D^T = W^T A^T, the same products and taps), so each lane ends
-// with 4 consecutive output channels of one pixel and stores them in one
-// 8-byte LDS write. The region is walked as a flat list of pixels in blocks
-// of 16 (the MFMA's N), two blocks per wave at a time. Inference only: the
-// activations are not kept for a backward pass.
+// at its wrapped position.
+//
+// Layer 1 (1 -> 32) does k_conv_in_fwd's FP32 operations in its order, so
+// a1 is the layer-by-layer kernel's bits. Layers 2-6 run on
+// v_mfma_f32_16x16x32_bf16 in the transposed form D^T = W^T A^T (rows =
+// output channels, columns = 16 pixels), so a lane ends with 4 consecutive
+// output channels of one pixel:
+//   * layers 2-5 (32 -> 32): one MFMA per tap and 16 output channels, K = the
+//     32 input channels; the weights sit in VGPRs, each layer's loaded from
+//     global memory during the layer before it;
+//   * layer 6 (32 -> 1): A rows 0 and 1 hold the high and low BF16 parts of
+//     the FP32 weights (~16 significant bits), d = D[0] + D[1].
+// Biases start the accumulators. The region is walked as a flat list of
+// pixels in blocks of 16 (the MFMA's N), two blocks per wave at a time. The
+// tile's sum of d is formed in a fixed order with compensation (per lane,
+// then a halving tree over the 16 lanes, then the 8 waves in order).
+// Inference only: no activations are kept for a backward pass. The result
+// follows the layer-by-layer forward to within FP32 summation order (the
+// bias first, the MFMA's internal order) and the split of layer 6; it is not
+// the same bits.
 constexpr int FT = 16, FNT = 512;
-constexpr int FR0 = FT + 12, FR1 = FT + 10;
+constexpr int FR0 = FT + 12;                    // 28: the x region
+constexpr int FRA = FT + 10;                    // 26: the a1 region
+constexpr int NX0 = FR0 * FR0;
+constexpr int ACT0_PIX = FRA * FRA;             // a1 (26^2), a3 (22^2), a5 (18^2)
+constexpr int ACT1_PIX = (FT + 8) * (FT + 8);   // a2 (24^2), a4 (20^2)
+// layer 6's weights split into BF16 high and low parts, [tap][high, low][ci]
+constexpr int FRAG6 = NTAP * 2 * CH, FRAG_N = FRAG6;
+
+__global__ void k_pack_frag(const float* __restrict__ P, __bf16* __restrict__ frag) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < FRAG6) {
+        const int tap = t / (2 * CH), part = (t / CH) % 2, ci = t % CH;
+        const float w = P[P_W6 + ci * 9 + tap];
+        const __bf16 hi = f2bf(w);
+        frag[t] = part == 0 ? hi : f2bf(w - bf2f(hi));
+    }
+}
+
+// compensated addition of (b, cb) into (a, ca)
+__device__ __forceinline__ void comp_add(double& a, double& ca, double b, double cb) {
+    const double t = a + b;
+    const double e = fabs(a) >= fabs(b) ? (a - t) + b : (b - t) + a;
+    a = t;
+    ca = ca + cb + e;
+}
 
 __global__ __launch_bounds__(FNT) void k_sur_fused_fwd(const float* __restrict__ c, const float* __restrict__ P,
-                                                       const __bf16* __restrict__ wpk, float* __restrict__ d,
+                                                       const __bf16* __restrict__ wpk,
+                                                       const __bf16* __restrict__ frag, float* __restrict__ d,
                                                        double* __restrict__ tpart, int H, int W) {
-    __shared__ float X[FR0 * FR0];
+    __shared__ __attribute__((aligned(16))) __bf16 ACT0[ACT0_PIX * PIXP];
+    __shared__ __attribute__((aligned(16))) __bf16 ACT1[ACT1_PIX * PIXP];
+    __shared__ float X[NX0];
     __shared__ float W1[CH * 10];   // layer 1: w [co][tap], then b
-    __shared__ __attribute__((aligned(16))) __bf16 A[2][FR1 * FR1 * PIXP];
-    __shared__ double RS[FT * FT], RC[FT * FT];
-    __shared__ __attribute__((aligned(16))) float W6T[9 * CH];   // layer 6: w [tap][ci]
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    __shared__ __attribute__((aligned(16))) __bf16 W6S[FRAG6];
+    __shared__ float BM[4 * CH];
+    __shared__ double RW[2 * (FNT / 64)];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, g = lane >> 4, lx = lane & 15;
     const int x0 = blockIdx.x * FT, y0 = blockIdx.y * FT;
     const long long img = (long long)blockIdx.z * H * W;
-    for (int t = tid; t < FR0 * FR0; t += FNT) {
+    // ---- prologue: global loads, then LDS stores
+    float xv[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int t = min(tid + k * FNT, NX0 - 1);
         const int yy = wrap(y0 - 6 + t / FR0, H), xx = wrap(x0 - 6 + t % FR0, W);
-        X[t] = c[img + (long long)yy * W + xx] - 0.5f;
+        xv[k] = c[img + (long long)yy * W + xx] - 0.5f;
     }
-    for (int t = tid; t < CH * 10; t += FNT) W1[t] = P[P_W1 + t];
-    for (int t = tid; t < 9 * CH; t += FNT) W6T[t] = P[P_W6 + (t % CH) * 9 + t / CH];
+    const uint4 w6v = tid < FRAG6 / 8 ? reinterpret_cast<const uint4*>(frag)[tid] : uint4{0, 0, 0, 0};
+    const float bmv = tid < 4 * CH ? P[P_MID + (tid / CH) * (MID_W + CH) + MID_W + tid % CH] : 0.f;
+    const float w1v = tid < CH * 10 ? P[P_W1 + tid] : 0.f;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int t = tid + k * FNT;
+        if (t < NX0) X[t] = xv[k];
+    }
+    if (tid < CH * 10) W1[tid] = w1v;
+    if (tid < FRAG6 / 8) reinterpret_cast<uint4*>(W6S)[tid] = w6v;
+    if (tid < 4 * CH) BM[tid] = bmv;
     __syncthreads();
-    // layer 1 (1 -> 32) on the 26x26 region: work item = (pixel, group of 8
-    // channels); FNT is a multiple of 4, so a thread's group is fixed and its
-    // 72 weights and 8 biases are read once
+    // layers 2-5 keep their weights in VGPRs; a layer's are loaded during the
+    // layer before it, inside that layer's first block step, so the loads are
+    // issued early and waited for at their use
+    bf16x8 bw[NTAP][2], bwn[NTAP][2];
+    auto load_w = [&](int l, bf16x8 (&r)[NTAP][2]) {
+        const __bf16* wl = wpk + l * MID_W;
+#pragma unroll
+        for (int tap = 0; tap < NTAP; ++tap)
+#pragma unroll
+            for (int mb = 0; mb < 2; ++mb)
+                r[tap][mb] = *reinterpret_cast<const bf16x8*>(wl + (tap * CH + lx + 16 * mb) * CH + 8 * g);
+    };
+    // ---- layer 1 (1 -> 32) on the 26x26 region in FP32, k_conv_in_fwd's
+    // operations in its order: work item = (pixel, group of 8 channels); FNT
+    // is a multiple of 4, so a thread's group is fixed and its 72 weights and
+    // 8 biases are read once
     {
         static_assert(FNT % 4 == 0, "channel group per thread");
         const int grp = tid & 3;
@@ -549,139 +614,137 @@ __global__ __launch_bounds__(FNT) void k_sur_fused_fwd(const float* __restrict__
 #pragma unroll
             for (int t = 0; t < 9; ++t) w[k][t] = W1[(8 * grp + k) * 9 + t];
         }
-        for (int it = tid; it < FR1 * FR1 * 4; it += FNT) {
-            const int p = it >> 2, py = p / FR1, px = p - py * FR1;
+        for (int it = tid; it < FRA * FRA * 4; it += FNT) {
+            if (it == tid) load_w(0, bwn);
+            const int p = it >> 2, py = p / FRA, px = p - py * FRA;
             float v[9];
 #pragma unroll
             for (int t = 0; t < 9; ++t) v[t] = X[(py + t / 3) * FR0 + px + t % 3];
             __attribute__((aligned(16))) __bf16 o[8];
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
-                float s = bb[k];
+                float sum = bb[k];
 #pragma unroll
-                for (int t = 0; t < 9; ++t) s += w[k][t] * v[t];
-                o[k] = f2bf(fmaxf(s, 0.f));
+                for (int t = 0; t < 9; ++t) sum += w[k][t] * v[t];
+                o[k] = f2bf(fmaxf(sum, 0.f));
             }
-            *reinterpret_cast<uint4*>(&A[0][p * PIXP + 8 * grp]) = *reinterpret_cast<const uint4*>(o);
+            *reinterpret_cast<uint4*>(&ACT0[p * PIXP + 8 * grp]) = *reinterpret_cast<const uint4*>(o);
         }
     }
-    __syncthreads();
-    // layers 2-5 (32 -> 32, MFMA): input region Rin, output region Rin - 2.
-    // A layer's weights are loaded while the layer before it computes.
-    // (with 1024 threads the registers for the prefetch are not there)
-    constexpr bool PREF = FNT <= 512;
-    bf16x8 bw[NTAP][2], bwn[NTAP][2];
-    auto load_w = [&](int l, bf16x8 (&r)[NTAP][2]) {
-        const __bf16* wl = wpk + l * MID_W;
+    // ---- layers 2-5 (32 -> 32): input region rin, output region rin - 2
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
 #pragma unroll
         for (int tap = 0; tap < NTAP; ++tap)
 #pragma unroll
-            for (int mb = 0; mb < 2; ++mb)
-                r[tap][mb] = *reinterpret_cast<const bf16x8*>(wl + (tap * CH + (lane & 15) + 16 * mb) * CH + 8 * (lane >> 4));
-    };
-    load_w(0, bw);
-#pragma unroll
-    for (int l = 0; l < 4; ++l) {
-        // unrolled: every layer's region size is a constant (cheap pixel decode)
-        const int rin = FR1 - 2 * l, rout = rin - 2, npix = rout * rout, nblk = (npix + 15) / 16;
-        const __bf16* in = A[l & 1];
-        __bf16* out = A[(l + 1) & 1];
-        const float* bl = P + P_MID + l * (MID_W + CH) + MID_W;
-        if (PREF) load_w(min(l + 1, 3), bwn);
-        else if (l > 0) load_w(l, bw);
+            for (int mb = 0; mb < 2; ++mb) bw[tap][mb] = bwn[tap][mb];
         float bias[2][4];
 #pragma unroll
         for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) bias[mb][r] = bl[16 * mb + 4 * (lane >> 4) + r];
+            for (int r = 0; r < 4; ++r) bias[mb][r] = BM[l * CH + 16 * mb + 4 * g + r];
+        if (l == 0) __syncthreads();   // a1 complete
+        const int rin = FRA - 2 * l, rout = rin - 2, npix = rout * rout, nblk = (npix + 15) / 16;
+        const __bf16* in = (l & 1) ? ACT1 : ACT0;
+        __bf16* out = (l & 1) ? ACT0 : ACT1;
         for (int nb0 = wv; nb0 < nblk; nb0 += 16) {
+            if (l < 3 && nb0 == wv) load_w(l + 1, bwn);
+            const bool two = nb0 + 8 < nblk;
             int pix[2], base[2];
-            bool val[2];
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
-                pix[k] = (nb0 + 8 * k) * 16 + (lane & 15);
-                val[k] = nb0 + 8 * k < nblk && pix[k] < npix;
+                pix[k] = (nb0 + 8 * k) * 16 + lx;
                 const int pc = min(pix[k], npix - 1);
                 const int py = pc / rout, px = pc - py * rout;
-                base[k] = (py * rin + px) * PIXP + 8 * (lane >> 4);
+                base[k] = (py * rin + px) * PIXP + 8 * g;
             }
             f32x4 acc[2][2];
 #pragma unroll
             for (int k = 0; k < 2; ++k)
 #pragma unroll
-                for (int mb = 0; mb < 2; ++mb) acc[k][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
+                for (int mb = 0; mb < 2; ++mb) acc[k][mb] = f32x4{bias[mb][0], bias[mb][1], bias[mb][2], bias[mb][3]};
+            if (two) {
+                bf16x8 bb[NTAP][2];
 #pragma unroll
-            for (int tap = 0; tap < NTAP; ++tap) {
-                const int off = ((tap / 3) * rin + tap % 3) * PIXP;
-                bf16x8 b[2];
+                for (int tap = 0; tap < NTAP; ++tap)
 #pragma unroll
-                for (int k = 0; k < 2; ++k) b[k] = *reinterpret_cast<const bf16x8*>(&in[base[k] + off]);
+                    for (int k = 0; k < 2; ++k)
+                        bb[tap][k] = *reinterpret_cast<const bf16x8*>(&in[base[k] + ((tap / 3) * rin + tap % 3) * PIXP]);
 #pragma unroll
-                for (int k = 0; k < 2; ++k)
+                for (int tap = 0; tap < NTAP; ++tap)
+#pragma unroll
+                    for (int k = 0; k < 2; ++k)
+#pragma unroll
+                        for (int mb = 0; mb < 2; ++mb)
+                            acc[k][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[tap][mb], bb[tap][k], acc[k][mb], 0, 0, 0);
+            } else {
+                bf16x8 bb[NTAP];
+#pragma unroll
+                for (int tap = 0; tap < NTAP; ++tap)
+                    bb[tap] = *reinterpret_cast<const bf16x8*>(&in[base[0] + ((tap / 3) * rin + tap % 3) * PIXP]);
+#pragma unroll
+                for (int tap = 0; tap < NTAP; ++tap)
 #pragma unroll
                     for (int mb = 0; mb < 2; ++mb)
-                        acc[k][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[tap][mb], b[k], acc[k][mb], 0, 0, 0);
+                        acc[0][mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[tap][mb], bb[tap], acc[0][mb], 0, 0, 0);
             }
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
-                if (!val[k]) continue;
+                if (pix[k] >= npix || (k == 1 && !two)) continue;
 #pragma unroll
                 for (int mb = 0; mb < 2; ++mb) {
                     __attribute__((aligned(8))) __bf16 o[4];
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) o[r] = f2bf(fmaxf(acc[k][mb][r] + bias[mb][r], 0.f));
-                    *reinterpret_cast<uint2*>(&out[pix[k] * PIXP + 16 * mb + 4 * (lane >> 4)]) =
-                        *reinterpret_cast<const uint2*>(o);
+                    for (int r = 0; r < 4; ++r) o[r] = f2bf(fmaxf(acc[k][mb][r], 0.f));
+                    *reinterpret_cast<uint2*>(&out[pix[k] * PIXP + 16 * mb + 4 * g]) = *reinterpret_cast<const uint2*>(o);
                 }
             }
         }
-        if (PREF) {
-#pragma unroll
-            for (int tap = 0; tap < NTAP; ++tap)
-#pragma unroll
-                for (int mb = 0; mb < 2; ++mb) bw[tap][mb] = bwn[tap][mb];
-        }
         __syncthreads();
     }
-    // layer 6 (32 -> 1) on the tile, from a5 on 18x18 (in A[0]), the
-    // operations of k_conv_out_fwd in its order; the weights are read from
-    // LDS four at a time (every lane the same address)
-    if (tid < FT * FT) {
-        const int py = tid / FT, px = tid % FT;
-        float s = P[P_B6];
-#pragma unroll 3
-        for (int t = 0; t < 9; ++t) {
-            const uint4* src = reinterpret_cast<const uint4*>(&A[0][((py + t / 3) * (FT + 2) + px + t % 3) * PIXP]);
-            const float4* wt = reinterpret_cast<const float4*>(&W6T[t * CH]);
+    // ---- layer 6 (32 -> 1): wave w computes tile rows w and w + 8 from a5
+    // (18x18, in ACT0); lanes 0-15 hold D rows 0 and 1
+    float dv[2];
+    {
+        bf16x8 w6[NTAP];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                uint4 u = src[q];
-                const float4 wa = wt[2 * q], wb = wt[2 * q + 1];
-                const float wv8[8] = {wa.x, wa.y, wa.z, wa.w, wb.x, wb.y, wb.z, wb.w};
-                const __bf16* e = reinterpret_cast<const __bf16*>(&u);
+        for (int tap = 0; tap < NTAP; ++tap)
+            w6[tap] = lx < 2 ? *reinterpret_cast<const bf16x8*>(&W6S[(tap * 2 + lx) * CH + 8 * g]) : bf16x8{};
+        const float b6 = P[P_B6];
 #pragma unroll
-                for (int j = 0; j < 8; ++j) s += wv8[j] * bf2f(e[j]);
+        for (int k = 0; k < 2; ++k) {
+            const int row = wv + 8 * k;
+            const int base = (row * (FT + 2) + lx) * PIXP + 8 * g;
+            f32x4 acc = {g == 0 ? b6 : 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int tap = 0; tap < NTAP; ++tap) {
+                const bf16x8 bx = *reinterpret_cast<const bf16x8*>(&ACT0[base + ((tap / 3) * (FT + 2) + tap % 3) * PIXP]);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w6[tap], bx, acc, 0, 0, 0);
             }
+            dv[k] = acc[0] + acc[1];
+            if (g == 0) d[img + (long long)(y0 + row) * W + x0 + lx] = dv[k];
         }
-        d[img + (long long)(y0 + py) * W + x0 + px] = s;
-        RS[tid] = (double)s;
-        RC[tid] = 0.0;
+    }
+    // ---- the tile's sum of d, compensated, in a fixed order
+    double sv = (double)dv[0], cv = 0.0;
+    comp_add(sv, cv, (double)dv[1], 0.0);
+#pragma unroll
+    for (int h = 8; h > 0; h >>= 1) {
+        const double so = __shfl_down(sv, h, 16), co = __shfl_down(cv, h, 16);
+        comp_add(sv, cv, so, co);
+    }
+    if (lane == 0) {
+        RW[2 * wv] = sv;
+        RW[2 * wv + 1] = cv;
     }
     __syncthreads();
-    // the tile's sum of d, compensated, in a fixed tree order
-    for (int h = FT * FT / 2; h > 0; h >>= 1) {
-        if (tid < h) {
-            const double a = RS[tid], b = RS[tid + h], t = a + b;
-            const double e = fabs(a) >= fabs(b) ? (a - t) + b : (b - t) + a;
-            RS[tid] = t;
-            RC[tid] += RC[tid + h] + e;
-        }
-        __syncthreads();
-    }
     if (tid == 0) {
+        double s = RW[0], cc = RW[1];
+#pragma unroll
+        for (int w = 1; w < FNT / 64; ++w) comp_add(s, cc, RW[2 * w], RW[2 * w + 1]);
         const long long tile = ((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-        tpart[2 * tile] = RS[0];
-        tpart[2 * tile + 1] = RC[0];
+        tpart[2 * tile] = s;
+        tpart[2 * tile + 1] = cc;
     }
 }
 
@@ -732,6 +795,7 @@ struct SurWS {
     float* part;      // WG_BLOCKS * PART
     __bf16* wfwd;     // 4 * MID_W
     __bf16* wbwd;     // 4 * MID_W
+    __bf16* frag;     // FRAG_N: operand tables of the fused forward
 };
 
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -757,6 +821,7 @@ static size_t ws_layout(int B, int H, int W, SurWS* w, char* base) {
     s.part = (float*)take((size_t)WG_BLOCKS * PART * 4);
     s.wfwd = (__bf16*)take(4 * MID_W * 2);
     s.wbwd = (__bf16*)take(4 * MID_W * 2);
+    s.frag = (__bf16*)take(FRAG_N * 2);
     if (w) *w = s;
     return off;
 }
@@ -773,12 +838,14 @@ static int sur_forward(const float* P, SurWS& S, const float* c, float* out, int
                        hipStream_t st) {
     const long long n = (long long)B * H * W;
     const int nb = (int)((n + 255) / 256);
-    if (!(flags & PHX_SUR_REUSE_WEIGHTS))
+    if (!(flags & PHX_SUR_REUSE_WEIGHTS)) {
         hipLaunchKernelGGL(k_pack, dim3((4 * MID_W + 255) / 256), dim3(256), 0, st, P, S.wfwd, S.wbwd);
+        hipLaunchKernelGGL(k_pack_frag, dim3((FRAG_N + 255) / 256), dim3(256), 0, st, P, S.frag);
+    }
     if (flags & PHX_SUR_FUSED) {
         const int ntile = (H / FT) * (W / FT);
-        hipLaunchKernelGGL(k_sur_fused_fwd, dim3(W / FT, H / FT, B), dim3(FNT), 0, st, c, P, S.wfwd, S.d, S.tpart, H,
-                           W);
+        hipLaunchKernelGGL(k_sur_fused_fwd, dim3(W / FT, H / FT, B), dim3(FNT), 0, st, c, P, S.wfwd, S.frag, S.d,
+                           S.tpart, H, W);
         hipLaunchKernelGGL(k_sur_project_tiles, dim3(nb), dim3(256), 0, st, c, S.d, S.tpart, ntile, H * W, n, out);
         PHX_LAUNCH_CHECK("phx_sur_forward");
         return PHX_OK;

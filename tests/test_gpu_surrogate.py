@@ -116,12 +116,11 @@ def unpack(p):
     return out
 
 
-def ref_forward_backward(p, c, tgt):
-    """FP64 network with the GPU's BF16 roundings (weights of the 32->32 layers,
-    every activation and every activation gradient), built on the conv oracle.
-    Returns (out, loss, grad vector)."""
+def ref_forward(p, c):
+    """FP64 forward with the GPU's BF16 roundings (weights of the 32->32
+    layers, every activation), built on the conv oracle. Returns (out, the
+    activations, the pre-activations)."""
     L = unpack(p.astype(np.float64))
-    B, H, W = c.shape
     x = (c - 0.5)[:, None].astype(np.float64)
     acts, pre = [x], []
     for i, (w, b) in enumerate(L):
@@ -131,7 +130,15 @@ def ref_forward_backward(p, c, tgt):
         if i < 5:
             acts.append(bf16_round(np.maximum(z, 0.0)).astype(np.float64))
     d = pre[-1][:, 0]
-    out = c + d - d.mean(axis=(1, 2), keepdims=True)
+    return c + d - d.mean(axis=(1, 2), keepdims=True), acts, pre
+
+
+def ref_forward_backward(p, c, tgt):
+    """FP64 network with the GPU's BF16 roundings (weights of the 32->32 layers,
+    every activation and every activation gradient), built on the conv oracle.
+    Returns (out, loss, grad vector)."""
+    L = unpack(p.astype(np.float64))
+    out, acts, pre = ref_forward(p, c)
     n = c.size
     g = 2 * (out - tgt) / n
     dd = (g - g.mean(axis=(1, 2), keepdims=True))[:, None]
@@ -329,12 +336,17 @@ def test_parareal_with_solver_lead_phase(dev):
 
 
 @pytest.mark.parametrize("B,H,W", [(1, 256, 256), (3, 48, 32), (1, 16, 16), (2, 16, 64)])
-def test_fused_forward_equals_layer_by_layer(dev, B, H, W):
+def test_fused_forward_against_fp64_and_layer_by_layer(dev, B, H, W):
     """PHX_SUR_FUSED (six layers per 16x16 tile in one launch, activations in
-    LDS, rings wrapped at the image edges) returns the layer-by-layer
-    forward's output: the network's d is the same bits; the mean of d is the
-    same compensated FP64 sum taken in another order (per tile), so the
-    projected output may differ in the last bit of a few cells."""
+    LDS, rings wrapped at the image edges, the 32->32 layers and the last
+    layer on the MFMA, the last with its weights split into BF16 high and low
+    parts) against the FP64 network with the GPU's BF16 roundings and against
+    the layer-by-layer forward, on the trained weights. Both GPU paths round
+    every activation to BF16 after an FP32 sum taken in its own order; a sum
+    that lands within rounding distance of a half-way point can round the
+    other way than in FP64, and such flips dominate both errors (~5e-3 of
+    max |d| layer by layer). The fused path's error is held to twice the
+    layer-by-layer path's; the mean projection is exact in both."""
     import torch
     from phix_amd import surrogate as S
     p = np.load(S.WEIGHTS_LATE)["params"]
@@ -343,17 +355,24 @@ def test_fused_forward_equals_layer_by_layer(dev, B, H, W):
     tp = torch.from_numpy(np.ascontiguousarray(p, np.float32)).to(dev)
     ws = torch.empty(capi.lib().phx_sur_workspace_bytes(B, H, W), dtype=torch.uint8, device=dev)
     rng = np.random.default_rng(B * 1000 + H + W)
-    c = torch.from_numpy((0.5 + 0.45 * np.tanh(3 * rng.standard_normal((B, H, W)))).astype(np.float32)).to(dev)
+    c_np = (0.5 + 0.45 * np.tanh(3 * rng.standard_normal((B, H, W)))).astype(np.float32)
+    c = torch.from_numpy(c_np).to(dev)
     a = torch.empty_like(c)
     b = torch.empty_like(c)
     capi.call("phx_sur_forward", capi.ptr(tp), capi.ptr(ws), capi.ptr(c), capi.ptr(a), B, H, W, 0, stream())
     capi.call("phx_sur_forward", capi.ptr(tp), capi.ptr(ws), capi.ptr(c), capi.ptr(b), B, H, W, 3, stream())
     torch.cuda.synchronize()
     assert len(p) == npar
-    diff = (a - b).abs().max().item()
-    ndiff = int((a != b).sum().item())
-    print(f"fused vs layer-by-layer B={B} {H}x{W}: max |diff| = {diff:.3e} in {ndiff} cells", flush=True)
-    assert diff <= 2.0 ** -23 and ndiff <= max(1, a.numel() // 1000), (diff, ndiff)
+    rout = ref_forward(p, c_np.astype(np.float64))[0]
+    scale = np.abs(rout - c_np).max()
+    e_lbl = np.abs(a.cpu().numpy() - rout).max()
+    e_fused = np.abs(b.cpu().numpy() - rout).max()
+    d_ab = (a - b).abs().max().item()
+    print(f"B={B} {H}x{W}: max |d| {scale:.3e}; vs FP64: layer-by-layer {e_lbl:.3e}, fused {e_fused:.3e}; "
+          f"fused vs layer-by-layer {d_ab:.3e}", flush=True)
+    assert e_lbl <= 1e-2 * scale
+    assert e_fused <= 2 * e_lbl + 1e-3 * scale
+    np.testing.assert_allclose(b.cpu().numpy().mean(axis=(1, 2)), c_np.mean(axis=(1, 2)), atol=2e-7)
     if net is not None:
         out = torch.empty_like(c)
         net.forward(c)
