@@ -20,6 +20,8 @@ Other workloads (--workload), each a configuration of the same path:
   ens512    512 periodic CH runs, sides 64 + 32*floor(15u), RK4, FP32,
             one persistent launch per stage, hipGraph-captured step loop
   gfa600    the Fe-B glass-formation program's 600^2 case, FP64
+  surrogate 256^2 CH to t_end = 200: surrogate-plus-solver (lead-phase parareal)
+            vs the plain GPU solver, speedup and relative L2 per initial condition
   copycal   plain 2 GiB device copy (calibrates the PMC byte counters)
 
 Prints ONE JSON line. `roofline` is computed for the dominant kernel from its
@@ -546,6 +548,71 @@ def bench_gfa(a, with_cpu):
                 cfg, roof, cpu)
 
 
+def bench_surrogate(a, with_cpu):
+    """Config 5: the surrogate-plus-solver form (parareal after a plain-solver
+    lead phase; coarse = the BF16 MFMA network, fine = the batched FP32 RK4
+    solver) against the plain FP32 GPU solver, 256^2 periodic CH to t_end = 200
+    (20,000 RK4 steps), over a fixed set of initial conditions, each final
+    field's relative L2 against the FP64 GPU solver. `value` is the median
+    speedup in time to solution; `roofline` is the fused network forward's."""
+    import torch
+    import numpy as np
+    from phix_amd import capi, surrogate as S
+    dev = torch.device("cuda:0")
+    params = np.load(S.WEIGHTS_LATE)["params"]
+    ics = S.EVAL_ICS[: a.ics]
+    ev = S.evaluate_parareal(dev, params, ics=ics, reps=3)
+    # the fused forward: a 190-jump rollout (one launch per jump, one
+    # projection at the end) timed with events on the launch stream
+    net = S.Surrogate(dev, 1, params=params)
+    x = torch.from_numpy(S.config5_initial(ics[0])).float().to(dev).reshape(1, 256, 256).contiguous()
+    bufs = [torch.empty_like(x) for _ in range(2)]
+    net.forward(x)
+    jumps = 190
+    stream = torch.cuda.current_stream()
+    net.rollout(x, jumps, bufs)
+    ms = min(timed(lambda: net.rollout(x, jumps, bufs), stream) for _ in range(3))
+    per_jump = ms / 1e3 / jumps
+    px = 256 * 256
+    flops = px * (2 * 9 * 32 + 4 * 2 * 9 * 32 * 32 + 2 * 9 * 32)   # the network's multiply-adds x 2 per jump
+    ach = flops / per_jump / 1e12
+    roof = {"bound": "mfma", "achieved": round(ach, 1), "peak": 2500.0, "unit": "TFLOP/s",
+            "frac": round(ach / 2500.0, 4), "traffic": None, "kernel": "k_sur_fused_fwd",
+            "alg_flops_per_launch": flops, "avg_launch_us": round(per_jump * 1e6, 2),
+            "note": "network FLOPs of one batch-1 256^2 jump (BF16 MFMA dense peak); the 16x16 tiles recompute "
+                    "a 6-pixel ring, 1.74x these FLOPs in layers 2-5; average over a 190-jump rollout "
+                    "(190 fused launches + 1 projection)"}
+    cpu = None
+    if with_cpu:
+        from phix_amd import init
+        from oracle import oracle as O
+        O.set_threads(1)
+        g = capi.make_grid((256, 256), bc=capi.PERIODIC)
+        pp = capi.CHParams(1.0, 0.0, 1.0, 0.5, 1.0, 0.01)
+        cc = init.to_stored(S.config5_initial(ics[0]), g)
+        w = np.zeros((6,) + cc.shape)
+        steps = 2000
+        t0 = time.perf_counter()
+        O.ch_rk4_steps(g, pp, cc, steps, w)
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(dt * 20000 / steps, 2), "unit": "s to t_end (plain solver, 20,000 RK4 steps)",
+               "cores": 1, "kind": "port",
+               "sample": f"{steps} RK4 steps of the same 256^2 problem, FP64 oracle (orc_ch_rk4_steps), "
+                         f"{dt:.1f} s on {cpu_model()}, scaled to 20,000 steps"}
+    pms = float(np.median([r["parareal_ms"] for r in ev["per_ic"]]))
+    cfg = {"workload": "surrogate_ch256_tend200_periodic", "grid": [256, 256], "bc": "periodic", "kappa": 0.5, "M": 1.0,
+           "dt": 0.01, "t_end_steps": 20000, "jump_steps": 100, "lead_steps": ev["lead_steps"], "slices": ev["slices"],
+           "slice_steps": ev["slice_steps"], "iterations": 1, "weights": S.WEIGHTS_LATE.name, "ics": list(ics),
+           "plain_solver": "fused FP32 RK4, autotuned 2D launch", "parallelism": "single-gpu"}
+    out = line("speedup in time to t_end of the surrogate-plus-solver form over the plain GPU solver (256^2 CH, "
+               "median over initial conditions; relative L2 vs FP64 per IC in 'surrogate')",
+               ev["speedup_median"], "x", 1, 1, pms, "bf16+f32",
+               "synthetic (c0 = 0.5 + U(-0.05, 0.05), SplitMix64 substream (1234, ic))", cfg, roof, cpu)
+    out["value"] = round(ev["speedup_median"], 3)
+    out["surrogate"] = ev
+    return out
+
+
 def bench_copy(a, with_cpu):
     """Plain device copy of 2 GiB (the STREAM-copy bandwidth of this box, and the
     known-size kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters)."""
@@ -565,7 +632,7 @@ def bench_copy(a, with_cpu):
 
 
 WORKLOADS = {"copycal": bench_copy, "ch3d": bench_ch3d, "ch2d1024": bench_ch2d, "ac1024": bench_ac, "ens512": bench_ens,
-             "euler128": bench_euler, "gfa600": bench_gfa}
+             "euler128": bench_euler, "gfa600": bench_gfa, "surrogate": bench_surrogate}
 
 
 def main(argv=None) -> int:
@@ -584,6 +651,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile", type=int, default=None, help="CH RK4 launch tile index (after any autotuning)")
     ap.add_argument("--zc", type=int, default=None, help="3D stage z-chunk (after any autotuning)")
+    ap.add_argument("--ics", type=int, default=10, help="surrogate: initial conditions evaluated (up to 10)")
     a = ap.parse_args(argv)
     a.reps = max(1, a.reps)
     if a.gpus != 1:

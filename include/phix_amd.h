@@ -396,6 +396,13 @@ size_t phx_sur_workspace_bytes(int B, int H, int W);
 #define PHX_SUR_FUSED 2
 int    phx_sur_forward(const float* params, void* ws, const float* c_in, float* c_out,
                        int B, int H, int W, int flags, phx_stream stream);
+/* jumps >= 1 inference jumps c_in -> c_out (no reference counterpart): the
+ * bits of `jumps` phx_sur_forward calls with PHX_SUR_FUSED, with each
+ * jump's mean projection formed by the next jump's launch as it loads its
+ * input (one launch per jump, plus one projection at the end). flags:
+ * PHX_SUR_REUSE_WEIGHTS as above. c_out may be c_in. */
+int    phx_sur_rollout(const float* params, void* ws, const float* c_in, float* c_out,
+                       int B, int H, int W, int jumps, int flags, phx_stream stream);
 /* Loss = mean((c_out - c_target)^2); grad (param layout) <- dLoss/dparams;
  * loss_dev (device double, may be NULL) <- sum of squared errors (divide by
  * B*H*W for the loss). Needs the preceding phx_sur_forward on the same ws. */

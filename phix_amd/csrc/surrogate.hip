@@ -554,7 +554,22 @@ __device__ __forceinline__ void comp_add(double& a, double& ca, double b, double
     ca = ca + cb + e;
 }
 
-__global__ __launch_bounds__(FNT) void k_sur_fused_fwd(const float* __restrict__ c, const float* __restrict__ P,
+// One step of the tile tree of k_sur_project_tiles: (a, ca) <- (a, ca) (+) (b, cb).
+__device__ __forceinline__ void tree_add(double& a, double& ca, double b, double cb) {
+    const double t = a + b;
+    const double e = fabs(a) >= fabs(b) ? (a - t) + b : (b - t) + a;
+    a = t;
+    ca += cb + e;
+}
+
+// With dprev set, the input is the previous jump's unprojected state: the
+// kernel first forms that jump's projection u = c + dprev - mean(dprev) at
+// every point it loads, the mean from the previous launch's tile partials
+// tprev with k_sur_project_tiles's operations in its order (so u is that
+// kernel's bits), and writes u on its own tile to cproj.
+__global__ __launch_bounds__(FNT) void k_sur_fused_fwd(const float* __restrict__ c, const float* __restrict__ dprev,
+                                                       const double* __restrict__ tprev, int ntile,
+                                                       float* __restrict__ cproj, const float* __restrict__ P,
                                                        const __bf16* __restrict__ wpk,
                                                        const __bf16* __restrict__ frag, float* __restrict__ d,
                                                        double* __restrict__ tpart, int H, int W) {
@@ -565,17 +580,61 @@ __global__ __launch_bounds__(FNT) void k_sur_fused_fwd(const float* __restrict__
     __shared__ __attribute__((aligned(16))) __bf16 W6S[FRAG6];
     __shared__ float BM[4 * CH];
     __shared__ double RW[2 * (FNT / 64)];
+    __shared__ double MEAN;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, g = lane >> 4, lx = lane & 15;
     const int x0 = blockIdx.x * FT, y0 = blockIdx.y * FT;
     const long long img = (long long)blockIdx.z * H * W;
     // ---- prologue: global loads, then LDS stores
-    float xv[2];
+    float cin[2], dv0[2];
+    long long at[2];
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const int t = min(tid + k * FNT, NX0 - 1);
         const int yy = wrap(y0 - 6 + t / FR0, H), xx = wrap(x0 - 6 + t % FR0, W);
-        xv[k] = c[img + (long long)yy * W + xx] - 0.5f;
+        at[k] = img + (long long)yy * W + xx;
+        cin[k] = c[at[k]];
+        dv0[k] = dprev ? dprev[at[k]] : 0.f;
     }
+    if (dprev) {
+        // the previous jump's mean of d: wave 0 holds k_sur_project_tiles's
+        // 256 per-thread sums in lanes (t, t + 64, t + 128, t + 192) and runs
+        // its tree (pairs h = 128, 64 in registers, then 32..1 across lanes)
+        if (wv == 0) {
+            const double* tp = tprev + 2LL * blockIdx.z * ntile;
+            double s4[4], c4[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                double sq = 0.0, cq = 0.0;
+                for (int k = lane + 64 * q; k < ntile; k += 256) {
+                    neu_add(sq, cq, tp[2 * k]);
+                    cq += tp[2 * k + 1];
+                }
+                s4[q] = sq;
+                c4[q] = cq;
+            }
+            tree_add(s4[0], c4[0], s4[2], c4[2]);
+            tree_add(s4[1], c4[1], s4[3], c4[3]);
+            tree_add(s4[0], c4[0], s4[1], c4[1]);
+            double sv = s4[0], cc = c4[0];
+#pragma unroll
+            for (int h = 32; h > 0; h >>= 1) {
+                const double so = __shfl_down(sv, h, 64), co = __shfl_down(cc, h, 64);
+                tree_add(sv, cc, so, co);
+            }
+            if (lane == 0) MEAN = (sv + cc) / ((double)H * W);
+        }
+        __syncthreads();
+        const double mean = MEAN;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            cin[k] = (float)((double)cin[k] + ((double)dv0[k] - mean));
+            const int t = tid + k * FNT, ty = t / FR0 - 6, tx = t % FR0 - 6;
+            if (t < NX0 && ty >= 0 && ty < FT && tx >= 0 && tx < FT) cproj[at[k]] = cin[k];
+        }
+    }
+    float xv[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) xv[k] = cin[k] - 0.5f;
     const uint4 w6v = tid < FRAG6 / 8 ? reinterpret_cast<const uint4*>(frag)[tid] : uint4{0, 0, 0, 0};
     const float bmv = tid < 4 * CH ? P[P_MID + (tid / CH) * (MID_W + CH) + MID_W + tid % CH] : 0.f;
     const float w1v = tid < CH * 10 ? P[P_W1 + tid] : 0.f;
@@ -792,6 +851,9 @@ struct SurWS {
     double* loss;     // 1
     double* spart;    // SUM_BLOCKS * 2 * B partial sums
     double* tpart;    // fused forward: 2 per 16x16 tile
+    float* d2;        // rollout: the second d buffer
+    double* tpart2;   // rollout: the second tile-partial buffer
+    float* u[2];      // rollout: projected states between jumps
     float* part;      // WG_BLOCKS * PART
     __bf16* wfwd;     // 4 * MID_W
     __bf16* wbwd;     // 4 * MID_W
@@ -818,6 +880,9 @@ static size_t ws_layout(int B, int H, int W, SurWS* w, char* base) {
     s.loss = (double*)take(16);
     s.spart = (double*)take((size_t)SUM_BLOCKS * 2 * B * 8);
     s.tpart = (double*)take((size_t)2 * B * (H / 16) * (W / 16) * 8);
+    s.d2 = (float*)take(npix * 4);
+    s.tpart2 = (double*)take((size_t)2 * B * (H / 16) * (W / 16) * 8);
+    for (int k = 0; k < 2; ++k) s.u[k] = (float*)take(npix * 4);
     s.part = (float*)take((size_t)WG_BLOCKS * PART * 4);
     s.wfwd = (__bf16*)take(4 * MID_W * 2);
     s.wbwd = (__bf16*)take(4 * MID_W * 2);
@@ -844,8 +909,8 @@ static int sur_forward(const float* P, SurWS& S, const float* c, float* out, int
     }
     if (flags & PHX_SUR_FUSED) {
         const int ntile = (H / FT) * (W / FT);
-        hipLaunchKernelGGL(k_sur_fused_fwd, dim3(W / FT, H / FT, B), dim3(FNT), 0, st, c, P, S.wfwd, S.frag, S.d,
-                           S.tpart, H, W);
+        hipLaunchKernelGGL(k_sur_fused_fwd, dim3(W / FT, H / FT, B), dim3(FNT), 0, st, c, (const float*)nullptr,
+                           (const double*)nullptr, 0, (float*)nullptr, P, S.wfwd, S.frag, S.d, S.tpart, H, W);
         hipLaunchKernelGGL(k_sur_project_tiles, dim3(nb), dim3(256), 0, st, c, S.d, S.tpart, ntile, H * W, n, out);
         PHX_LAUNCH_CHECK("phx_sur_forward");
         return PHX_OK;
@@ -859,6 +924,35 @@ static int sur_forward(const float* P, SurWS& S, const float* c, float* out, int
     seg_sums(S.d, (long long)H * W, B, S.spart, S.sums, st);
     hipLaunchKernelGGL(k_project, dim3(nb), dim3(256), 0, st, c, S.d, S.sums, H * W, n, out);
     PHX_LAUNCH_CHECK("phx_sur_forward");
+    return PHX_OK;
+}
+
+// `jumps` fused forwards from c, each projection formed by the next launch
+// as it loads its input (k_sur_fused_fwd with dprev), the last by
+// k_sur_project_tiles: the same bits as `jumps` calls of sur_forward with
+// PHX_SUR_FUSED, with one launch per jump instead of two.
+static int sur_rollout(const float* P, SurWS& S, const float* c, float* out, int B, int H, int W, int jumps,
+                       int flags, hipStream_t st) {
+    const long long n = (long long)B * H * W;
+    const int nb = (int)((n + 255) / 256), ntile = (H / FT) * (W / FT);
+    if (!(flags & PHX_SUR_REUSE_WEIGHTS)) {
+        hipLaunchKernelGGL(k_pack, dim3((4 * MID_W + 255) / 256), dim3(256), 0, st, P, S.wfwd, S.wbwd);
+        hipLaunchKernelGGL(k_pack_frag, dim3((FRAG_N + 255) / 256), dim3(256), 0, st, P, S.frag);
+    }
+    float* dbuf[2] = {S.d, S.d2};
+    double* tbuf[2] = {S.tpart, S.tpart2};
+    const float* base = c;   // the unprojected state the next launch reads
+    for (int j = 0; j < jumps; ++j) {
+        const float* dp = j ? dbuf[(j - 1) & 1] : nullptr;
+        const double* tp = j ? tbuf[(j - 1) & 1] : nullptr;
+        float* up = j ? S.u[j & 1] : nullptr;
+        hipLaunchKernelGGL(k_sur_fused_fwd, dim3(W / FT, H / FT, B), dim3(FNT), 0, st, base, dp, tp, ntile, up, P,
+                           S.wfwd, S.frag, dbuf[j & 1], tbuf[j & 1], H, W);
+        if (j) base = up;
+    }
+    hipLaunchKernelGGL(k_sur_project_tiles, dim3(nb), dim3(256), 0, st, base, dbuf[(jumps - 1) & 1],
+                       tbuf[(jumps - 1) & 1], ntile, H * W, n, out);
+    PHX_LAUNCH_CHECK("phx_sur_rollout");
     return PHX_OK;
 }
 
@@ -883,6 +977,17 @@ int phx_sur_forward(const float* params, void* ws, const float* c_in, float* c_o
     SurWS S;
     ws_layout(B, H, W, &S, (char*)ws);
     return sur_forward(params, S, c_in, c_out, B, H, W, flags, as_stream(s));
+}
+
+int phx_sur_rollout(const float* params, void* ws, const float* c_in, float* c_out, int B, int H, int W, int jumps,
+                    int flags, phx_stream s) {
+    int rc = check_shape(B, H, W, "phx_sur_rollout");
+    if (rc) return rc;
+    if (!params || !ws || !c_in || !c_out) return fail(PHX_ERR_INVALID, "phx_sur_rollout: NULL argument");
+    if (jumps < 1) return fail(PHX_ERR_INVALID, "phx_sur_rollout: jumps < 1");
+    SurWS S;
+    ws_layout(B, H, W, &S, (char*)ws);
+    return sur_rollout(params, S, c_in, c_out, B, H, W, jumps, flags, as_stream(s));
 }
 
 int phx_sur_backward(const float* params, void* ws, const float* c_in, const float* c_out, const float* c_target,

@@ -168,12 +168,23 @@ class Surrogate:
         return out
 
     def rollout(self, c, jumps, bufs, fused=True):
-        """`jumps` forwards from c, alternating between the two buffers of
-        `bufs`; returns the buffer holding the result."""
+        """`jumps` forwards from c; returns the buffer of `bufs` holding the
+        result (bufs[(jumps - 1) % 2]). fused: one phx_sur_rollout call (one
+        launch per jump, each jump's mean projection formed by the next
+        launch; the bits of `jumps` fused forwards); else layer by layer,
+        alternating between the two buffers."""
+        if jumps < 1:
+            return c
+        if fused:
+            out = bufs[(jumps - 1) % 2]
+            assert c.shape == out.shape == (self.B, self.n, self.n) and c.is_contiguous() and out.is_contiguous()
+            capi.call("phx_sur_rollout", capi.ptr(self.p), capi.ptr(self.ws), capi.ptr(c), capi.ptr(out), self.B,
+                      self.n, self.n, int(jumps), 1, capi.stream_handle())
+            return out
         cur = c
         for j in range(jumps):
             nxt = bufs[j % 2]
-            self.forward_to(cur, nxt, fused=fused)
+            self.forward_to(cur, nxt, fused=False)
             cur = nxt
         return cur
 
@@ -428,47 +439,52 @@ class PararealSolver:
         assert slice_steps % every == 0
         self.dev, self.n, self.dt, self.every, self.kappa = dev, n, dt, every, kappa
         self.S, self.steps = slices, slice_steps
+        self.jumps = slice_steps // every
         self.net = Surrogate(dev, 1, n=n, params=params)
+        self.net.forward(torch.full((1, n, n), 0.5, device=dev))   # packs the BF16 weights once
         # the fine propagator: the plain solver's own RK4 launch, all slices
         # at once (phx_ch_rk4_steps_batch: one stored 2D field per slice)
         self.judge = PlainSolver(dev, n=n, kappa=kappa, dt=dt)
         self.arena = torch.zeros((slices,) + tuple(self.judge.shape), dtype=torch.float32, device=dev)
         self.work = torch.empty(slices * self.judge.work.numel(), dtype=torch.float32, device=dev)
-        self.h = None
-        self.bufs = [torch.empty((1, n, n), dtype=torch.float32, device=dev) for _ in range(2)]
-        self.graph = None
+        # slice-boundary states: U (current iterate), G (the surrogate's values
+        # G(U[s]) of the last sweep), D = F(U[s]) - G(U[s])
+        self.U = torch.zeros((slices + 1, 1, n, n), dtype=torch.float32, device=dev)
+        self.G = torch.zeros((slices, 1, n, n), dtype=torch.float32, device=dev)
+        self.D = torch.zeros((slices, 1, n, n), dtype=torch.float32, device=dev)
+        self.Fs = torch.zeros((slices, 1, n, n), dtype=torch.float32, device=dev)   # F(U[s]) of the last sweep
+        self.eslots = torch.empty(2 * slices, dtype=torch.float64, device=dev)
         self.lead = None
 
     def close(self):
         pass
 
-    def G(self, u):
-        if self.graph is None:
-            self.graph = RolloutGraph(self.net, self.steps // self.every)
-        return self.graph(u).clone()
+    def _G(self, u, out):
+        """out <- G(u): the slice's jumps in one phx_sur_rollout call."""
+        net = self.net
+        capi.call("phx_sur_rollout", capi.ptr(net.p), capi.ptr(net.ws), capi.ptr(u), capi.ptr(out), 1, self.n,
+                  self.n, self.jumps, 1, capi.stream_handle())
 
-    def F_all(self, U):
-        import torch
+    def _F_all(self):
+        """F applied to U[0..S-1] at once; returns the (S, n, n) view of the
+        solver arena's physical cells holding the results."""
         n, J = self.n, self.judge
         phys = self.arena[:, 1, 1:n + 1, 1:n + 1]
-        phys.copy_(torch.cat(U[: self.S]))
+        phys.copy_(self.U[: self.S, 0])
         capi.call("phx_ch_rk4_steps_batch", C.byref(J.g), J.dtype, self.S, capi.ptr(self.arena), capi.ptr(self.work),
                   C.byref(J.p), self.steps, capi.stream_handle())
-        return phys.clone()
+        return phys
 
-    def energies(self, U):
-        """Free energies of the states U, each reduced on the device into its
-        own slot; one readback for all of them."""
-        import torch
-        if getattr(self, "_eslots", None) is None or self._eslots.numel() < 2 * len(U):
-            self._eslots = torch.empty(2 * len(U), dtype=torch.float64, device=self.dev)
-        J = self.judge
-        base = capi.ptr(self._eslots)
-        for k, u in enumerate(U):
-            J.set(u[0])
-            capi.call("phx_mass_energy", C.byref(J.g), J.dtype, capi.ptr(J.c), C.byref(J.p), capi.ptr(J.red),
-                      C.c_void_p(base + 16 * k), capi.stream_handle())
-        return self._eslots[: 2 * len(U)].view(len(U), 2)[:, 1].tolist()
+    def _energies(self, states):
+        """Free energies of the (S, n, n) states (phx_mass_energy per state on
+        the solver arena, each into its own device slot; one readback)."""
+        n, J = self.n, self.judge
+        self.arena[:, 1, 1:n + 1, 1:n + 1].copy_(states)
+        base, stride = capi.ptr(self.arena), self.arena[0].numel() * 4
+        for k in range(self.S):
+            capi.call("phx_mass_energy", C.byref(J.g), J.dtype, C.c_void_p(base + k * stride), C.byref(J.p),
+                      capi.ptr(J.red), C.c_void_p(capi.ptr(self.eslots) + 16 * k), capi.stream_handle())
+        return self.eslots.view(self.S, 2)[:, 1].tolist()
 
     def run(self, c0, iters, lead_steps=0):
         """Advances c0 (n x n FP32 device tensor) by lead_steps plain solver
@@ -480,39 +496,38 @@ class PararealSolver:
         import torch
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        n = self.n
+        n, S, U, G, D = self.n, self.S, self.U, self.G, self.D
         if lead_steps:
             if self.lead is None:
                 self.lead = PlainSolver(self.dev, n=n, kappa=self.kappa, dt=self.dt)
                 self.lead.autotune()   # the launch is kept per shape: F uses it too
             self.lead.set(c0.reshape(n, n))
             self.lead.steps(lead_steps)
-            c0 = self.lead.phys()
-        U = [c0.reshape(1, n, n).contiguous().clone()]
-        Gp = []
-        for s in range(self.S):
-            g = self.G(U[s])
-            Gp.append(g)
-            U.append(g)
+            U[0, 0].copy_(self.lead.phys())
+        else:
+            U[0, 0].copy_(c0.reshape(n, n))
+        # prediction: U[s+1] = G(U[s]), so G(U[s]) is U[s+1]
+        for s in range(S):
+            self._G(U[s], U[s + 1])
+        G.copy_(U[1:])
         incs = []
         ok = True
         for _ in range(iters):
-            Fk = self.F_all(U)
-            newU = [U[0]]
-            for s in range(self.S):
-                g = self.G(newU[s])
-                newU.append(g + Fk[s:s + 1] - Gp[s])
-                Gp[s] = g
-            incs.append(((newU[-1] - U[-1]).norm() / newU[-1].norm()).item())
-            es = self.energies(newU[1:])
+            self.Fs[:, 0].copy_(self._F_all())
+            torch.sub(self.Fs, G, out=D)
+            last = U[S].clone()
+            for s in range(S):
+                self._G(U[s], G[s])
+                torch.add(G[s], D[s], out=U[s + 1])
+            incs.append(((U[S] - last).norm() / U[S].norm()).item())
+            es = self._energies(U[1:, 0])
             if any(b > a * (1 + 1e-6) for a, b in zip(es, es[1:])):
                 ok = False   # fallback: the solver's states of this iteration
-                newU = [U[0]] + [Fk[s:s + 1].clone() for s in range(self.S)]
-            U = newU
+                U[1:].copy_(self.Fs)
         torch.cuda.synchronize()
-        jumps = self.S * (self.steps // self.every) * (1 + iters)
-        return U[-1][0], PararealReport(time.perf_counter() - t0, iters, self.S, jumps,
-                                        lead_steps + iters * self.S * self.steps, incs, ok)
+        jumps = S * self.jumps * (1 + iters)
+        return U[S, 0].clone(), PararealReport(time.perf_counter() - t0, iters, S, jumps,
+                                               lead_steps + iters * S * self.steps, incs, ok)
 
 
 # ------------------------------------------------------------------ CLI -----
@@ -544,6 +559,62 @@ def fp64_reference(dev, c0_np, nsteps, n=256, kappa=0.5, dt=0.01):
     S.steps(nsteps)
     torch.cuda.synchronize()
     return S.phys().cpu().numpy()
+
+
+# The parareal form measured against the plain solver: the solver runs the
+# first LEAD_STEPS steps, then SLICES slices of (20000 - LEAD_STEPS) / SLICES
+# steps, one iteration, coarse propagator = the late-phase weights.
+LEAD_STEPS, SLICES = 1000, 38
+EVAL_ICS = tuple(range(7, 17))   # initial conditions: SplitMix64 substream (1234, ic)
+
+
+def config5_initial(ic, n=256):
+    """c0 = 0.5 + U(-0.05, 0.05) of initial condition `ic` (numpy n x n)."""
+    from . import init as _init
+    return _init.uniform(_init.substream(1234, ic), n * n, 0.45, 0.55).reshape(n, n)
+
+
+def evaluate_parareal(dev, params, ics=EVAL_ICS, lead=LEAD_STEPS, slices=SLICES, tend_steps=20000, reps=3):
+    """Time-to-solution of the plain FP32 GPU solver and of the lead-phase
+    parareal form over the initial conditions `ics` (each timed `reps` times,
+    the fastest kept), each final field's relative L2 against the FP64 GPU
+    solver. Returns a dict of per-IC records and summary figures."""
+    import torch
+    n = 256
+    plain = PlainSolver(dev, n=n)
+    P = PararealSolver(dev, params, slices, (tend_steps - lead) // slices, n=n)
+    x = torch.from_numpy(config5_initial(ics[0])).float().to(dev)
+    plain.set(x)
+    plain.autotune()
+    P.run(x, 1, lead)   # warm-up: lead-solver autotune, weight packing
+    per = []
+    for ic in ics:
+        c0 = config5_initial(ic)
+        ref = fp64_reference(dev, c0, tend_steps, n=n)
+        x = torch.from_numpy(c0).float().to(dev)
+        tp = []
+        for _ in range(reps):
+            plain.set(x)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            plain.steps(tend_steps)
+            torch.cuda.synchronize()
+            tp.append(time.perf_counter() - t0)
+        perr = float(np.linalg.norm(plain.phys().cpu().numpy() - ref) / np.linalg.norm(ref))
+        tq, out, rep = [], None, None
+        for _ in range(reps):
+            out, rep = P.run(x, 1, lead)
+            tq.append(rep.seconds)
+        err = float(np.linalg.norm(out.cpu().numpy() - ref) / np.linalg.norm(ref))
+        per.append({"ic": ic, "plain_ms": 1e3 * min(tp), "plain_rel_l2": perr, "parareal_ms": 1e3 * min(tq),
+                    "rel_l2": err, "speedup": min(tp) / min(tq), "energy_ok": rep.energy_ok,
+                    "increment": rep.increments[-1]})
+    sp = [r["speedup"] for r in per]
+    return {"lead_steps": lead, "slices": slices, "slice_steps": (tend_steps - lead) // slices, "iterations": 1,
+            "per_ic": per, "speedup_median": float(np.median(sp)), "speedup_min": float(min(sp)),
+            "rel_l2_median": float(np.median([r["rel_l2"] for r in per])),
+            "rel_l2_max": float(max(r["rel_l2"] for r in per)),
+            "meets_1e-2": sum(1 for r in per if r["rel_l2"] <= 1e-2)}
 
 
 def main(argv=None):
