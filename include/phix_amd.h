@@ -313,6 +313,14 @@ size_t phx_reduce_workspace_bytes(const phx_grid* grid);
 int phx_mass_energy(const phx_grid* grid, int dtype, const void* c,
                     const phx_ch_params* p, void* work, double* out_dev,
                     phx_stream stream);
+/* phx_mass_energy of nbatch same-shape stored fields laid back to back in c
+ * (no reference counterpart; the parareal judge's energies of all slice
+ * boundaries in one launch pair): out_dev holds 2*nbatch doubles, field b's
+ * pair at 2b, each the bits of phx_mass_energy on that field; work is
+ * nbatch * phx_reduce_workspace_bytes(grid) bytes. */
+int phx_mass_energy_batch(const phx_grid* grid, int dtype, int nbatch, const void* c,
+                          const phx_ch_params* p, void* work, double* out_dev,
+                          phx_stream stream);
 
 /* ------------------------------------- fused multi-field Allen-Cahn ---- */
 size_t phx_ac_rk4_workspace_bytes(const phx_grid* grid, int dtype, int nfields);

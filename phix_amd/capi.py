@@ -144,6 +144,7 @@ SIG = {
     "phx_ch_timing_read": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "phx_reduce_workspace_bytes": (C.c_size_t, [GP]),
     "phx_mass_energy": (C.c_int, [GP, C.c_int, VP, C.POINTER(CHParams), VP, VP, VP]),
+    "phx_mass_energy_batch": (C.c_int, [GP, C.c_int, C.c_int, VP, C.POINTER(CHParams), VP, VP, VP]),
     "phx_ac_rk4_workspace_bytes": (C.c_size_t, [GP, C.c_int, C.c_int]),
     "phx_ac_rk4_steps": (C.c_int, [GP, C.c_int, C.c_int, C.POINTER(VP), VP, C.POINTER(ACParams), C.c_int, VP]),
     "phx_ac_set_two_stage": (C.c_int, [C.c_int]),

@@ -476,14 +476,17 @@ class PararealSolver:
         return phys
 
     def _energies(self, states):
-        """Free energies of the (S, n, n) states (phx_mass_energy per state on
-        the solver arena, each into its own device slot; one readback)."""
+        """Free energies of the (S, n, n) states: one phx_mass_energy_batch
+        over the solver arena (FP64 compensated, each state's the bits of
+        phx_mass_energy on it); one readback."""
+        import torch
         n, J = self.n, self.judge
         self.arena[:, 1, 1:n + 1, 1:n + 1].copy_(states)
-        base, stride = capi.ptr(self.arena), self.arena[0].numel() * 4
-        for k in range(self.S):
-            capi.call("phx_mass_energy", C.byref(J.g), J.dtype, C.c_void_p(base + k * stride), C.byref(J.p),
-                      capi.ptr(J.red), C.c_void_p(capi.ptr(self.eslots) + 16 * k), capi.stream_handle())
+        if getattr(self, "_ered", None) is None:
+            self._ered = torch.empty(self.S * capi.lib().phx_reduce_workspace_bytes(C.byref(J.g)) // 8,
+                                     dtype=torch.float64, device=self.dev)
+        capi.call("phx_mass_energy_batch", C.byref(J.g), J.dtype, self.S, capi.ptr(self.arena), C.byref(J.p),
+                  capi.ptr(self._ered), capi.ptr(self.eslots), capi.stream_handle())
         return self.eslots.view(self.S, 2)[:, 1].tolist()
 
     def run(self, c0, iters, lead_steps=0):

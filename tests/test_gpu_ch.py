@@ -228,3 +228,29 @@ def test_euler_one_launch_per_step_equals_two(dev, dtype, n, bc, vals, nsteps):
 def torch_zeros_like(t):
     import torch
     return torch.zeros_like(t)
+
+
+@pytest.mark.parametrize("n,bc,dtype", [((256, 256), capi.PERIODIC, capi.F32), ((37, 50), capi.NOFLUX, capi.F64),
+                                        ((20, 12, 9), capi.PERIODIC, capi.F64)])
+def test_mass_energy_batch_equals_single(dev, n, bc, dtype):
+    """phx_mass_energy_batch over fields stored back to back returns each
+    field's phx_mass_energy bits."""
+    import torch
+    nb = 5
+    g = capi.make_grid(n, bc=bc)
+    p = ch_params()
+    tdt = torch.float64 if dtype == capi.F64 else torch.float32
+    shape = capi.stored_shape(g)
+    rng = np.random.default_rng(len(n) * 100 + n[0])
+    c = torch.from_numpy(rng.uniform(0.2, 0.8, (nb,) + tuple(shape))).to(tdt).to(dev)
+    wb = capi.lib().phx_reduce_workspace_bytes(C.byref(g))
+    red = torch.empty(nb * wb // 8, dtype=torch.float64, device=dev)
+    outb = torch.empty(2 * nb, dtype=torch.float64, device=dev)
+    capi.call("phx_mass_energy_batch", C.byref(g), dtype, nb, capi.ptr(c), C.byref(p), capi.ptr(red),
+              capi.ptr(outb), stream())
+    outs = torch.empty(2 * nb, dtype=torch.float64, device=dev)
+    for b in range(nb):
+        capi.call("phx_mass_energy", C.byref(g), dtype, capi.ptr(c[b]), C.byref(p), capi.ptr(red),
+                  capi.ptr(outs[2 * b:]), stream())
+    torch.cuda.synchronize()
+    assert torch.equal(outb, outs)
