@@ -213,6 +213,13 @@ int phx_ch_rk4_steps(const phx_grid* grid, int dtype, void* c, void* work,
  * counterpart (the reference advances one field per solver). */
 int phx_ch_rk4_steps_batch(const phx_grid* grid, int dtype, int nbatch, void* c, void* work,
                            const phx_ch_params* p, int nsteps, phx_stream stream);
+/* Times the whole-step tiles on `steps` steps of the caller's batch (it is
+ * advanced: pass a scratch copy) and keeps the fastest for
+ * phx_ch_rk4_steps_batch with this shape, dtype and nbatch (no reference
+ * counterpart); every tile returns the same bits. */
+int phx_ch_rk4_batch_autotune(const phx_grid* grid, int dtype, int nbatch, void* c, void* work,
+                              const phx_ch_params* p, int steps, double* best_ms_per_step,
+                              phx_stream stream);
 
 /* No reference counterpart: for callers that cannot allocate device memory
  * (code under HIP graph capture, tight serving loops, harnesses that forbid

@@ -129,6 +129,8 @@ SIG = {
     "phx_gfa_euler_steps": (C.c_int, [GP, VP, VP, VP, VP, C.POINTER(GfaParams), C.c_int, VP]),
     "phx_ch_rk4_steps": (C.c_int, [GP, C.c_int, VP, VP, C.POINTER(CHParams), C.c_int, VP]),
     "phx_ch_rk4_steps_batch": (C.c_int, [GP, C.c_int, C.c_int, VP, VP, C.POINTER(CHParams), C.c_int, VP]),
+    "phx_ch_rk4_batch_autotune": (C.c_int, [GP, C.c_int, C.c_int, VP, VP, C.POINTER(CHParams), C.c_int,
+                                            C.POINTER(C.c_double), VP]),
     "phx_ch_rk4_steps_noalloc": (C.c_int, [GP, C.c_int, VP, C.POINTER(CHParams), C.c_int, VP]),
     "phx_ch_rk4_set_zchunk": (C.c_int, [GP, C.c_int, C.c_int]),
     "phx_ch_set_xcd_remap": (C.c_int, [C.c_int]),

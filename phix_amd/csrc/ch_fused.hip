@@ -2682,6 +2682,18 @@ extern "C" int phx_ch_rk4_steps_noalloc(const phx_grid* G, int dtype, void* c, c
 
 // nbatch independent fields of one 2D grid, stored back to back, advanced
 // together: the whole-step schedule with the batch on the launch's z axis.
+// Whole-step tile of phx_ch_rk4_steps_batch per (shape, dtype, nbatch), set
+// by phx_ch_rk4_batch_autotune: a batch fills the chip, so the tile that is
+// fastest for it can differ from the single field's (which is latency bound
+// on small grids).
+static std::map<std::tuple<int, int, int, int, int, int>, int> g_batch_tile;
+
+static int batch_tile(const phx_grid* G, int dtype, int nbatch) {
+    std::lock_guard<std::mutex> lk(g_zc_mu);
+    auto it = g_batch_tile.find(std::make_tuple(G->dim, G->n[0], G->n[1], G->n[2], dtype, nbatch));
+    return it == g_batch_tile.end() ? -1 : it->second;
+}
+
 int phx_ch_rk4_steps_batch(const phx_grid* G, int dtype, int nbatch, void* c, void* work, const phx_ch_params* p,
                            int nsteps, phx_stream s) {
     int rc = check_grid_fused(G, "phx_ch_rk4_steps_batch");
@@ -2694,13 +2706,69 @@ int phx_ch_rk4_steps_batch(const phx_grid* G, int dtype, int nbatch, void* c, vo
     if (!step2d_ok(G))
         return fail(PHX_ERR_UNSUPPORTED, "phx_ch_rk4_steps_batch: 2D grids with n >= 8 on both axes only");
     if (nsteps == 0) return PHX_OK;
-    Launch3 l = launch_for(G, dtype);
-    const int stile = l.tile >= 1 ? l.tile - 1 : step2d_tile(4, dtype == PHX_F64 ? 8 : 4);
+    // the batch's own launch when phx_ch_rk4_batch_autotune chose one, else
+    // the single-field launch of this shape
+    int stile = batch_tile(G, dtype, nbatch);
+    if (stile < 0) {
+        Launch3 l = launch_for(G, dtype);
+        stile = l.tile >= 1 ? l.tile - 1 : step2d_tile(4, dtype == PHX_F64 ? 8 : 4);
+    }
     hipStream_t st = as_stream(s);
     if (dtype == PHX_F32)
         return rk4_step2d_steps_t<float, false>(G, (float*)c, (float*)work, p, nsteps, st, stile, 0, nbatch);
     return g_fp64_exact ? rk4_step2d_steps_t<double, true>(G, (double*)c, (double*)work, p, nsteps, st, stile, 0, nbatch)
                         : rk4_step2d_steps_t<double, false>(G, (double*)c, (double*)work, p, nsteps, st, stile, 0, nbatch);
+}
+
+// Times the whole-step tiles on `steps` steps of the caller's batch (advancing
+// it) and keeps the fastest for phx_ch_rk4_steps_batch with this shape,
+// dtype and nbatch.
+int phx_ch_rk4_batch_autotune(const phx_grid* G, int dtype, int nbatch, void* c, void* work, const phx_ch_params* p,
+                              int steps, double* best_ms_per_step, phx_stream s) {
+    int rc = check_grid_fused(G, "phx_ch_rk4_batch_autotune");
+    if (rc) return rc;
+    if ((rc = check_dtype(dtype, "phx_ch_rk4_batch_autotune"))) return rc;
+    if ((rc = check_ch_params(p, "phx_ch_rk4_batch_autotune"))) return rc;
+    if (!c || !work) return fail(PHX_ERR_INVALID, "phx_ch_rk4_batch_autotune: NULL buffer");
+    if (steps < 1) return fail(PHX_ERR_INVALID, "phx_ch_rk4_batch_autotune: steps < 1");
+    if (nbatch < 1 || nbatch > 65535)
+        return fail(PHX_ERR_INVALID, "phx_ch_rk4_batch_autotune: nbatch must be in [1, 65535]");
+    if (!step2d_ok(G))
+        return fail(PHX_ERR_UNSUPPORTED, "phx_ch_rk4_batch_autotune: 2D grids with n >= 8 on both axes only");
+    hipStream_t st = as_stream(s);
+    auto run = [&](int stile, int n) -> int {
+        if (dtype == PHX_F32) return rk4_step2d_steps_t<float, false>(G, (float*)c, (float*)work, p, n, st, stile, 0, nbatch);
+        return g_fp64_exact ? rk4_step2d_steps_t<double, true>(G, (double*)c, (double*)work, p, n, st, stile, 0, nbatch)
+                            : rk4_step2d_steps_t<double, false>(G, (double*)c, (double*)work, p, n, st, stile, 0, nbatch);
+    };
+    hipEvent_t e0, e1;
+    PHX_HIP(hipEventCreate(&e0));
+    PHX_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    int bt = 0;
+    for (int stile = 0; stile < kNumTilesS2; ++stile) {
+        if (step2d_tile(stile, dtype == PHX_F64 ? 8 : 4) != stile) continue;   // runs as another tile
+        if ((rc = run(stile, 2))) return rc;
+        PHX_HIP(hipEventRecord(e0, st));
+        if ((rc = run(stile, steps))) return rc;
+        PHX_HIP(hipEventRecord(e1, st));
+        PHX_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        PHX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ms /= steps;
+        if (ms < best) {
+            best = ms;
+            bt = stile;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    {
+        std::lock_guard<std::mutex> lk(g_zc_mu);
+        g_batch_tile[std::make_tuple(G->dim, G->n[0], G->n[1], G->n[2], dtype, nbatch)] = bt;
+    }
+    if (best_ms_per_step) *best_ms_per_step = best;
+    return PHX_OK;
 }
 
 // Times every launch candidate on `steps` steps of the caller's state

@@ -467,10 +467,18 @@ class PararealSolver:
 
     def _F_all(self):
         """F applied to U[0..S-1] at once; returns the (S, n, n) view of the
-        solver arena's physical cells holding the results."""
+        solver arena's physical cells holding the results. The first call
+        chooses the batch's whole-step launch (phx_ch_rk4_batch_autotune on a
+        scratch copy; every launch returns the same bits)."""
         n, J = self.n, self.judge
         phys = self.arena[:, 1, 1:n + 1, 1:n + 1]
         phys.copy_(self.U[: self.S, 0])
+        if not getattr(self, "_tuned", False):
+            scratch = self.arena.clone()
+            ms = C.c_double()
+            capi.call("phx_ch_rk4_batch_autotune", C.byref(J.g), J.dtype, self.S, capi.ptr(scratch),
+                      capi.ptr(self.work), C.byref(J.p), 20, C.byref(ms), capi.stream_handle())
+            self._tuned = True
         capi.call("phx_ch_rk4_steps_batch", C.byref(J.g), J.dtype, self.S, capi.ptr(self.arena), capi.ptr(self.work),
                   C.byref(J.p), self.steps, capi.stream_handle())
         return phys

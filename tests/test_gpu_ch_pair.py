@@ -289,9 +289,11 @@ def test_step2d_timing_kind(dev):
     ((40, 33), [[F, N], [P, P]], [[0.45, 0], [0, 0]], 4),
     ((64, 48), N, 0.0, 1),
 ])
-def test_batch_equals_per_field(dev, dtype, n, bc, vals, nsteps):
+@pytest.mark.parametrize("tuned", [False, True])
+def test_batch_equals_per_field(dev, dtype, n, bc, vals, nsteps, tuned):
     """phx_ch_rk4_steps_batch on 5 fields leaves each one exactly as
-    phx_ch_rk4_steps on that field alone (stored ghosts included)."""
+    phx_ch_rk4_steps on that field alone (stored ghosts included), also
+    after phx_ch_rk4_batch_autotune chose the batch's own launch."""
     import torch
     g = capi.make_grid(n, bc=bc, bc_value=vals)
     p = ch_params(rho=1.0, kappa=0.5, dt=0.01)
@@ -301,6 +303,12 @@ def test_batch_equals_per_field(dev, dtype, n, bc, vals, nsteps):
     batch = torch.stack([to_dev(f, dev, tdt) for f in fields]).contiguous()
     wbytes = capi.lib().phx_ch_rk4_workspace_bytes(C.byref(g), dtype)
     work = torch.zeros(nb * wbytes // esz, dtype=tdt, device=dev)
+    if tuned:
+        scratch = batch.clone()
+        ms = C.c_double()
+        capi.call("phx_ch_rk4_batch_autotune", C.byref(g), dtype, nb, capi.ptr(scratch), capi.ptr(work), C.byref(p), 3,
+                  C.byref(ms), stream())
+        assert ms.value > 0
     capi.call("phx_ch_rk4_steps_batch", C.byref(g), dtype, nb, capi.ptr(batch), capi.ptr(work), C.byref(p), nsteps,
               stream())
     for k in range(nb):

@@ -436,3 +436,4 @@ def test_config5_surrogate_plus_solver_against_plain_solver(dev):
         assert r["rel_l2"] <= 6e-2
     assert ev["speedup_median"] >= 3.5
     assert ev["rel_l2_median"] <= 3e-2
+
