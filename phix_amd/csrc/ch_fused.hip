@@ -1480,57 +1480,70 @@ __global__ __launch_bounds__(256) void k_ch_pair_3d(PairArgs<T> A) {
 // level lives in one frame: position (x, y) of the block's tile plus its
 // 8-cell ring sits at index (y - y0 + 8) * P + (x - x0 + 8) in SC, SA, SB
 // and SACC alike (P = TX + 16). A level covers whole frame rows -- ring R
-// spans rows 8-R .. 8+TY+R-1, all P columns -- so a thread's positions are
-// f = F0 + tid + NT i and every LDS address is tid plus a constant: no index
-// arithmetic per position. The frame columns outside the ring (at most 16 of
-// P) are evaluated too: they read written, finite entries of the previous
-// level (its rows cover one more row on each side) and feed only other
-// out-of-ring entries, never a ring or tile cell. The last partial pass reads
-// a valid position and skips the store. Positions are handled in groups of
-// G whose LDS operands are read before any is used (independent chains for
-// the scheduler). GEN: the block needs the ghost rules of NOFLUX / FIXED
-// faces.
+// spans rows 8-R .. 8+TY+R-1, all P columns -- cut into runs of V = 16 /
+// sizeof(T) consecutive positions; run q of a level starts at F0 + V q and
+// thread tid takes runs tid + NT i. A run's operands are three 16-byte LDS
+// reads (the run, the rows above and below) and the two scalars left and
+// right of it; its results are one 16-byte store per array. The arithmetic
+// per position is the one-stage kernels' (mu_at, k_at, axpy), so the levels
+// return their bits. The frame columns outside the ring (at most 16 of P) are
+// evaluated too: they read written, finite entries of the previous level
+// (its rows cover one more row on each side, which also holds the scalars
+// beyond a level's first and last run) and feed only other out-of-ring
+// entries, never a ring or tile cell. The last partial pass reads a valid run
+// and skips the store. GEN: the block needs the ghost rules of NOFLUX /
+// FIXED faces (a code per position).
 template <typename T, int TX, int TY, int NT, bool EXACT, bool GEN>
 __device__ __forceinline__ void step2d_levels(const PairArgs<T>& A, const T* SC, T* SA, T* SB, T* SACC,
                                               const T* BCV, int x0, int y0, long long zb) {
     constexpr int P = TX + 16;
-    constexpr int G = sizeof(T) == 8 ? 3 : 4;
+    constexpr int V = 16 / (int)sizeof(T);
+    typedef T VT __attribute__((ext_vector_type(V)));
+    static_assert(P % V == 0, "frame rows hold whole runs");
     static_assert(NT % TX == 0, "tile rows must divide the workgroup");
     const int tid = threadIdx.x;
+    // the run at f and its neighbours: xp/xm along x, yp/ym along y, with the
+    // ghost swaps and the FIXED code of each position
+    auto operands = [&](const T* src, int f, T (&c)[V], T (&xp)[V], T (&xm)[V], T (&yp)[V], T (&ym)[V],
+                        int (&code)[V]) {
+        const VT cv = *reinterpret_cast<const VT*>(&src[f]);
+        const VT uv = *reinterpret_cast<const VT*>(&src[f + P]);
+        const VT dv = *reinterpret_cast<const VT*>(&src[f - P]);
+        const T l = src[f - 1], r = src[f + V];
+#pragma unroll
+        for (int q = 0; q < V; ++q) {
+            c[q] = cv[q];
+            xp[q] = q + 1 < V ? cv[q + 1] : r;
+            xm[q] = q > 0 ? cv[q - 1] : l;
+            yp[q] = uv[q];
+            ym[q] = dv[q];
+            code[q] = GEN ? ghost_code_xy(x0 - 8 + (f + q) % P, y0 - 8 + (f + q) / P, A) : 0;
+            if (GEN) {
+                if (code[q] & GC_SWX) { const T t = xp[q]; xp[q] = xm[q]; xm[q] = t; }
+                if (code[q] & GC_SWY) { const T t = yp[q]; yp[q] = ym[q]; ym[q] = t; }
+            }
+        }
+    };
     // mu on ring R from the state level on ring R+1
     auto eval_mu = [&](auto rr, const T* src) {
         constexpr int R = decltype(rr)::value;
-        constexpr int F0 = (8 - R) * P, ND = (TY + 2 * R) * P;
-        constexpr int NIT = (ND + NT - 1) / NT;
+        constexpr int F0 = (8 - R) * P, NQ = (TY + 2 * R) * P / V;
+        constexpr int NIT = (NQ + NT - 1) / NT;
 #pragma unroll
-        for (int i0 = 0; i0 < NIT; i0 += G) {
-            T s0[G], s1[G], s2[G], s3[G], s4[G];
-            int code[G];
+        for (int i = 0; i < NIT; ++i) {
+            const int q = tid + NT * i;
+            const int f = F0 + V * (NQ % NT != 0 && i == NIT - 1 ? min(q, NQ - 1) : q);
+            T c[V], xp[V], xm[V], yp[V], ym[V];
+            int code[V];
+            operands(src, f, c, xp, xm, yp, ym, code);
+            VT o;
 #pragma unroll
-            for (int j = 0; j < G; ++j) {
-                if (i0 + j >= NIT) break;
-                const int f = F0 + (ND % NT != 0 && i0 + j == NIT - 1 ? min(tid + NT * (i0 + j), ND - 1)
-                                                                           : tid + NT * (i0 + j));
-                code[j] = GEN ? ghost_code_xy(x0 - 8 + f % P, y0 - 8 + f / P, A) : 0;
-                s0[j] = src[f];
-                s1[j] = src[f + 1];
-                s2[j] = src[f - 1];
-                s3[j] = src[f + P];
-                s4[j] = src[f - P];
+            for (int k = 0; k < V; ++k) {
+                T v = mu_at<T, EXACT, 2>(A, c[k], xp[k], xm[k], yp[k], ym[k], (T)0, (T)0);
+                if (GEN && (code[k] & GC_FIX)) v = ghost_value_xy(code[k], BCV);
+                o[k] = v;
             }
-#pragma unroll
-            for (int j = 0; j < G; ++j) {
-                if (i0 + j >= NIT) break;
-                T xp = s1[j], xm = s2[j], yp = s3[j], ym = s4[j];
-                if (GEN) {
-                    if (code[j] & GC_SWX) { const T q = xp; xp = xm; xm = q; }
-                    if (code[j] & GC_SWY) { const T q = yp; yp = ym; ym = q; }
-                }
-                T v = mu_at<T, EXACT, 2>(A, s0[j], xp, xm, yp, ym, (T)0, (T)0);
-                if (GEN && (code[j] & GC_FIX)) v = ghost_value_xy(code[j], BCV);
-                const int t = tid + NT * (i0 + j);
-                if (ND % NT == 0 || i0 + j < NIT - 1 || t < ND) SA[F0 + t] = v;
-            }
+            if (NQ % NT == 0 || i < NIT - 1 || q < NQ) *reinterpret_cast<VT*>(&SA[f]) = o;
         }
     };
     // stage state on ring R from the mu level on ring R+1: c_n + a*k, and the
@@ -1539,48 +1552,36 @@ __device__ __forceinline__ void step2d_levels(const PairArgs<T>& A, const T* SC,
     auto eval_state = [&](auto rr, auto ss, T a) {
         constexpr int R = decltype(rr)::value;
         constexpr int S = decltype(ss)::value;
-        constexpr int F0 = (8 - R) * P, ND = (TY + 2 * R) * P;
-        constexpr int NIT = (ND + NT - 1) / NT;
+        constexpr int F0 = (8 - R) * P, NQ = (TY + 2 * R) * P / V;
+        constexpr int NIT = (NQ + NT - 1) / NT;
 #pragma unroll
-        for (int i0 = 0; i0 < NIT; i0 += G) {
-            T s0[G], s1[G], s2[G], s3[G], s4[G], cc[G], ac[G];
-            int code[G];
+        for (int i = 0; i < NIT; ++i) {
+            const int q = tid + NT * i;
+            const int f = F0 + V * (NQ % NT != 0 && i == NIT - 1 ? min(q, NQ - 1) : q);
+            T m[V], xp[V], xm[V], yp[V], ym[V];
+            int code[V];
+            operands(SA, f, m, xp, xm, yp, ym, code);
+            const VT cc = *reinterpret_cast<const VT*>(&SC[f]);
+            VT ac;
+            if (S != 1) ac = *reinterpret_cast<const VT*>(&SACC[f]);
+            VT vo, no;
 #pragma unroll
-            for (int j = 0; j < G; ++j) {
-                if (i0 + j >= NIT) break;
-                const int f = F0 + (ND % NT != 0 && i0 + j == NIT - 1 ? min(tid + NT * (i0 + j), ND - 1)
-                                                                           : tid + NT * (i0 + j));
-                code[j] = GEN ? ghost_code_xy(x0 - 8 + f % P, y0 - 8 + f / P, A) : 0;
-                cc[j] = SC[f];
-                ac[j] = S == 1 ? (T)0 : SACC[f];
-                s0[j] = SA[f];
-                s1[j] = SA[f + 1];
-                s2[j] = SA[f - 1];
-                s3[j] = SA[f + P];
-                s4[j] = SA[f - P];
-            }
-#pragma unroll
-            for (int j = 0; j < G; ++j) {
-                if (i0 + j >= NIT) break;
-                T xp = s1[j], xm = s2[j], yp = s3[j], ym = s4[j];
-                if (GEN) {
-                    if (code[j] & GC_SWX) { const T q = xp; xp = xm; xm = q; }
-                    if (code[j] & GC_SWY) { const T q = yp; yp = ym; ym = q; }
-                }
-                T k = k_at<T, EXACT, 2>(A, s0[j], xp, xm, yp, ym, (T)0, (T)0);
-                T v = axpy<T, EXACT>(a, k, cc[j]);
-                if (GEN && (code[j] & GC_FIX)) {
-                    v = ghost_value_xy(code[j], BCV);
-                    k = (T)0;
+            for (int k = 0; k < V; ++k) {
+                T kk = k_at<T, EXACT, 2>(A, m[k], xp[k], xm[k], yp[k], ym[k], (T)0, (T)0);
+                T v = axpy<T, EXACT>(a, kk, cc[k]);
+                if (GEN && (code[k] & GC_FIX)) {
+                    v = ghost_value_xy(code[k], BCV);
+                    kk = (T)0;
                 }
                 T nac;
-                if (S == 1) nac = EXACT ? k : axpy<T, EXACT>(A.dt6, k, cc[j]);
-                else nac = EXACT ? ac[j] + (T)2 * k : axpy<T, EXACT>(A.dt3, k, ac[j]);
-                const int t = tid + NT * (i0 + j);
-                if (ND % NT == 0 || i0 + j < NIT - 1 || t < ND) {
-                    SB[F0 + t] = v;
-                    SACC[F0 + t] = nac;
-                }
+                if (S == 1) nac = EXACT ? kk : axpy<T, EXACT>(A.dt6, kk, cc[k]);
+                else nac = EXACT ? ac[k] + (T)2 * kk : axpy<T, EXACT>(A.dt3, kk, ac[k]);
+                vo[k] = v;
+                no[k] = nac;
+            }
+            if (NQ % NT == 0 || i < NIT - 1 || q < NQ) {
+                *reinterpret_cast<VT*>(&SB[f]) = vo;
+                *reinterpret_cast<VT*>(&SACC[f]) = no;
             }
         }
     };
@@ -1639,11 +1640,12 @@ __device__ __forceinline__ void step2d_levels(const PairArgs<T>& A, const T* SC,
 template <typename T, int TX, int TY, int NT, bool EXACT>
 __global__ __launch_bounds__(NT) void k_ch_step_2d(PairArgs<T> A) {
     constexpr int WC = TX + 16, NC = WC * (TY + 16);
-    // every level in the same frame as c_n (see step2d_levels)
-    __shared__ T SC[NC];
-    __shared__ T SA[NC];
-    __shared__ T SB[NC];
-    __shared__ T SACC[NC];
+    // every level in the same frame as c_n (see step2d_levels); 16-byte
+    // aligned for the levels' vector accesses
+    __shared__ __attribute__((aligned(16))) T SC[NC];
+    __shared__ __attribute__((aligned(16))) T SA[NC];
+    __shared__ __attribute__((aligned(16))) T SB[NC];
+    __shared__ __attribute__((aligned(16))) T SACC[NC];
     __shared__ T BCV[6];
     const int tid = threadIdx.x;
     stage_bc_values(BCV, A);
