@@ -509,14 +509,15 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 // translation invariant, so a ring pixel's activation equals the activation
 // at its wrapped position.
 //
-// Layer 1 (1 -> 32) does k_conv_in_fwd's FP32 operations in its order, so
-// a1 is the layer-by-layer kernel's bits. Layers 2-6 run on
-// v_mfma_f32_16x16x32_bf16 in the transposed form D^T = W^T A^T (rows =
+// Every layer runs on the MFMA in the transposed form D^T = W^T A^T (rows =
 // output channels, columns = 16 pixels), so a lane ends with 4 consecutive
 // output channels of one pixel:
-//   * layers 2-5 (32 -> 32): one MFMA per tap and 16 output channels, K = the
-//     32 input channels; the weights sit in VGPRs, each layer's loaded from
-//     global memory during the layer before it;
+//   * layer 1 (1 -> 32): FP32 operands on v_mfma_f32_16x16x4_f32, the 9 taps
+//     in three K steps of 4 (FP32 precision, as k_conv_in_fwd, another
+//     summation order);
+//   * layers 2-5 (32 -> 32): v_mfma_f32_16x16x32_bf16, one per tap and 16
+//     output channels, K = the 32 input channels; the weights sit in VGPRs,
+//     each layer's loaded from global memory during the layer before it;
 //   * layer 6 (32 -> 1): A rows 0 and 1 hold the high and low BF16 parts of
 //     the FP32 weights (~16 significant bits), d = D[0] + D[1].
 // Biases start the accumulators. The region is walked as a flat list of
@@ -525,7 +526,7 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 // then a halving tree over the 16 lanes, then the 8 waves in order).
 // Inference only: no activations are kept for a backward pass. The result
 // follows the layer-by-layer forward to within FP32 summation order (the
-// bias first, the MFMA's internal order) and the split of layer 6; it is not
+// bias first, the MFMAs' internal order) and the split of layer 6; it is not
 // the same bits.
 constexpr int FT = 16, FNT = 512;
 constexpr int FR0 = FT + 12;                    // 28: the x region
@@ -659,35 +660,45 @@ __global__ __launch_bounds__(FNT) void k_sur_fused_fwd(const float* __restrict__
             for (int mb = 0; mb < 2; ++mb)
                 r[tap][mb] = *reinterpret_cast<const bf16x8*>(wl + (tap * CH + lx + 16 * mb) * CH + 8 * g);
     };
-    // ---- layer 1 (1 -> 32) on the 26x26 region in FP32, k_conv_in_fwd's
-    // operations in its order: work item = (pixel, group of 8 channels); FNT
-    // is a multiple of 4, so a thread's group is fixed and its 72 weights and
-    // 8 biases are read once
+    // ---- layer 1 (1 -> 32) on the 26x26 region in FP32 on
+    // v_mfma_f32_16x16x4_f32: D^T (16 channels x 16 pixels) = W^T (16 x 12)
+    // X (12 x 16), K = the 9 taps in three steps of 4 (the weights of slots
+    // 9-11 are 0). Lane (g, lx) holds A[lx][g] = w[lx + 16 mb][4 j + g] and
+    // B[g][lx] = x of pixel lx at tap 4 j + g.
     {
-        static_assert(FNT % 4 == 0, "channel group per thread");
-        const int grp = tid & 3;
-        float w[8][9], bb[8];
+        constexpr int npix = FRA * FRA, nblk = (npix + 15) / 16;
+        float a1f[2][3], b1v[2][4];
+        int toff[3];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            bb[k] = W1[CH * 9 + 8 * grp + k];
+        for (int j = 0; j < 3; ++j) {
+            const int t = 4 * j + g;
+            toff[j] = t < 9 ? (t / 3) * FR0 + t % 3 : 0;
 #pragma unroll
-            for (int t = 0; t < 9; ++t) w[k][t] = W1[(8 * grp + k) * 9 + t];
+            for (int mb = 0; mb < 2; ++mb) a1f[mb][j] = t < 9 ? W1[(lx + 16 * mb) * 9 + t] : 0.f;
         }
-        for (int it = tid; it < FRA * FRA * 4; it += FNT) {
-            if (it == tid) load_w(0, bwn);
-            const int p = it >> 2, py = p / FRA, px = p - py * FRA;
-            float v[9];
 #pragma unroll
-            for (int t = 0; t < 9; ++t) v[t] = X[(py + t / 3) * FR0 + px + t % 3];
-            __attribute__((aligned(16))) __bf16 o[8];
+        for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                float sum = bb[k];
+            for (int r = 0; r < 4; ++r) b1v[mb][r] = W1[CH * 9 + 16 * mb + 4 * g + r];
+        for (int b = wv; b < nblk; b += FNT / 64) {
+            if (b == wv) load_w(0, bwn);
+            const int q = 16 * b + lx, qc = min(q, npix - 1);
+            const int py = qc / FRA, px = qc - py * FRA, base = py * FR0 + px;
+            float xb[3];
 #pragma unroll
-                for (int t = 0; t < 9; ++t) sum += w[k][t] * v[t];
-                o[k] = f2bf(fmaxf(sum, 0.f));
+            for (int j = 0; j < 3; ++j) xb[j] = X[base + toff[j]];
+#pragma unroll
+            for (int mb = 0; mb < 2; ++mb) {
+                f32x4 acc = {b1v[mb][0], b1v[mb][1], b1v[mb][2], b1v[mb][3]};
+#pragma unroll
+                for (int j = 0; j < 3; ++j) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a1f[mb][j], xb[j], acc, 0, 0, 0);
+                if (q < npix) {
+                    __attribute__((aligned(8))) __bf16 o[4];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) o[r] = f2bf(fmaxf(acc[r], 0.f));
+                    *reinterpret_cast<uint2*>(&ACT0[q * PIXP + 16 * mb + 4 * g]) = *reinterpret_cast<const uint2*>(o);
+                }
             }
-            *reinterpret_cast<uint4*>(&ACT0[p * PIXP + 8 * grp]) = *reinterpret_cast<const uint4*>(o);
         }
     }
     // ---- layers 2-5 (32 -> 32): input region rin, output region rin - 2
