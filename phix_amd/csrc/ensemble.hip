@@ -242,20 +242,27 @@ __device__ __forceinline__ int wrap4(int p, int n) {
 
 template <int TX, int TY, bool SECOND>
 __global__ __launch_bounds__(256) void k_ens_pair(EnsArgs A) {
-    constexpr int UX = TX + 8, UY = TY + 8;   // first-stage input, radius 4
-    constexpr int AX = TX + 6, AY = TY + 6;   // first-stage mu
-    constexpr int BX = TX + 4, BY = TY + 4;   // second-stage input (and c_n, SECOND)
-    constexpr int MX = TX + 2, MY = TY + 2;   // second-stage mu
-    constexpr int NS = (UX * UY + 255) / 256;
-    constexpr int NC = SECOND ? (BX * BY + 255) / 256 : 1;
+    // Every level lives in one frame of pitch UX: tile position (x, y) sits at
+    // (y + 4) * UX + x + 4 in U, MA, UB, MB, CB and KA alike. A level covers
+    // whole frame rows (ring R: rows 4-R .. 4+TY+R-1, all UX columns), cut
+    // into runs of 4 positions: a run's operands are three 16-byte LDS reads
+    // and the two scalars beside it, its results one 16-byte store. Columns
+    // outside a level's ring read finite entries of the level before (whose
+    // rows reach one further) and feed only other out-of-ring entries.
+    constexpr int UX = TX + 8, UY = TY + 8;   // the frame: first-stage input, radius 4
+    constexpr int NF = UX * UY;
+    constexpr int NS = (NF + 255) / 256;
+    constexpr int CR0 = 2 * UX, NCB = (TY + 4) * UX;   // c_n (SECOND) on whole rows of ring 2
+    constexpr int NC = SECOND ? (NCB + 255) / 256 : 1;
     constexpr int NO = (TX * TY) / 256;
     static_assert((TX * TY) % 256 == 0, "tile must be a multiple of 256 cells");
-    __shared__ float U[UY * UX];
-    __shared__ float MA[AY * AX];
-    __shared__ float UB[BY * BX];
-    __shared__ float MB[MY * MX];
-    __shared__ float CB[SECOND ? BY * BX : 1];
-    __shared__ float KA[TX * TY];   // first-stage RHS at the tile cells
+    static_assert(UX % 4 == 0, "frame rows hold whole runs");
+    __shared__ __attribute__((aligned(16))) float U[NF];
+    __shared__ __attribute__((aligned(16))) float MA[NF];
+    __shared__ __attribute__((aligned(16))) float UB[NF];
+    __shared__ __attribute__((aligned(16))) float MB[NF];
+    __shared__ __attribute__((aligned(16))) float CB[SECOND ? NF : 4];
+    __shared__ __attribute__((aligned(16))) float KA[NF];   // first-stage RHS (read at the tile cells)
     const int tid = threadIdx.x;
     // persistent walk, as k_ens_stage (XCD-aware when the grid allows)
     int w, G, last;
@@ -306,9 +313,9 @@ __global__ __launch_bounds__(256) void k_ens_pair(EnsArgs A) {
         if constexpr (SECOND) {
 #pragma unroll
             for (int i = 0; i < NC; ++i) {
-                const int t = min(tid + 256 * i, BX * BY - 1);
-                const int ly = t / BX, lx = t - ly * BX;
-                const int px = wrap4(min(T.x0 - 2 + lx, n + 3), n), py = wrap4(min(T.y0 - 2 + ly, n + 3), n);
+                const int t = min(tid + 256 * i, NCB - 1);
+                const int ly = t / UX, lx = t - ly * UX;
+                const int px = wrap4(min(T.x0 - 4 + lx, n + 3), n), py = wrap4(min(T.y0 - 2 + ly, n + 3), n);
                 cv[i] = A.c[off + (long long)py * n + px];
             }
 #pragma unroll
@@ -333,7 +340,7 @@ __global__ __launch_bounds__(256) void k_ens_pair(EnsArgs A) {
         if constexpr (SECOND) {
 #pragma unroll
             for (int i = 0; i < NC; ++i)
-                if (tid + 256 * i < BX * BY) CB[tid + 256 * i] = cv[i];
+                if (tid + 256 * i < NCB) CB[CR0 + tid + 256 * i] = cv[i];
 #pragma unroll
             for (int j = 0; j < NO; ++j) pc[j] = pa[j];
         }
@@ -342,28 +349,61 @@ __global__ __launch_bounds__(256) void k_ens_pair(EnsArgs A) {
         const int n = R.n;
         const long long off = R.off;
         const float nk = -R.kappa, M = R.M;
-        for (int t = tid; t < AX * AY; t += 256) {
-            const int ay = t / AX, ax = t - ay * AX;
-            const int c0 = (ay + 1) * UX + ax + 1;
-            MA[t] = ens_mu(U[c0], U[c0 + 1], U[c0 - 1], U[c0 + UX], U[c0 - UX], nk);
-        }
+        typedef float f4 __attribute__((ext_vector_type(4)));
+        // the run at f of a level held in src: centre, x and y neighbours
+        auto run_ops = [&](const float* src, int f, f4& c, f4& xp, f4& xm, f4& yp, f4& ym) {
+            c = *reinterpret_cast<const f4*>(&src[f]);
+            yp = *reinterpret_cast<const f4*>(&src[f + UX]);
+            ym = *reinterpret_cast<const f4*>(&src[f - UX]);
+            const float l = src[f - 1], r = src[f + 4];
+            xp = f4{c[1], c[2], c[3], r};
+            xm = f4{l, c[0], c[1], c[2]};
+        };
+        // runs of the level on ring R: thread tid takes runs tid + 256 i (the
+        // last partial pass reads a valid run and skips the store)
+        auto level = [&](auto rr, auto body) {
+            constexpr int R = decltype(rr)::value;
+            constexpr int F0 = (4 - R) * UX, NQ = (TY + 2 * R) * UX / 4, NIT = (NQ + 255) / 256;
+#pragma unroll
+            for (int i = 0; i < NIT; ++i) {
+                const int q = tid + 256 * i;
+                const int f = F0 + 4 * (NQ % 256 != 0 && i == NIT - 1 ? min(q, NQ - 1) : q);
+                body(f, NQ % 256 == 0 || i < NIT - 1 || q < NQ);
+            }
+        };
+        // first-stage mu on ring 3
+        level(std::integral_constant<int, 3>{}, [&](int f, bool st4) {
+            f4 c, xp, xm, yp, ym, o;
+            run_ops(U, f, c, xp, xm, yp, ym);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) o[k] = ens_mu(c[k], xp[k], xm[k], yp[k], ym[k], nk);
+            if (st4) *reinterpret_cast<f4*>(&MA[f]) = o;
+        });
         __syncthreads();
-        // first stage: u_B = base + a k on tile+2; k kept at the tile cells
-        for (int t = tid; t < BX * BY; t += 256) {
-            const int by = t / BX, bx = t - by * BX;
-            const int m0 = (by + 1) * AX + bx + 1;
-            const float k = ens_k(MA[m0], MA[m0 + 1], MA[m0 - 1], MA[m0 + AX], MA[m0 - AX], M);
-            const float base = SECOND ? CB[t] : U[(by + 2) * UX + bx + 2];
-            UB[t] = __builtin_fmaf(A.a, k, base);
-            const int ox = bx - 2, oy = by - 2;
-            if (ox >= 0 && ox < TX && oy >= 0 && oy < TY) KA[oy * TX + ox] = k;
-        }
+        // first stage: u_B = base + a k on ring 2; k kept (read at the tile cells)
+        level(std::integral_constant<int, 2>{}, [&](int f, bool st4) {
+            f4 m, xp, xm, yp, ym, o, kk;
+            run_ops(MA, f, m, xp, xm, yp, ym);
+            const f4 base = *reinterpret_cast<const f4*>(SECOND ? &CB[f] : &U[f]);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                kk[k] = ens_k(m[k], xp[k], xm[k], yp[k], ym[k], M);
+                o[k] = __builtin_fmaf(A.a, kk[k], base[k]);
+            }
+            if (st4) {
+                *reinterpret_cast<f4*>(&UB[f]) = o;
+                *reinterpret_cast<f4*>(&KA[f]) = kk;
+            }
+        });
         __syncthreads();
-        for (int t = tid; t < MX * MY; t += 256) {
-            const int my = t / MX, mx = t - my * MX;
-            const int c0 = (my + 1) * BX + mx + 1;
-            MB[t] = ens_mu(UB[c0], UB[c0 + 1], UB[c0 - 1], UB[c0 + BX], UB[c0 - BX], nk);
-        }
+        // second-stage mu on ring 1
+        level(std::integral_constant<int, 1>{}, [&](int f, bool st4) {
+            f4 c, xp, xm, yp, ym, o;
+            run_ops(UB, f, c, xp, xm, yp, ym);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) o[k] = ens_mu(c[k], xp[k], xm[k], yp[k], ym[k], nk);
+            if (st4) *reinterpret_cast<f4*>(&MB[f]) = o;
+        });
         __syncthreads();
         // second stage and the RK4 epilogue at the tile cells
 #pragma unroll
@@ -372,16 +412,16 @@ __global__ __launch_bounds__(256) void k_ens_pair(EnsArgs A) {
             const int oy = t / TX, ox = t - oy * TX;
             const int x = T.x0 + ox, y = T.y0 + oy;
             if (x >= n || y >= n) continue;
-            const int m0 = (oy + 1) * MX + ox + 1;
-            const float k = ens_k(MB[m0], MB[m0 + 1], MB[m0 - 1], MB[m0 + MX], MB[m0 - MX], M);
+            const int m0 = (oy + 4) * UX + ox + 4;
+            const float k = ens_k(MB[m0], MB[m0 + 1], MB[m0 - 1], MB[m0 + UX], MB[m0 - UX], M);
             const long long gi = off + (long long)y * n + x;
             if constexpr (!SECOND) {
-                const float cn = U[(oy + 4) * UX + ox + 4];
-                const float am = __builtin_fmaf(A.dt6, KA[t], cn);   // accumulator after stage 1
+                const float cn = U[m0];
+                const float am = __builtin_fmaf(A.dt6, KA[m0], cn);   // accumulator after stage 1
                 A.acc[gi] = __builtin_fmaf(A.dt3, k, am);
                 A.out[gi] = __builtin_fmaf(A.a2, k, cn);
             } else {
-                const float am = __builtin_fmaf(A.dt3, KA[t], pc[j]);   // after stage 3
+                const float am = __builtin_fmaf(A.dt3, KA[m0], pc[j]);   // after stage 3
                 A.out[gi] = __builtin_fmaf(A.dt6, k, am);
             }
         }
