@@ -315,6 +315,93 @@ def train_rollout(dev, snaps, batch=16, epochs=1, lr=1e-3, seed=0, params=None, 
     return net, hist
 
 
+def dagger_pairs(dev, params, snaps, npairs, max_jumps, batch=64, seed=0, n=256, kappa=0.5, dt=0.01, every=100):
+    """Pairs (x, F(x)) on the network's own rollout states: x is the network
+    rolled j jumps (j uniform in 1..max_jumps, one j per batch) from a solver
+    snapshot of `snaps` (K+1, R, n, n), and the target is the plain FP32
+    solver's `every` RK4 steps from x (phx_ch_rk4_steps_batch), so the
+    network learns F where parareal evaluates it: at the states its own
+    prediction sweep produces. Returns (X, Y), (npairs, n, n) FP32 device
+    tensors."""
+    import torch
+    K1, R = snaps.shape[0], snaps.shape[1]
+    rng = np.random.default_rng(seed)
+    net = Surrogate(dev, batch, n=n, params=params)
+    net.forward(snaps[0, :batch].contiguous() if R >= batch else snaps[0, :1].expand(batch, n, n).contiguous())
+    J = PlainSolver(dev, n=n, kappa=kappa, dt=dt)
+    arena = torch.zeros((batch,) + tuple(J.shape), dtype=torch.float32, device=dev)
+    work = torch.empty(batch * J.work.numel(), dtype=torch.float32, device=dev)
+    phys = arena[:, 1, 1:n + 1, 1:n + 1]
+    X = torch.empty((npairs, n, n), dtype=torch.float32, device=dev)
+    Y = torch.empty((npairs, n, n), dtype=torch.float32, device=dev)
+    bufs = [torch.empty((batch, n, n), dtype=torch.float32, device=dev) for _ in range(2)]
+    flat = snaps.reshape(K1 * R, n, n)
+    for b0 in range(0, npairs, batch):
+        m = min(batch, npairs - b0)
+        j = int(rng.integers(1, max_jumps + 1))
+        k = rng.integers(0, max(1, K1 - 1), batch)
+        r = rng.integers(0, R, batch)
+        x0 = flat.index_select(0, torch.from_numpy(k * R + r).to(dev)).contiguous()
+        x = net.rollout(x0, j, bufs)
+        phys.copy_(x)
+        X[b0:b0 + m] = x[:m]
+        capi.call("phx_ch_rk4_steps_batch", C.byref(J.g), J.dtype, batch, capi.ptr(arena), capi.ptr(work),
+                  C.byref(J.p), every, capi.stream_handle())
+        Y[b0:b0 + m] = phys[:m]
+    return X, Y
+
+
+def train_dagger(dev, snaps, X, Y, batch=16, lr=1e-4, seed=0, params=None, log=None, max_seconds=600,
+                 frac=0.5, noise=1e-3, steps=None):
+    """Fine-tuning on a mix: a fraction `frac` of the batches from the pairs
+    (X, Y) of dagger_pairs, the rest one-step pairs of the snapshots with
+    input noise; cosine learning-rate decay over `steps` batches (or the time
+    budget). Returns (Surrogate, list of (step, mean loss) records)."""
+    import torch
+    K1, R, n, _ = snaps.shape
+    net = Surrogate(dev, batch, n=n, seed=seed, params=params)
+    rng = np.random.default_rng(seed)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    flat = snaps.reshape(K1 * R, n, n)
+    npairs = X.shape[0]
+    total = steps or 10 ** 9
+    t0 = time.time()
+    hist = []
+    acc = torch.zeros(1, dtype=torch.float64, device=dev)
+    cnt = 0
+    b = 0
+    while b < total:
+        el = time.time() - t0
+        if el > max_seconds:
+            break
+        prog = b / total if steps else el / max_seconds
+        lr_now = lr * 0.5 * (1 + math.cos(math.pi * min(1.0, prog)))
+        if rng.random() < frac:
+            idx = torch.from_numpy(rng.integers(0, npairs, batch)).to(dev)
+            x = X.index_select(0, idx).contiguous()
+            y = Y.index_select(0, idx).contiguous()
+        else:
+            k = rng.integers(0, K1 - 1, batch)
+            r = rng.integers(0, R, batch)
+            src = torch.from_numpy(k * R + r).to(dev)
+            x = flat.index_select(0, src)
+            if noise > 0:
+                x = x + noise * torch.randn(x.shape, device=dev, generator=gen)
+            x = x.contiguous()
+            y = flat.index_select(0, src + R).contiguous()
+        acc += net.train_step(x, y, lr_now)
+        cnt += 1
+        b += 1
+        if b % 2000 == 0:
+            hist.append((b, acc.item() / (cnt * batch * n * n)))
+            acc.zero_()
+            cnt = 0
+            if log:
+                log(f"batch {b}: mse {hist[-1][1]:.3e} ({time.time() - t0:.0f} s)")
+    return net, hist
+
+
 def rollout_error(dev, params, snaps, traj, jumps):
     """Relative L2 of a pure surrogate rollout of `jumps` jumps against the
     solver snapshots of trajectory `traj`, after each jump."""
@@ -633,7 +720,7 @@ def main(argv=None):
     import json
     import torch
     ap = argparse.ArgumentParser(description="train or evaluate the CH surrogate")
-    ap.add_argument("mode", choices=["train", "train-rollout", "eval"])
+    ap.add_argument("mode", choices=["train", "train-rollout", "train-dagger", "eval"])
     ap.add_argument("--ntraj", type=int, default=64)
     ap.add_argument("--nsnap", type=int, default=40)
     ap.add_argument("--epochs", type=int, default=4)
@@ -654,10 +741,26 @@ def main(argv=None):
     ap.add_argument("--parareal", action="store_true", help="eval the parareal form")
     ap.add_argument("--slices", type=int, default=40)
     ap.add_argument("--ic", type=int, default=7, help="initial condition: SplitMix64 substream (1234, ic)")
+    ap.add_argument("--dagger-pairs", type=int, default=20000, help="train-dagger: rollout-state pairs")
+    ap.add_argument("--max-jumps", type=int, default=100, help="train-dagger: longest rollout before F")
     ap.add_argument("--lead", type=int, default=0,
                     help="parareal: plain solver steps before the parareal phase (slices divide the rest)")
     a = ap.parse_args(argv)
     dev = torch.device("cuda:0")
+    if a.mode == "train-dagger":
+        t0 = time.time()
+        snaps = generate_snapshots(dev, a.ntraj, a.nsnap, seed=a.seed, skip=a.skip)
+        init_p = np.load(a.init)["params"]
+        X, Y = dagger_pairs(dev, init_p, snaps, a.dagger_pairs, a.max_jumps, seed=a.seed)
+        torch.cuda.synchronize()
+        print(f"data: {a.ntraj} trajectories, {a.dagger_pairs} rollout-state pairs in {time.time() - t0:.1f} s",
+              flush=True)
+        net, hist = train_dagger(dev, snaps, X, Y, batch=a.batch, lr=a.lr, params=init_p,
+                                 log=lambda m: print(m, flush=True), max_seconds=a.seconds, noise=a.noise)
+        np.savez(a.out, params=net.params_numpy(), mse=np.array(hist), ntraj=a.ntraj, nsnap=a.nsnap, seed=a.seed,
+                 skip=a.skip, dagger_pairs=a.dagger_pairs, max_jumps=a.max_jumps)
+        print(json.dumps({"saved": a.out, "mse": hist[-3:]}), flush=True)
+        return
     if a.mode == "train-rollout":
         t0 = time.time()
         snaps = generate_snapshots(dev, a.ntraj, a.nsnap, seed=a.seed, skip=a.skip)

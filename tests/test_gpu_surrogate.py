@@ -437,3 +437,23 @@ def test_config5_surrogate_plus_solver_against_plain_solver(dev):
     assert ev["speedup_median"] >= 3.5
     assert ev["rel_l2_median"] <= 3e-2
 
+
+
+def test_dagger_pairs_targets_are_the_solver_from_rollout_states(dev):
+    """dagger_pairs: each target is the plain solver's 100 RK4 steps from its
+    input (the network's rollout of a snapshot), bit for bit, and the inputs
+    keep their snapshot's mean."""
+    import torch
+    from phix_amd import surrogate as S
+    p = np.load(S.WEIGHTS_LATE)["params"]
+    snaps = S.generate_snapshots(dev, 3, 2, seed=3, n=32)
+    X, Y = S.dagger_pairs(dev, p, snaps, 6, 3, batch=4, seed=1, n=32)
+    solver = S.PlainSolver(dev, n=32)
+    means = snaps[0].mean(dim=(1, 2)).double()
+    for i in range(6):
+        solver.set(X[i])
+        solver.steps(100)
+        torch.cuda.synchronize()
+        assert torch.equal(solver.phys(), Y[i]), i
+        assert torch.isfinite(X[i]).all()
+        assert (means - X[i].double().mean()).abs().min().item() < 1e-5
