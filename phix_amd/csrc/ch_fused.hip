@@ -908,11 +908,23 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, NT))) void k_ch_stage_3
         }
     };
 
-    T u0[NS], u1[NS], u2[NS], upf[NS], qv[NQ];
+    T u0[NS], u1[NS], u2[NS];
     T mum[NO], mu0[NO], mup[NS];
-    T pc[NO], pa[NO], pq[NO], nc[NO], na[NO], nq[NO];
-    bool ufix, qfix;
-    T uval, qval;
+    // Loads in flight. DEPTH 2 (FP64, and FP32 tiles of at most two cells
+    // per thread): two sets, one per plane parity; iteration o consumes its
+    // set (state plane o+2 at the mu slots, outer plane o+1, epilogue
+    // operands of plane o) and refills it for iteration o+2, so every load
+    // has two iterations to land and no register copy waits for one.
+    // DEPTH 1 (FP32 tiles of four cells per thread): one set refilled for
+    // the next iteration, the epilogue operands staged in the second set's
+    // and copied at the iteration's end -- fewer VGPRs, which these need
+    // for occupancy.
+    constexpr int DEPTH = (sizeof(T) == 8 || NO <= 2) ? 2 : 1;
+    struct Pending {
+        T upf[NS], qv[NQ], ec[NO], ea[NO], eb[NO];
+        bool ufix, qfix;
+        T uval, qval;
+    } PA, PB;
     // ---- prologue: state planes z0-2 .. z0+1 at the mu slots, z0-1 and z0
     // on the outer ring; mu(z0-1) on the interior, mu(z0) everywhere
     {
@@ -925,7 +937,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, NT))) void k_ch_stage_3
         load_cols(z0 + 1, r3, f3, v3);
         load_outer(z0 - 1, o0, g0, w0);
         load_outer(z0, o1, g1, w1);
-        load_epi(z0, pc, pa, pq);
+        load_epi(z0, PA.ec, PA.ea, PA.eb);
         T a0[NS], a1[NS];
         resolve_cols(r0, f0, v0, a0);
         resolve_cols(r1, f1, v1, a1);
@@ -940,8 +952,13 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, NT))) void k_ch_stage_3
         for (int j = 0; j < NO; ++j) mum[j] = zfix ? zv : mu_slot(j, u0[j], a0[j]);
         __syncthreads();
         store_state(u0, o1, g1, w1);
-        load_cols(min(z0 + 2, z1 + 1), upf, ufix, uval);
-        load_outer(min(z0 + 1, z1), qv, qfix, qval);
+        load_cols(min(z0 + 2, z1 + 1), PA.upf, PA.ufix, PA.uval);
+        load_outer(min(z0 + 1, z1), PA.qv, PA.qfix, PA.qval);
+        if constexpr (DEPTH == 2) {
+            load_cols(min(z0 + 3, z1 + 1), PB.upf, PB.ufix, PB.uval);
+            load_outer(min(z0 + 2, z1), PB.qv, PB.qfix, PB.qval);
+            load_epi(min(z0 + 1, z1 - 1), PB.ec, PB.ea, PB.eb);
+        }
         __syncthreads();
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
@@ -957,13 +974,13 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, NT))) void k_ch_stage_3
         __syncthreads();
     }
 
-    for (int o = z0; o < z1; ++o) {
-        resolve_cols(upf, ufix, uval, u2);
-        store_state(u1, qv, qfix, qval);
+    auto iteration = [&](int o, Pending& P, Pending& N) {
+        resolve_cols(P.upf, P.ufix, P.uval, u2);
+        store_state(u1, P.qv, P.qfix, P.qval);
         // unconditional prefetch (clamped planes): the same loads every iteration
-        load_cols(min(o + 3, z1 + 1), upf, ufix, uval);
-        load_outer(min(o + 2, z1), qv, qfix, qval);
-        load_epi(min(o + 1, z1 - 1), nc, na, nq);
+        load_cols(min(o + 2 + DEPTH, z1 + 1), P.upf, P.ufix, P.uval);
+        load_outer(min(o + 1 + DEPTH, z1), P.qv, P.qfix, P.qval);
+        if constexpr (DEPTH == 1) load_epi(min(o + 1, z1 - 1), N.ec, N.ea, N.eb);
         __syncthreads();   // B1
         bool zfix, znf;
         T zv;
@@ -992,17 +1009,18 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, NT))) void k_ch_stage_3
                     acc = k;
                     out = uc + A.a * k;
                 } else if (st < 4) {
-                    acc = pa[j] + (T)2 * k;
-                    out = pc[j] + A.a * k;
+                    acc = P.ea[j] + (T)2 * k;
+                    out = P.ec[j] + A.a * k;
                 } else {
-                    out = pc[j] + A.dt6 * (pa[j] + k);
+                    out = P.ec[j] + A.dt6 * (P.ea[j] + k);
                 }
             } else {
                 if (st == 1) out = axpy<T, false>(A.a, k, uc);
-                else if (st < 4) out = axpy<T, false>(A.a, k, pc[j]);
-                else out = rk4_fin<T>(A.dt6, A.third, pc[j], pa[j], pq[j], uc, k);
+                else if (st < 4) out = axpy<T, false>(A.a, k, P.ec[j]);
+                else out = rk4_fin<T>(A.dt6, A.third, P.ec[j], P.ea[j], P.eb[j], uc, k);
             }
         }
+        if constexpr (DEPTH == 2) load_epi(min(o + 2, z1 - 1), P.ec, P.ea, P.eb);
         __syncthreads();   // B2
 #pragma unroll
         for (int s = 0; s < NS; ++s)
@@ -1016,10 +1034,22 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, NT))) void k_ch_stage_3
         for (int j = 0; j < NO; ++j) {
             mum[j] = mu0[j];
             mu0[j] = mup[j];
-            pc[j] = nc[j];
-            pa[j] = na[j];
-            pq[j] = nq[j];
+            if constexpr (DEPTH == 1) {
+                P.ec[j] = N.ec[j];
+                P.ea[j] = N.ea[j];
+                P.eb[j] = N.eb[j];
+            }
         }
+    };
+    int o = z0;
+    if constexpr (DEPTH == 2) {
+        for (; o + 1 < z1; o += 2) {
+            iteration(o, PA, PA);
+            iteration(o + 1, PB, PB);
+        }
+        if (o < z1) iteration(o, PA, PA);
+    } else {
+        for (; o < z1; ++o) iteration(o, PA, PB);
     }
 }
 
@@ -2493,9 +2523,7 @@ static Launch3 launch_for(const phx_grid* G, int dtype) {
     // defaults: the launches measured fastest at 1024^2 and 256^3 on MI355X
     // (2D: whole-step 128x32 per 1024 threads in FP32 on grids of 256^2 and
     // more, 32x32 per 512 threads otherwise and in FP64; 3D: z-register
-    // kernel with 64x16 columns per 256-thread workgroup at z-chunk 16 in
-    // FP32, 128x16 per 512 threads at z-chunk 32 in FP64); every launch
-    // returns the same bits
+    // kernel per stage class, below); every launch returns the same bits
     if (G->dim == 2) {
         if (!step2d_ok(G)) return uniform_launch(1, 0);
         return uniform_launch(1, dtype == PHX_F32 && G->n[0] >= 256 && G->n[1] >= 256 ? 7 : 5);
@@ -2511,14 +2539,15 @@ static Launch3 launch_for(const phx_grid* G, int dtype) {
         l.zc4 = (int)cdiv(G->n[2], 16);
         return l;
     }
-    // FP64, per stage class (256^3 measurement): stage 1 64x16/256 at 12
-    // chunks, stages 2-3 128x8/512 at 8 chunks, stage 4 128x8/512 at 4 chunks
-    l.tile1 = kFirstZ + 0;
-    l.zc1 = (int)cdiv(G->n[2], 12);
+    // FP64, per stage class (256^3 measurement, loads two planes ahead):
+    // stage 1 128x16/512 at 8 chunks, stages 2-3 128x8/512 at 4 chunks,
+    // stage 4 128x8/512 at 8 chunks
+    l.tile1 = kFirstZ + 2;
+    l.zc1 = (int)cdiv(G->n[2], 8);
     l.tile = kFirstZ + 8;
-    l.zc = (int)cdiv(G->n[2], 8);
+    l.zc = (int)cdiv(G->n[2], 4);
     l.tile4 = kFirstZ + 8;
-    l.zc4 = (int)cdiv(G->n[2], 4);
+    l.zc4 = (int)cdiv(G->n[2], 8);
     return l;
 }
 

@@ -81,7 +81,7 @@ def test_launch_table_defaults_and_ranges():
     lib = capi.lib()
     g3 = capi.make_grid((40, 36, 34), bc=capi.NOFLUX)
     assert _launch(g3, capi.F32) == (2, 11)     # stages 2-3: z-register 128x16/512 at 24 z-chunks
-    assert _launch(g3, capi.F64) == (5, 17)     # stages 2-3: z-register 128x8/512 at 8 z-chunks
+    assert _launch(g3, capi.F64) == (9, 17)     # stages 2-3: z-register 128x8/512 at 4 z-chunks
     g2 = capi.make_grid((1024, 1024), bc=capi.PERIODIC)
     assert _launch(g2, capi.F32) == (1, 7)      # whole-step 128x32, 1024 threads
     assert _launch(g2, capi.F64) == (1, 5)      # whole-step 32x32, 512 threads
@@ -107,7 +107,7 @@ def test_launch_table_defaults_and_ranges():
     # settings are kept per (dim, shape, dtype)
     assert lib.phx_ch_rk4_set_zchunk(C.byref(g3), capi.F32, 22) == 0
     assert _launch(g3, capi.F32) == (22, 17)
-    assert _launch(g3, capi.F64) == (5, 17)
+    assert _launch(g3, capi.F64) == (9, 17)
     assert lib.phx_ch_rk4_set_zchunk(C.byref(g3), capi.F32, 0) == capi.PHX_ERR_INVALID
     capi.call("phx_ch_rk4_set_tile", C.byref(g3), capi.F32, 11)
     capi.call("phx_ch_rk4_set_zchunk", C.byref(g3), capi.F32, 16)
@@ -165,7 +165,7 @@ def test_stage_class_launches():
     argument checks (host state only)."""
     lib = capi.lib()
     g = capi.make_grid((256, 256, 256), bc=capi.NOFLUX)
-    assert [_stage(g, capi.F64, k) for k in range(3)] == [(22, 9), (32, 17), (64, 17)]
+    assert [_stage(g, capi.F64, k) for k in range(3)] == [(32, 11), (64, 17), (32, 17)]
     assert [_stage(g, capi.F32, k) for k in range(3)] == [(11, 11), (11, 11), (16, 12)]
     g5 = capi.make_grid((64, 48, 40), bc=capi.NOFLUX)
     capi.call("phx_ch_rk4_set_stage_launch", C.byref(g5), capi.F32, 2, 3, 11)
