@@ -464,9 +464,12 @@ def bench_euler(a, with_cpu):
     run(10)
     ms = timed(lambda: run(steps), stream)
     cells = NX * NX
-    # one launch per step (k_ch_stage_2d in its exact form, Euler stage; the
-    # last step of a call uses k_euler_mu + k_euler_c): read c, write c'
-    roof = roofline(2 * 8 * cells, (ms / 1e3) / steps, "k_ch_stage_2d (Euler step)", "ch2d_euler_f64_128")
+    # steps 1..996 of the call run as 249 launches of four steps each (the
+    # whole-step kernel's Euler form), then three one-step launches and the
+    # last step's k_euler_mu + k_euler_c: a launch is credited with four
+    # steps of read c, write c' and given four steps' share of the window
+    roof = roofline(4 * 2 * 8 * cells, 4 * (ms / 1e3) / steps, "k_ch_step_2d (Euler form, four steps per launch)",
+                    "ch2d_euler_f64_128")
     cpu = None
     if with_cpu:
         from oracle import oracle as O
@@ -479,6 +482,7 @@ def bench_euler(a, with_cpu):
                "sample": f"the full {steps}-step run, FP64 oracle, {dt:.1f} s on {cpu_model()}"}
     cfg = {"workload": "ch2d_euler_128sq_periodic", "grid": [NX, NX], "bc": "periodic", "scheme": "euler multi-step",
            "dt": 0.01, "kappa": 0.5, "M": 1.0, "bulk": "f64", "parallelism": "single-gpu",
+           "launch": "four steps per launch, 32x16 cells per 512-thread workgroup",
            "note": "135 KB working set is cache resident: latency bound, not HBM bound"}
     return line("cell-updates/s, Cahn-Hilliard multi-step Euler (config-1 app)", cells * steps / (ms / 1e3),
                 "cell-updates/s", steps, 10, ms, "f64", "synthetic (c0 = 0.5 + U(-0.01, 0.01), SplitMix64 seed 1)",

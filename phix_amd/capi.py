@@ -123,6 +123,7 @@ SIG = {
     "phx_convert_f64_to_f32": (C.c_int, [VP, VP, C.c_size_t, VP]),
     "phx_convert_f32_to_f64": (C.c_int, [VP, VP, C.c_size_t, VP]),
     "phx_ch_euler_set_fused": (C.c_int, [C.c_int]),
+    "phx_ch_euler_set_quad": (C.c_int, [C.c_int]),
     "phx_ch_euler_steps": (C.c_int, [GP, C.c_int, VP, VP, C.POINTER(CHParams), C.c_int, VP]),
     "phx_ch_rk4_workspace_bytes": (C.c_size_t, [GP, C.c_int]),
     "phx_gfa_workspace_bytes": (C.c_size_t, [GP]),

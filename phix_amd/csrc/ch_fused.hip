@@ -1651,6 +1651,101 @@ __device__ __forceinline__ void step2d_levels(const PairArgs<T>& A, const T* SC,
     }
 }
 
+// Four explicit Euler steps of a periodic 2D field in the whole-step frame:
+// c in SC (ring 8), the mu levels in SA. Step s evaluates mu on ring 7-2s
+// from the state on ring 8-2s, then the state c + dt*k on ring 6-2s in
+// place (a run reads its own centre and SA's neighbours only); the last
+// update writes the tile cells to global memory. Runs and frame as in
+// step2d_levels; the operations are the one-stage kernel's Euler step in
+// the reference's order (mu_at, k_at, c + dt*k, exact form), so one launch
+// gives the bits of four one-stage Euler launches.
+template <typename T, int TX, int TY, int NT>
+__device__ __forceinline__ void euler2d_levels(const PairArgs<T>& A, T* SC, T* SA, int x0, int y0, long long zb) {
+    constexpr int P = TX + 16;
+    constexpr int V = 16 / (int)sizeof(T);
+    typedef T VT __attribute__((ext_vector_type(V)));
+    static_assert(P % V == 0, "frame rows hold whole runs");
+    static_assert(NT % TX == 0, "tile rows must divide the workgroup");
+    const int tid = threadIdx.x;
+    auto operands = [&](const T* src, int f, T (&c)[V], T (&xp)[V], T (&xm)[V], T (&yp)[V], T (&ym)[V]) {
+        const VT cv = *reinterpret_cast<const VT*>(&src[f]);
+        const VT uv = *reinterpret_cast<const VT*>(&src[f + P]);
+        const VT dv = *reinterpret_cast<const VT*>(&src[f - P]);
+        const T l = src[f - 1], r = src[f + V];
+#pragma unroll
+        for (int q = 0; q < V; ++q) {
+            c[q] = cv[q];
+            xp[q] = q + 1 < V ? cv[q + 1] : r;
+            xm[q] = q > 0 ? cv[q - 1] : l;
+            yp[q] = uv[q];
+            ym[q] = dv[q];
+        }
+    };
+    auto eval_mu = [&](auto rr) {
+        constexpr int R = decltype(rr)::value;
+        constexpr int F0 = (8 - R) * P, NQ = (TY + 2 * R) * P / V;
+        constexpr int NIT = (NQ + NT - 1) / NT;
+#pragma unroll
+        for (int i = 0; i < NIT; ++i) {
+            const int q = tid + NT * i;
+            const int f = F0 + V * (NQ % NT != 0 && i == NIT - 1 ? min(q, NQ - 1) : q);
+            T c[V], xp[V], xm[V], yp[V], ym[V];
+            operands(SC, f, c, xp, xm, yp, ym);
+            VT o;
+#pragma unroll
+            for (int k = 0; k < V; ++k) o[k] = mu_at<T, true, 2>(A, c[k], xp[k], xm[k], yp[k], ym[k], (T)0, (T)0);
+            if (NQ % NT == 0 || i < NIT - 1 || q < NQ) *reinterpret_cast<VT*>(&SA[f]) = o;
+        }
+    };
+    auto eval_update = [&](auto rr) {
+        constexpr int R = decltype(rr)::value;
+        constexpr int F0 = (8 - R) * P, NQ = (TY + 2 * R) * P / V;
+        constexpr int NIT = (NQ + NT - 1) / NT;
+#pragma unroll
+        for (int i = 0; i < NIT; ++i) {
+            const int q = tid + NT * i;
+            const int f = F0 + V * (NQ % NT != 0 && i == NIT - 1 ? min(q, NQ - 1) : q);
+            T m[V], xp[V], xm[V], yp[V], ym[V];
+            operands(SA, f, m, xp, xm, yp, ym);
+            const VT cc = *reinterpret_cast<const VT*>(&SC[f]);
+            VT vo;
+#pragma unroll
+            for (int k = 0; k < V; ++k)
+                vo[k] = cc[k] + A.a1 * k_at<T, true, 2>(A, m[k], xp[k], xm[k], yp[k], ym[k], (T)0, (T)0);
+            if (NQ % NT == 0 || i < NIT - 1 || q < NQ) *reinterpret_cast<VT*>(&SC[f]) = vo;
+        }
+    };
+    eval_mu(std::integral_constant<int, 7>{});
+    __syncthreads();
+    eval_update(std::integral_constant<int, 6>{});
+    __syncthreads();
+    eval_mu(std::integral_constant<int, 5>{});
+    __syncthreads();
+    eval_update(std::integral_constant<int, 4>{});
+    __syncthreads();
+    eval_mu(std::integral_constant<int, 3>{});
+    __syncthreads();
+    eval_update(std::integral_constant<int, 2>{});
+    __syncthreads();
+    eval_mu(std::integral_constant<int, 1>{});
+    __syncthreads();
+    // the fourth update on the tile cells: thread tid owns column tid % TX of
+    // rows tid / TX + (NT / TX) j
+    constexpr int RS = NT / TX, NIT = TY / RS;
+    static_assert(TY % RS == 0, "tile rows must be a multiple of NT / TX");
+    const int ox = tid % TX, oy0 = tid / TX;
+    const int fb = (8 + oy0) * P + 8 + ox;
+    const int x = x0 + ox;
+    const long long gb = zb + (long long)(y0 + oy0 + A.L.g) * A.L.sx + (x + A.L.g);
+#pragma unroll
+    for (int j = 0; j < NIT; ++j) {
+        const int f = fb + j * RS * P;
+        const T k = k_at<T, true, 2>(A, SA[f], SA[f + 1], SA[f - 1], SA[f + P], SA[f - P], (T)0, (T)0);
+        const T r = SC[f] + A.a1 * k;
+        if (x < A.L.n0 && y0 + oy0 + j * RS < A.L.n1) A.out[gb + (long long)j * RS * A.L.sx] = r;
+    }
+}
+
 // ------------------------------------------------ 2D, whole step fused --
 // All four RK4 stages of a 2D step in one launch: the block loads c_n on its
 // tile plus an 8-cell ring (HBM -> LDS once) and evaluates, in LDS, the eight
@@ -1667,15 +1762,17 @@ __device__ __forceinline__ void step2d_levels(const PairArgs<T>& A, const T* SC,
 // SB = stage states 2..4, SACC = accumulator. A level reads the other buffer
 // and is separated by a barrier from the level that last read the buffer it
 // writes.
-template <typename T, int TX, int TY, int NT, bool EXACT>
+// EULER: four explicit Euler steps of a periodic field instead (see
+// euler2d_levels; c_n and mu only, SB and SACC unused).
+template <typename T, int TX, int TY, int NT, bool EXACT, bool EULER = false>
 __global__ __launch_bounds__(NT) void k_ch_step_2d(PairArgs<T> A) {
     constexpr int WC = TX + 16, NC = WC * (TY + 16);
     // every level in the same frame as c_n (see step2d_levels); 16-byte
     // aligned for the levels' vector accesses
     __shared__ __attribute__((aligned(16))) T SC[NC];
     __shared__ __attribute__((aligned(16))) T SA[NC];
-    __shared__ __attribute__((aligned(16))) T SB[NC];
-    __shared__ __attribute__((aligned(16))) T SACC[NC];
+    __shared__ __attribute__((aligned(16))) T SB[EULER ? 4 : NC];
+    __shared__ __attribute__((aligned(16))) T SACC[EULER ? 4 : NC];
     __shared__ T BCV[6];
     const int tid = threadIdx.x;
     stage_bc_values(BCV, A);
@@ -1746,10 +1843,14 @@ __global__ __launch_bounds__(NT) void k_ch_step_2d(PairArgs<T> A) {
     // per-position ghost rules; every other block (all of them on a periodic
     // grid: the wrapped load already holds the images) runs the same levels
     // with those rules compiled out.
-    const bool per = A.bx.klo == PHX_BC_PERIODIC && A.bx.khi == PHX_BC_PERIODIC &&
-                     A.by.klo == PHX_BC_PERIODIC && A.by.khi == PHX_BC_PERIODIC;
-    if (xy_in || per) step2d_levels<T, TX, TY, NT, EXACT, false>(A, SC, SA, SB, SACC, BCV, x0, y0, zb);
-    else step2d_levels<T, TX, TY, NT, EXACT, true>(A, SC, SA, SB, SACC, BCV, x0, y0, zb);
+    if constexpr (EULER) {
+        euler2d_levels<T, TX, TY, NT>(A, SC, SA, x0, y0, zb);   // periodic grids only (the host checks)
+    } else {
+        const bool per = A.bx.klo == PHX_BC_PERIODIC && A.bx.khi == PHX_BC_PERIODIC &&
+                         A.by.klo == PHX_BC_PERIODIC && A.by.khi == PHX_BC_PERIODIC;
+        if (xy_in || per) step2d_levels<T, TX, TY, NT, EXACT, false>(A, SC, SA, SB, SACC, BCV, x0, y0, zb);
+        else step2d_levels<T, TX, TY, NT, EXACT, true>(A, SC, SA, SB, SACC, BCV, x0, y0, zb);
+    }
 }
 
 // ---------------------------------------------------------- Euler pieces --
@@ -1909,6 +2010,14 @@ static int g_fp64_exact = 0;
 // between c and mu (free until the last step); every step but the last runs
 // this way, the last one with the two kernels on c, which leaves c's ghost
 // layer and mu as the two-kernel path does.
+static int g_euler4_tile = 0;   // four-step Euler tile (phx_ch_euler_set_quad); -1: off
+static bool euler4_ok(const phx_grid* G);
+template <typename T>
+static void euler4_run(const phx_grid* G, T*& X, T*& Y, const phx_ch_params* p, int count, hipStream_t st);
+
+// Blocks of four of those steps run as one launch of the whole-step
+// kernel's Euler form when the grid allows it (2D periodic, n >= 8; same
+// bits), the rest one launch per step.
 template <typename T>
 static int euler_fused_steps_t(const phx_grid* G, T* c, T* mu, const phx_ch_params* p, int nsteps, hipStream_t st) {
     CHArgs<T> A;
@@ -1927,14 +2036,17 @@ static int euler_fused_steps_t(const phx_grid* G, T* c, T* mu, const phx_ch_para
     const dim3 grid(cdiv(A.L.n0, 32), G->dim >= 2 ? cdiv(A.L.n1, 8) : 1, 1);
     A.xcd_remap = g_xcd_remap && (grid.x * grid.y) % 8 == 0;
     const int fused = nsteps - 1;
+    const int quads = g_euler4_tile >= 0 && euler4_ok(G) ? fused / 4 : 0;
+    const int singles = fused - 4 * quads;
     T* X = c;
     T* Y = mu;
-    if (fused % 2) {   // odd count: start from a copy in mu so the state ends in c
+    if ((quads + singles) % 2) {   // odd launch count: start from a copy in mu so the state ends in c
         PHX_HIP(hipMemcpyAsync(mu, c, (size_t)A.L.total() * sizeof(T), hipMemcpyDeviceToDevice, st));
         X = mu;
         Y = c;
     }
-    for (int s = 0; s < fused; ++s) {
+    if (quads) euler4_run<T>(G, X, Y, p, quads, st);
+    for (int s = 0; s < singles; ++s) {
         A.u = X;
         A.c = X;
         A.acc = Y;
@@ -2097,6 +2209,57 @@ static int step2d_tile(int stile, size_t esz) { return stile == 6 && esz == 8 ? 
 
 // The 2D whole-step kernel needs depth-8 BC images inside the domain.
 static bool step2d_ok(const phx_grid* G) { return G->dim == 2 && G->n[0] >= 8 && G->n[1] >= 8; }
+
+// Four-step Euler launches (the whole-step kernel's EULER form): tiles
+// 0: 32x16/512 (fastest at 128^2), 1: 32x32/512, 2: 64x16/256, 3: 32x16/256;
+// -1 disables them.
+static const TileZ kTilesE4[] = {{32, 16, 512}, {32, 32, 512}, {64, 16, 256}, {32, 16, 256}};
+constexpr int kNumTilesE4 = sizeof(kTilesE4) / sizeof(kTilesE4[0]);
+
+template <typename T>
+static void launch_euler4(int tile, dim3 grid, const PairArgs<T>& P, hipStream_t st) {
+    switch (tile) {
+        case 1: hipLaunchKernelGGL((k_ch_step_2d<T, 32, 32, 512, true, true>), grid, dim3(512), 0, st, P); break;
+        case 2: hipLaunchKernelGGL((k_ch_step_2d<T, 64, 16, 256, true, true>), grid, dim3(256), 0, st, P); break;
+        case 3: hipLaunchKernelGGL((k_ch_step_2d<T, 32, 16, 256, true, true>), grid, dim3(256), 0, st, P); break;
+        default: hipLaunchKernelGGL((k_ch_step_2d<T, 32, 16, 512, true, true>), grid, dim3(512), 0, st, P); break;
+    }
+}
+
+// The four-step Euler form: 2D, periodic on both axes, depth-8 images.
+static bool euler4_ok(const phx_grid* G) {
+    return step2d_ok(G) && G->bc[0][0] == PHX_BC_PERIODIC && G->bc[0][1] == PHX_BC_PERIODIC &&
+           G->bc[1][0] == PHX_BC_PERIODIC && G->bc[1][1] == PHX_BC_PERIODIC;
+}
+
+// `count` launches of four Euler steps each, X -> Y, swapping after each.
+template <typename T>
+static void euler4_run(const phx_grid* G, T*& X, T*& Y, const phx_ch_params* p, int count, hipStream_t st) {
+    PairArgs<T> P;
+    P.L = make_layout(G);
+    P.bx = axis_bc<T>(G, 0);
+    P.by = axis_bc<T>(G, 1);
+    P.bz = axis_bc<T>(G, 2);
+    fill_common<T>(G, p, P.idx2, P.idy2, P.idz2, P.two_rho, P.ca, P.cb, P.nkappa, P.M);
+    P.F = fast_coef<T>(G, p);
+    P.a1 = (T)p->dt;
+    P.a2 = (T)0;
+    P.dt6 = P.dt3 = P.third = (T)0;
+    P.c = nullptr;
+    P.acc = nullptr;
+    P.zc = 1;
+    P.bstride = P.L.total();
+    const int tile = std::min(g_euler4_tile, kNumTilesE4 - 1);
+    const TileZ tl = kTilesE4[tile];
+    const dim3 grid(cdiv(P.L.n0, tl.tx), cdiv(P.L.n1, tl.ty), 1);
+    P.xcd_remap = g_xcd_remap && (grid.x * grid.y) % 8 == 0;
+    for (int i = 0; i < count; ++i) {
+        P.u = X;
+        P.out = Y;
+        launch_euler4<T>(tile, grid, P, st);
+        std::swap(X, Y);
+    }
+}
 
 // Optional per-launch timing for benchmarks: while enabled, every launch of
 // the fused RK4 kernels is bracketed by HIP events on its stream, tagged with
@@ -2476,6 +2639,12 @@ extern "C" {
 
 int phx_ch_euler_set_fused(int on) {
     g_euler_fused = on ? 1 : 0;
+    return PHX_OK;
+}
+
+int phx_ch_euler_set_quad(int tile) {
+    if (tile < -1 || tile >= kNumTilesE4) return fail(PHX_ERR_INVALID, "phx_ch_euler_set_quad: tile must be -1..3");
+    g_euler4_tile = tile;
     return PHX_OK;
 }
 

@@ -230,6 +230,40 @@ def torch_zeros_like(t):
     return torch.zeros_like(t)
 
 
+@pytest.mark.parametrize("dtype", [capi.F64, capi.F32])
+@pytest.mark.parametrize("n", [(128, 128), (72, 40), (8, 8), (9, 21)])
+def test_euler_four_steps_per_launch_equals_two_kernels(dev, dtype, n):
+    """2D periodic Euler in blocks of four steps per launch (every tile of
+    the whole-step kernel's Euler form, step counts that leave 0-3 single
+    steps and odd or even launch counts) returns the two-kernel path's bits
+    for c and mu, ghost layers included."""
+    g = capi.make_grid(n, bc=capi.PERIODIC)
+    p = ch_params(rho=1.0, ca=0.0, cb=1.0, kappa=0.5, M=1.0, dt=0.01)
+    c0 = init.to_stored(init.ch_field(n, 11, 0.5, 0.05), g)
+    if dtype == capi.F32:
+        c0 = c0.astype(np.float32)
+
+    def run(nsteps):
+        tc = to_dev(c0, dev)
+        tm = torch_zeros_like(tc)
+        capi.call("phx_ch_euler_steps", C.byref(g), dtype, capi.ptr(tc), capi.ptr(tm), C.byref(p), nsteps, stream())
+        return to_np(tc), to_np(tm)
+
+    try:
+        for nsteps in (5, 6, 8, 9, 14, 21):
+            capi.call("phx_ch_euler_set_fused", 0)
+            ref = run(nsteps)
+            capi.call("phx_ch_euler_set_fused", 1)
+            for tile in (-1, 0, 1, 2, 3):
+                capi.call("phx_ch_euler_set_quad", tile)
+                got = run(nsteps)
+                assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), (nsteps, tile)
+    finally:
+        capi.call("phx_ch_euler_set_fused", 1)
+        capi.call("phx_ch_euler_set_quad", 0)
+    assert capi.lib().phx_ch_euler_set_quad(4) == capi.PHX_ERR_INVALID
+
+
 @pytest.mark.parametrize("n,bc,dtype", [((256, 256), capi.PERIODIC, capi.F32), ((37, 50), capi.NOFLUX, capi.F64),
                                         ((20, 12, 9), capi.PERIODIC, capi.F64)])
 def test_mass_energy_batch_equals_single(dev, n, bc, dtype):
