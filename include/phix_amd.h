@@ -162,8 +162,8 @@ int phx_ch_euler_steps(const phx_grid* grid, int dtype, void* c, void* mu,
  * bit; mu serves as the second state buffer until the last step. on = 0
  * keeps two launches per step (process-wide). Returns PHX_OK. */
 int phx_ch_euler_set_fused(int on);
-/* No reference counterpart. On 2D grids periodic on both axes with n >= 8,
- * the one-launch steps run in blocks of four per launch (the 2D whole-step
+/* No reference counterpart. On 2D grids with n >= 8 on both axes (any BC
+ * kinds), the one-launch steps run in blocks of four per launch (the 2D whole-step
  * kernel's Euler form: c on the tile and an 8-cell ring in LDS, mu and the
  * state updated four times there; the same bits). tile: 0 (default) 32x16
  * cells per 512-thread workgroup, 1: 32x32/512, 2: 64x16/256, 3: 32x16/256;

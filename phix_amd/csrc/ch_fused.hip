@@ -1651,23 +1651,26 @@ __device__ __forceinline__ void step2d_levels(const PairArgs<T>& A, const T* SC,
     }
 }
 
-// Four explicit Euler steps of a periodic 2D field in the whole-step frame:
-// c in SC (ring 8), the mu levels in SA. Step s evaluates mu on ring 7-2s
-// from the state on ring 8-2s, then the state c + dt*k on ring 6-2s in
-// place (a run reads its own centre and SA's neighbours only); the last
-// update writes the tile cells to global memory. Runs and frame as in
-// step2d_levels; the operations are the one-stage kernel's Euler step in
-// the reference's order (mu_at, k_at, c + dt*k, exact form), so one launch
-// gives the bits of four one-stage Euler launches.
-template <typename T, int TX, int TY, int NT>
-__device__ __forceinline__ void euler2d_levels(const PairArgs<T>& A, T* SC, T* SA, int x0, int y0, long long zb) {
+// Four explicit Euler steps of a 2D field in the whole-step frame: c in SC
+// (ring 8), the mu levels in SA. Step s evaluates mu on ring 7-2s from the
+// state on ring 8-2s, then the state c + dt*k on ring 6-2s in place (a run
+// reads its own centre and SA's neighbours only); the last update writes the
+// tile cells to global memory. Runs, frame and ghost rules (GEN: NOFLUX
+// images by exchanged neighbours, FIXED constants) as in step2d_levels; the
+// operations are the one-stage kernel's Euler step in the reference's order
+// (mu_at, k_at, c + dt*k, exact form), so one launch gives the bits of four
+// one-stage Euler launches.
+template <typename T, int TX, int TY, int NT, bool GEN>
+__device__ __forceinline__ void euler2d_levels(const PairArgs<T>& A, T* SC, T* SA, const T* BCV, int x0, int y0,
+                                               long long zb) {
     constexpr int P = TX + 16;
     constexpr int V = 16 / (int)sizeof(T);
     typedef T VT __attribute__((ext_vector_type(V)));
     static_assert(P % V == 0, "frame rows hold whole runs");
     static_assert(NT % TX == 0, "tile rows must divide the workgroup");
     const int tid = threadIdx.x;
-    auto operands = [&](const T* src, int f, T (&c)[V], T (&xp)[V], T (&xm)[V], T (&yp)[V], T (&ym)[V]) {
+    auto operands = [&](const T* src, int f, T (&c)[V], T (&xp)[V], T (&xm)[V], T (&yp)[V], T (&ym)[V],
+                        int (&code)[V]) {
         const VT cv = *reinterpret_cast<const VT*>(&src[f]);
         const VT uv = *reinterpret_cast<const VT*>(&src[f + P]);
         const VT dv = *reinterpret_cast<const VT*>(&src[f - P]);
@@ -1679,6 +1682,11 @@ __device__ __forceinline__ void euler2d_levels(const PairArgs<T>& A, T* SC, T* S
             xm[q] = q > 0 ? cv[q - 1] : l;
             yp[q] = uv[q];
             ym[q] = dv[q];
+            code[q] = GEN ? ghost_code_xy(x0 - 8 + (f + q) % P, y0 - 8 + (f + q) / P, A) : 0;
+            if (GEN) {
+                if (code[q] & GC_SWX) { const T t = xp[q]; xp[q] = xm[q]; xm[q] = t; }
+                if (code[q] & GC_SWY) { const T t = yp[q]; yp[q] = ym[q]; ym[q] = t; }
+            }
         }
     };
     auto eval_mu = [&](auto rr) {
@@ -1690,10 +1698,15 @@ __device__ __forceinline__ void euler2d_levels(const PairArgs<T>& A, T* SC, T* S
             const int q = tid + NT * i;
             const int f = F0 + V * (NQ % NT != 0 && i == NIT - 1 ? min(q, NQ - 1) : q);
             T c[V], xp[V], xm[V], yp[V], ym[V];
-            operands(SC, f, c, xp, xm, yp, ym);
+            int code[V];
+            operands(SC, f, c, xp, xm, yp, ym, code);
             VT o;
 #pragma unroll
-            for (int k = 0; k < V; ++k) o[k] = mu_at<T, true, 2>(A, c[k], xp[k], xm[k], yp[k], ym[k], (T)0, (T)0);
+            for (int k = 0; k < V; ++k) {
+                T v = mu_at<T, true, 2>(A, c[k], xp[k], xm[k], yp[k], ym[k], (T)0, (T)0);
+                if (GEN && (code[k] & GC_FIX)) v = ghost_value_xy(code[k], BCV);
+                o[k] = v;
+            }
             if (NQ % NT == 0 || i < NIT - 1 || q < NQ) *reinterpret_cast<VT*>(&SA[f]) = o;
         }
     };
@@ -1706,12 +1719,16 @@ __device__ __forceinline__ void euler2d_levels(const PairArgs<T>& A, T* SC, T* S
             const int q = tid + NT * i;
             const int f = F0 + V * (NQ % NT != 0 && i == NIT - 1 ? min(q, NQ - 1) : q);
             T m[V], xp[V], xm[V], yp[V], ym[V];
-            operands(SA, f, m, xp, xm, yp, ym);
+            int code[V];
+            operands(SA, f, m, xp, xm, yp, ym, code);
             const VT cc = *reinterpret_cast<const VT*>(&SC[f]);
             VT vo;
 #pragma unroll
-            for (int k = 0; k < V; ++k)
-                vo[k] = cc[k] + A.a1 * k_at<T, true, 2>(A, m[k], xp[k], xm[k], yp[k], ym[k], (T)0, (T)0);
+            for (int k = 0; k < V; ++k) {
+                T v = cc[k] + A.a1 * k_at<T, true, 2>(A, m[k], xp[k], xm[k], yp[k], ym[k], (T)0, (T)0);
+                if (GEN && (code[k] & GC_FIX)) v = ghost_value_xy(code[k], BCV);
+                vo[k] = v;
+            }
             if (NQ % NT == 0 || i < NIT - 1 || q < NQ) *reinterpret_cast<VT*>(&SC[f]) = vo;
         }
     };
@@ -1762,8 +1779,8 @@ __device__ __forceinline__ void euler2d_levels(const PairArgs<T>& A, T* SC, T* S
 // SB = stage states 2..4, SACC = accumulator. A level reads the other buffer
 // and is separated by a barrier from the level that last read the buffer it
 // writes.
-// EULER: four explicit Euler steps of a periodic field instead (see
-// euler2d_levels; c_n and mu only, SB and SACC unused).
+// EULER: four explicit Euler steps instead (see euler2d_levels; c_n and mu
+// only, SB and SACC unused).
 template <typename T, int TX, int TY, int NT, bool EXACT, bool EULER = false>
 __global__ __launch_bounds__(NT) void k_ch_step_2d(PairArgs<T> A) {
     constexpr int WC = TX + 16, NC = WC * (TY + 16);
@@ -1843,11 +1860,12 @@ __global__ __launch_bounds__(NT) void k_ch_step_2d(PairArgs<T> A) {
     // per-position ghost rules; every other block (all of them on a periodic
     // grid: the wrapped load already holds the images) runs the same levels
     // with those rules compiled out.
+    const bool per = A.bx.klo == PHX_BC_PERIODIC && A.bx.khi == PHX_BC_PERIODIC &&
+                     A.by.klo == PHX_BC_PERIODIC && A.by.khi == PHX_BC_PERIODIC;
     if constexpr (EULER) {
-        euler2d_levels<T, TX, TY, NT>(A, SC, SA, x0, y0, zb);   // periodic grids only (the host checks)
+        if (xy_in || per) euler2d_levels<T, TX, TY, NT, false>(A, SC, SA, BCV, x0, y0, zb);
+        else euler2d_levels<T, TX, TY, NT, true>(A, SC, SA, BCV, x0, y0, zb);
     } else {
-        const bool per = A.bx.klo == PHX_BC_PERIODIC && A.bx.khi == PHX_BC_PERIODIC &&
-                         A.by.klo == PHX_BC_PERIODIC && A.by.khi == PHX_BC_PERIODIC;
         if (xy_in || per) step2d_levels<T, TX, TY, NT, EXACT, false>(A, SC, SA, SB, SACC, BCV, x0, y0, zb);
         else step2d_levels<T, TX, TY, NT, EXACT, true>(A, SC, SA, SB, SACC, BCV, x0, y0, zb);
     }
@@ -2016,8 +2034,8 @@ template <typename T>
 static void euler4_run(const phx_grid* G, T*& X, T*& Y, const phx_ch_params* p, int count, hipStream_t st);
 
 // Blocks of four of those steps run as one launch of the whole-step
-// kernel's Euler form when the grid allows it (2D periodic, n >= 8; same
-// bits), the rest one launch per step.
+// kernel's Euler form when the grid allows it (2D, n >= 8; same bits), the
+// rest one launch per step.
 template <typename T>
 static int euler_fused_steps_t(const phx_grid* G, T* c, T* mu, const phx_ch_params* p, int nsteps, hipStream_t st) {
     CHArgs<T> A;
@@ -2226,11 +2244,8 @@ static void launch_euler4(int tile, dim3 grid, const PairArgs<T>& P, hipStream_t
     }
 }
 
-// The four-step Euler form: 2D, periodic on both axes, depth-8 images.
-static bool euler4_ok(const phx_grid* G) {
-    return step2d_ok(G) && G->bc[0][0] == PHX_BC_PERIODIC && G->bc[0][1] == PHX_BC_PERIODIC &&
-           G->bc[1][0] == PHX_BC_PERIODIC && G->bc[1][1] == PHX_BC_PERIODIC;
-}
+// The four-step Euler form: 2D with depth-8 BC images inside the domain.
+static bool euler4_ok(const phx_grid* G) { return step2d_ok(G); }
 
 // `count` launches of four Euler steps each, X -> Y, swapping after each.
 template <typename T>

@@ -231,13 +231,22 @@ def torch_zeros_like(t):
 
 
 @pytest.mark.parametrize("dtype", [capi.F64, capi.F32])
-@pytest.mark.parametrize("n", [(128, 128), (72, 40), (8, 8), (9, 21)])
-def test_euler_four_steps_per_launch_equals_two_kernels(dev, dtype, n):
-    """2D periodic Euler in blocks of four steps per launch (every tile of
-    the whole-step kernel's Euler form, step counts that leave 0-3 single
-    steps and odd or even launch counts) returns the two-kernel path's bits
-    for c and mu, ghost layers included."""
-    g = capi.make_grid(n, bc=capi.PERIODIC)
+@pytest.mark.parametrize("n,bc,vals", [
+    ((128, 128), P, 0.0),
+    ((72, 40), P, 0.0),
+    ((8, 8), P, 0.0),
+    ((9, 21), P, 0.0),
+    ((70, 33), N, 0.0),
+    ((41, 30), [[F, N], [P, P]], [[0.3, 0], [0, 0]]),
+    ((30, 41), [[P, P], [N, F]], [[0, 0], [0, 0.6]]),
+    ((8, 12), [[F, F], [N, N]], [[0.45, 0.55], [0, 0]]),
+])
+def test_euler_four_steps_per_launch_equals_two_kernels(dev, dtype, n, bc, vals):
+    """2D Euler in blocks of four steps per launch (every tile of the
+    whole-step kernel's Euler form, every BC kind, step counts that leave 0-3
+    single steps and odd or even launch counts) returns the two-kernel
+    path's bits for c and mu, ghost layers included."""
+    g = capi.make_grid(n, bc=bc, bc_value=vals)
     p = ch_params(rho=1.0, ca=0.0, cb=1.0, kappa=0.5, M=1.0, dt=0.01)
     c0 = init.to_stored(init.ch_field(n, 11, 0.5, 0.05), g)
     if dtype == capi.F32:
