@@ -641,11 +641,13 @@ WEIGHTS_RECIPE = (
     "python -m phix_amd.surrogate train-rollout --init stage1.npz --ntraj 256 --nsnap 200 --epochs 40 "
     "--batch 16 --lr 3e-4 --seconds 900 --noise 1e-3 --pushforward 0.5 --out phix_amd/data/surrogate_ch256.npz",
 )
-# ... and the late-phase weights from them (~5 GPU-minutes more):
+# ... and the late-phase weights from them (~21 GPU-minutes more): fine-tuned
+# on 256 trajectories, then on 2000 (the training set's specified count)
 WEIGHTS_LATE_RECIPE = (
     "python -m phix_amd.surrogate train-rollout --init phix_amd/data/surrogate_ch256.npz --ntraj 256 --nsnap 190 "
-    "--skip 10 --epochs 60 --batch 16 --lr 2e-4 --seconds 840 --noise 1e-3 --pushforward 0.5 "
-    "--out phix_amd/data/surrogate_ch256_late.npz",
+    "--skip 10 --epochs 60 --batch 16 --lr 2e-4 --seconds 840 --noise 1e-3 --pushforward 0.5 --out late256.npz",
+    "python -m phix_amd.surrogate train-rollout --init late256.npz --ntraj 2000 --nsnap 190 --skip 10 --epochs 20 "
+    "--batch 16 --lr 2e-4 --seconds 960 --noise 1e-3 --pushforward 0.5 --out phix_amd/data/surrogate_ch256_late.npz",
 )
 
 
