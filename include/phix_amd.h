@@ -425,6 +425,17 @@ int    phx_sur_forward(const float* params, void* ws, const float* c_in, float* 
  * PHX_SUR_REUSE_WEIGHTS as above. c_out may be c_in. */
 int    phx_sur_rollout(const float* params, void* ws, const float* c_in, float* c_out,
                        int B, int H, int W, int jumps, int flags, phx_stream stream);
+/* As phx_sur_rollout, keeping every `every`-th state (no reference
+ * counterpart; the surrogate sweeps of a parallel-in-time solver):
+ * states[k] (B*H*W floats each, k = 0 .. jumps/every - 1) <- the state after
+ * every*(k+1) jumps, projected to the mean and, when corr is not NULL, plus
+ * corr[k] (FP32 add); the jumps after a kept state continue from it. The
+ * same bits as jumps/every phx_sur_rollout calls of `every` jumps, each
+ * followed by the addition of corr[k]. Needs every >= 1 dividing jumps;
+ * states must not overlap c_in or corr. */
+int    phx_sur_rollout_states(const float* params, void* ws, const float* c_in, float* states,
+                              const float* corr, int B, int H, int W, int jumps, int every,
+                              int flags, phx_stream stream);
 /* Loss = mean((c_out - c_target)^2); grad (param layout) <- dLoss/dparams;
  * loss_dev (device double, may be NULL) <- sum of squared errors (divide by
  * B*H*W for the loss). Needs the preceding phx_sur_forward on the same ws. */

@@ -169,6 +169,8 @@ SIG = {
     "phx_sur_workspace_bytes": (C.c_size_t, [C.c_int, C.c_int, C.c_int]),
     "phx_sur_forward": (C.c_int, [VP, VP, VP, VP, C.c_int, C.c_int, C.c_int, C.c_int, VP]),
     "phx_sur_rollout": (C.c_int, [VP, VP, VP, VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, VP]),
+    "phx_sur_rollout_states": (C.c_int, [VP, VP, VP, VP, VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                         VP]),
     "phx_sur_backward": (C.c_int, [VP, VP, VP, VP, VP, VP, VP, C.c_int, C.c_int, C.c_int, VP]),
     "phx_sur_adam": (C.c_int, [VP, VP, VP, VP, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, VP]),
     "phx_conv3x3_ws_bytes": (C.c_size_t, []),

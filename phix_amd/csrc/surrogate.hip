@@ -570,7 +570,8 @@ __device__ __forceinline__ void tree_add(double& a, double& ca, double b, double
 // kernel's bits), and writes u on its own tile to cproj.
 __global__ __launch_bounds__(FNT) void k_sur_fused_fwd(const float* __restrict__ c, const float* __restrict__ dprev,
                                                        const double* __restrict__ tprev, int ntile,
-                                                       float* __restrict__ cproj, const float* __restrict__ P,
+                                                       float* __restrict__ cproj, const float* __restrict__ cadd,
+                                                       const float* __restrict__ P,
                                                        const __bf16* __restrict__ wpk,
                                                        const __bf16* __restrict__ frag, float* __restrict__ d,
                                                        double* __restrict__ tpart, int H, int W) {
@@ -586,7 +587,7 @@ __global__ __launch_bounds__(FNT) void k_sur_fused_fwd(const float* __restrict__
     const int x0 = blockIdx.x * FT, y0 = blockIdx.y * FT;
     const long long img = (long long)blockIdx.z * H * W;
     // ---- prologue: global loads, then LDS stores
-    float cin[2], dv0[2];
+    float cin[2], dv0[2], av[2];
     long long at[2];
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
@@ -595,6 +596,7 @@ __global__ __launch_bounds__(FNT) void k_sur_fused_fwd(const float* __restrict__
         at[k] = img + (long long)yy * W + xx;
         cin[k] = c[at[k]];
         dv0[k] = dprev ? dprev[at[k]] : 0.f;
+        av[k] = cadd ? cadd[at[k]] : 0.f;
     }
     if (dprev) {
         // the previous jump's mean of d: wave 0 holds k_sur_project_tiles's
@@ -629,6 +631,7 @@ __global__ __launch_bounds__(FNT) void k_sur_fused_fwd(const float* __restrict__
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             cin[k] = (float)((double)cin[k] + ((double)dv0[k] - mean));
+            if (cadd) cin[k] = cin[k] + av[k];   // a correction of the projected state (FP32 add)
             const int t = tid + k * FNT, ty = t / FR0 - 6, tx = t % FR0 - 6;
             if (t < NX0 && ty >= 0 && ty < FT && tx >= 0 && tx < FT) cproj[at[k]] = cin[k];
         }
@@ -823,7 +826,8 @@ __global__ __launch_bounds__(FNT) void k_sur_fused_fwd(const float* __restrict__
 // block alike).
 __global__ __launch_bounds__(256) void k_sur_project_tiles(const float* __restrict__ c, const float* __restrict__ d,
                                                            const double* __restrict__ tpart, int ntile, int HW,
-                                                           long long n, float* __restrict__ out) {
+                                                           long long n, float* __restrict__ out,
+                                                           const float* __restrict__ cadd = nullptr) {
     __shared__ double S[256], Cc[256];
     const long long p0 = blockIdx.x * 256LL;
     const int smp = (int)(p0 / HW);   // HW is a multiple of 256: one sample per block
@@ -847,7 +851,11 @@ __global__ __launch_bounds__(256) void k_sur_project_tiles(const float* __restri
     }
     const double mean = (S[0] + Cc[0]) / HW;
     const long long p = p0 + threadIdx.x;
-    if (p < n) out[p] = (float)((double)c[p] + ((double)d[p] - mean));
+    if (p < n) {
+        float v = (float)((double)c[p] + ((double)d[p] - mean));
+        if (cadd) v = v + cadd[p];   // a correction of the projected state (FP32 add)
+        out[p] = v;
+    }
 }
 
 // --------------------------------------------------------------- host ----
@@ -921,7 +929,8 @@ static int sur_forward(const float* P, SurWS& S, const float* c, float* out, int
     if (flags & PHX_SUR_FUSED) {
         const int ntile = (H / FT) * (W / FT);
         hipLaunchKernelGGL(k_sur_fused_fwd, dim3(W / FT, H / FT, B), dim3(FNT), 0, st, c, (const float*)nullptr,
-                           (const double*)nullptr, 0, (float*)nullptr, P, S.wfwd, S.frag, S.d, S.tpart, H, W);
+                           (const double*)nullptr, 0, (float*)nullptr, (const float*)nullptr, P, S.wfwd, S.frag, S.d,
+                           S.tpart, H, W);
         hipLaunchKernelGGL(k_sur_project_tiles, dim3(nb), dim3(256), 0, st, c, S.d, S.tpart, ntile, H * W, n, out);
         PHX_LAUNCH_CHECK("phx_sur_forward");
         return PHX_OK;
@@ -943,7 +952,8 @@ static int sur_forward(const float* P, SurWS& S, const float* c, float* out, int
 // k_sur_project_tiles: the same bits as `jumps` calls of sur_forward with
 // PHX_SUR_FUSED, with one launch per jump instead of two.
 static int sur_rollout(const float* P, SurWS& S, const float* c, float* out, int B, int H, int W, int jumps,
-                       int flags, hipStream_t st) {
+                       int flags, hipStream_t st, float* states = nullptr, int every = 0,
+                       const float* corr = nullptr) {
     const long long n = (long long)B * H * W;
     const int nb = (int)((n + 255) / 256), ntile = (H / FT) * (W / FT);
     if (!(flags & PHX_SUR_REUSE_WEIGHTS)) {
@@ -956,13 +966,26 @@ static int sur_rollout(const float* P, SurWS& S, const float* c, float* out, int
     for (int j = 0; j < jumps; ++j) {
         const float* dp = j ? dbuf[(j - 1) & 1] : nullptr;
         const double* tp = j ? tbuf[(j - 1) & 1] : nullptr;
-        float* up = j ? S.u[j & 1] : nullptr;
-        hipLaunchKernelGGL(k_sur_fused_fwd, dim3(W / FT, H / FT, B), dim3(FNT), 0, st, base, dp, tp, ntile, up, P,
+        // the projected (and corrected) state after j jumps: a slot of
+        // `states` at every `every` jumps, else a workspace buffer
+        float* up = nullptr;
+        const float* ad = nullptr;
+        if (j) {
+            if (states && j % every == 0) {
+                up = states + (long long)(j / every - 1) * n;
+                if (corr) ad = corr + (long long)(j / every - 1) * n;
+            } else {
+                up = S.u[j & 1];
+            }
+        }
+        hipLaunchKernelGGL(k_sur_fused_fwd, dim3(W / FT, H / FT, B), dim3(FNT), 0, st, base, dp, tp, ntile, up, ad, P,
                            S.wfwd, S.frag, dbuf[j & 1], tbuf[j & 1], H, W);
         if (j) base = up;
     }
+    float* dst = states ? states + (long long)(jumps / every - 1) * n : out;
+    const float* ad = states && corr ? corr + (long long)(jumps / every - 1) * n : nullptr;
     hipLaunchKernelGGL(k_sur_project_tiles, dim3(nb), dim3(256), 0, st, base, dbuf[(jumps - 1) & 1],
-                       tbuf[(jumps - 1) & 1], ntile, H * W, n, out);
+                       tbuf[(jumps - 1) & 1], ntile, H * W, n, dst, ad);
     PHX_LAUNCH_CHECK("phx_sur_rollout");
     return PHX_OK;
 }
@@ -999,6 +1022,18 @@ int phx_sur_rollout(const float* params, void* ws, const float* c_in, float* c_o
     SurWS S;
     ws_layout(B, H, W, &S, (char*)ws);
     return sur_rollout(params, S, c_in, c_out, B, H, W, jumps, flags, as_stream(s));
+}
+
+int phx_sur_rollout_states(const float* params, void* ws, const float* c_in, float* states, const float* corr, int B,
+                           int H, int W, int jumps, int every, int flags, phx_stream s) {
+    int rc = check_shape(B, H, W, "phx_sur_rollout_states");
+    if (rc) return rc;
+    if (!params || !ws || !c_in || !states) return fail(PHX_ERR_INVALID, "phx_sur_rollout_states: NULL argument");
+    if (jumps < 1 || every < 1 || jumps % every)
+        return fail(PHX_ERR_INVALID, "phx_sur_rollout_states: need jumps >= 1 and every >= 1 dividing jumps");
+    SurWS S;
+    ws_layout(B, H, W, &S, (char*)ws);
+    return sur_rollout(params, S, c_in, nullptr, B, H, W, jumps, flags, as_stream(s), states, every, corr);
 }
 
 int phx_sur_backward(const float* params, void* ws, const float* c_in, const float* c_out, const float* c_target,

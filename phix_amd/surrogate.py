@@ -552,6 +552,16 @@ class PararealSolver:
         capi.call("phx_sur_rollout", capi.ptr(net.p), capi.ptr(net.ws), capi.ptr(u), capi.ptr(out), 1, self.n,
                   self.n, self.jumps, 1, capi.stream_handle())
 
+    def _sweep(self, corr=None):
+        """U[s+1] <- G(U[s]) (+ corr[s]) for s = 0..S-1 in order, as one
+        phx_sur_rollout_states call over all S * jumps jumps (each slice
+        boundary's state kept, corrected and continued from as it is loaded;
+        the bits of S phx_sur_rollout calls each followed by the addition)."""
+        net = self.net
+        capi.call("phx_sur_rollout_states", capi.ptr(net.p), capi.ptr(net.ws), capi.ptr(self.U[0]),
+                  capi.ptr(self.U[1]), capi.ptr(corr) if corr is not None else None, 1, self.n, self.n,
+                  self.S * self.jumps, self.jumps, 1, capi.stream_handle())
+
     def _F_all(self):
         """F applied to U[0..S-1] at once; returns the (S, n, n) view of the
         solver arena's physical cells holding the results. The first call
@@ -605,18 +615,20 @@ class PararealSolver:
         else:
             U[0, 0].copy_(c0.reshape(n, n))
         # prediction: U[s+1] = G(U[s]), so G(U[s]) is U[s+1]
-        for s in range(S):
-            self._G(U[s], U[s + 1])
+        self._sweep()
         G.copy_(U[1:])
         incs = []
         ok = True
-        for _ in range(iters):
+        for it in range(iters):
             self.Fs[:, 0].copy_(self._F_all())
             torch.sub(self.Fs, G, out=D)
             last = U[S].clone()
-            for s in range(S):
-                self._G(U[s], G[s])
-                torch.add(G[s], D[s], out=U[s + 1])
+            if it == iters - 1:
+                self._sweep(D)   # the last sweep: G(U_new[s]) itself is not needed again
+            else:
+                for s in range(S):
+                    self._G(U[s], G[s])
+                    torch.add(G[s], D[s], out=U[s + 1])
             incs.append(((U[S] - last).norm() / U[S].norm()).item())
             es = self._energies(U[1:, 0])
             if any(b > a * (1 + 1e-6) for a, b in zip(es, es[1:])):

@@ -408,6 +408,43 @@ def test_rollout_equals_repeated_fused_forward(dev, B, H, W, jumps):
     assert torch.equal(inplace, ref)
 
 
+@pytest.mark.parametrize("B,H,W,jumps,every", [(1, 64, 48, 6, 2), (2, 32, 64, 6, 3), (1, 256, 256, 10, 5),
+                                               (1, 48, 48, 4, 4)])
+def test_rollout_states_equals_chained_rollouts(dev, B, H, W, jumps, every):
+    """phx_sur_rollout_states keeps the state after every `every` jumps,
+    each plus its correction, and continues from it: the bits of jumps/every
+    phx_sur_rollout calls each followed by a torch addition of the
+    correction; without corrections, the plain chained states."""
+    import torch
+    from phix_amd import surrogate as S
+    p = np.load(S.WEIGHTS_LATE)["params"]
+    tp = torch.from_numpy(np.ascontiguousarray(p, np.float32)).to(dev)
+    ws = torch.empty(capi.lib().phx_sur_workspace_bytes(B, H, W), dtype=torch.uint8, device=dev)
+    rng = np.random.default_rng(5 * B + H + W + jumps + every)
+    c = torch.from_numpy((0.5 + 0.45 * np.tanh(3 * rng.standard_normal((B, H, W)))).astype(np.float32)).to(dev)
+    K = jumps // every
+    corr = torch.from_numpy((1e-3 * rng.standard_normal((K, B, H, W))).astype(np.float32)).to(dev)
+    for use_corr in (False, True):
+        ref, x = [], c.clone()
+        for k in range(K):
+            y = torch.empty_like(c)
+            capi.call("phx_sur_rollout", capi.ptr(tp), capi.ptr(ws), capi.ptr(x), capi.ptr(y), B, H, W, every,
+                      1 if k or use_corr else 0, stream())
+            if use_corr:
+                y = y + corr[k]
+            ref.append(y)
+            x = y
+        got = torch.empty((K, B, H, W), dtype=torch.float32, device=dev)
+        capi.call("phx_sur_rollout_states", capi.ptr(tp), capi.ptr(ws), capi.ptr(c), capi.ptr(got),
+                  capi.ptr(corr) if use_corr else None, B, H, W, jumps, every, 1, stream())
+        torch.cuda.synchronize()
+        for k in range(K):
+            assert torch.equal(got[k], ref[k]), (use_corr, k, (got[k] - ref[k]).abs().max().item())
+    lib = capi.lib()
+    assert lib.phx_sur_rollout_states(capi.ptr(tp), capi.ptr(ws), capi.ptr(c), capi.ptr(got), None, B, H, W, jumps,
+                                      jumps + 1, 1, None) == capi.PHX_ERR_INVALID   # every must divide jumps
+
+
 @pytest.mark.timeout(300)
 def test_config5_surrogate_plus_solver_against_plain_solver(dev):
     """Config 5 end to end: 256^2 periodic CH to t_end = 200 (20,000 RK4
