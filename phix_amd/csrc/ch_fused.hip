@@ -2103,6 +2103,18 @@ static int euler_steps_t(const phx_grid* G, T* c, T* mu, const phx_ch_params* p,
 }
 
 int bc_fill_impl(const phx_grid* G, int dtype, void* data, hipStream_t st);
+int bc_fill_batch_impl(const phx_grid* G, int dtype, void* data, int nbatch, hipStream_t st);
+
+// dst <- src on the physical cells of nbatch 2D fields back to back (blockIdx.y)
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy_phys2d(T* __restrict__ dst, const T* __restrict__ src, Layout L,
+                                                     long long bstride) {
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long long)L.n0 * L.n1) return;
+    const int y = (int)(i / L.n0), x = (int)(i - (long long)y * L.n0);
+    const long long o = (long long)blockIdx.y * bstride + L.at(x, y, 0);
+    dst[o] = src[o];
+}
 
 // Tile shapes (autotuned later; these are the defaults).
 constexpr int T2X = 64, T2Y = 16;
@@ -2626,16 +2638,18 @@ static int rk4_step2d_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_par
     T* Y = even ? c : w1;
     if (even) PHX_HIP(hipMemcpyAsync(w1, c, (size_t)NB * sizeof(T), hipMemcpyDeviceToDevice, st));
     for (int s = 0; s < nsteps - 1; ++s) step(s % 2 == 0 ? X : Y, s % 2 == 0 ? Y : X);
-    for (int b = 0; b < nbatch; ++b) {
-        int rc = bc_fill_impl(G, sizeof(T) == 8 ? PHX_F64 : PHX_F32, c + b * N, st);
-        if (rc) return rc;
-    }
+    int rc = bc_fill_batch_impl(G, sizeof(T) == 8 ? PHX_F64 : PHX_F32, c, nbatch, st);
+    if (rc) return rc;
     step(c, w0);
-    const size_t pitch = (size_t)L.sx * sizeof(T);
-    const long long o0 = L.at(0, 0, 0);
-    for (int b = 0; b < nbatch; ++b)
-        PHX_HIP(hipMemcpy2DAsync(c + b * N + o0, pitch, w0 + b * N + o0, pitch, (size_t)L.n0 * sizeof(T), L.n1,
+    if (nbatch == 1) {
+        const size_t pitch = (size_t)L.sx * sizeof(T);
+        const long long o0 = L.at(0, 0, 0);
+        PHX_HIP(hipMemcpy2DAsync(c + o0, pitch, w0 + o0, pitch, (size_t)L.n0 * sizeof(T), L.n1,
                                  hipMemcpyDeviceToDevice, st));
+    } else {
+        const dim3 cg((unsigned)cdiv((long long)L.n0 * L.n1, 256), (unsigned)nbatch);
+        hipLaunchKernelGGL(k_copy_phys2d<T>, cg, dim3(256), 0, st, c, (const T*)w0, L, N);
+    }
     PHX_LAUNCH_CHECK("phx_ch_rk4_steps");
     return PHX_OK;
 }

@@ -132,10 +132,12 @@ struct FillGeom {
     int kind[3][2];
     double value[3][2];
     long long nx_slab, ny_slab, nz_slab;   // ghost cells in each group
+    long long bstride;                     // elements between the fields of a batch (blockIdx.y)
 };
 
 template <typename T>
 __global__ void k_bc_fill_all(T* __restrict__ data, FillGeom F) {
+    data += (long long)blockIdx.y * F.bstride;
     const Layout& L = F.L;
     const int g = L.g, g2 = 2 * g;
     long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -190,7 +192,7 @@ static bool fill_all_ok(const phx_grid* G) {
 }
 
 template <typename T>
-static int bc_fill_all_t(const phx_grid* G, T* data, hipStream_t st) {
+static int bc_fill_all_t(const phx_grid* G, T* data, hipStream_t st, int nbatch = 1) {
     FillGeom F{};
     F.L = make_layout(G);
     for (int a = 0; a < 3; ++a)
@@ -206,8 +208,25 @@ static int bc_fill_all_t(const phx_grid* G, T* data, hipStream_t st) {
     const long long total = F.nx_slab + F.ny_slab + F.nz_slab;
     const long long blocks = (total + 255) / 256;
     if (blocks > 0x7fffffffLL) return fail(PHX_ERR_UNSUPPORTED, "phx_bc_fill: field too large");
-    hipLaunchKernelGGL(k_bc_fill_all<T>, dim3((unsigned)blocks), dim3(256), 0, st, data, F);
+    F.bstride = L.total();
+    hipLaunchKernelGGL(k_bc_fill_all<T>, dim3((unsigned)blocks, (unsigned)nbatch), dim3(256), 0, st, data, F);
     PHX_LAUNCH_CHECK("phx_bc_fill");
+    return PHX_OK;
+}
+
+int bc_fill_impl(const phx_grid* G, int dtype, void* data, hipStream_t st);
+
+// The BC fill of nbatch fields of G's stored shape back to back: one launch
+// when the one-launch fill applies, else field by field.
+int bc_fill_batch_impl(const phx_grid* G, int dtype, void* data, int nbatch, hipStream_t st) {
+    if (fill_all_ok(G))
+        return dtype == PHX_F64 ? bc_fill_all_t(G, (double*)data, st, nbatch)
+                                : bc_fill_all_t(G, (float*)data, st, nbatch);
+    const size_t bytes = (size_t)make_layout(G).total() * (dtype == PHX_F64 ? 8 : 4);
+    for (int b = 0; b < nbatch; ++b) {
+        int rc = bc_fill_impl(G, dtype, (char*)data + b * bytes, st);
+        if (rc) return rc;
+    }
     return PHX_OK;
 }
 
