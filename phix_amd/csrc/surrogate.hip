@@ -1073,7 +1073,7 @@ int phx_sur_backward(const float* params, void* ws, const float* c_in, const flo
     hipLaunchKernelGGL(k_edge_wgrad<0>, dim3(WG_BLOCKS), dim3(320), 0, st, c_in, S.dz[cur], (const float*)nullptr,
                        S.part, B, H, W);
     hipLaunchKernelGGL(k_reduce_edge, dim3(2), dim3(256), 0, st, S.part, WG_BLOCKS, 320, grad + P_W1);
-    if (loss_dev) hipMemcpyAsync(loss_dev, S.loss, sizeof(double), hipMemcpyDeviceToDevice, st);
+    if (loss_dev) PHX_HIP(hipMemcpyAsync(loss_dev, S.loss, sizeof(double), hipMemcpyDeviceToDevice, st));
     PHX_LAUNCH_CHECK("phx_sur_backward");
     return PHX_OK;
 }
