@@ -66,6 +66,18 @@ def test_argument_errors_are_reported():
     assert capi.lib().phx_stored_size(C.byref(capi.make_grid((128, 128)))) == 130 * 130 * 3
     with pytest.raises(capi.PhxError):
         capi.call("phx_grad_accumulate", C.byref(capi.make_grid((8, 8))), C.c_void_p(1), C.c_void_p(1), 2, 1.0, None)
+    lib = capi.lib()
+    # the four-step Euler tile: -1..3
+    for t in (-2, 4):
+        assert lib.phx_ch_euler_set_quad(t) == capi.PHX_ERR_INVALID
+    assert lib.phx_ch_euler_set_quad(-1) == 0 and lib.phx_ch_euler_set_quad(0) == 0
+    # surrogate rollout with kept states: shape, NULL and `every` checks (no launch)
+    fake = C.c_void_p(1)
+    assert lib.phx_sur_rollout_states(fake, fake, fake, fake, None, 1, 40, 32, 4, 2, 0, None) == capi.PHX_ERR_INVALID
+    assert lib.phx_sur_rollout_states(fake, fake, fake, None, None, 1, 32, 32, 4, 2, 0, None) == capi.PHX_ERR_INVALID
+    assert lib.phx_sur_rollout_states(fake, fake, fake, fake, None, 1, 32, 32, 4, 3, 0, None) == capi.PHX_ERR_INVALID
+    assert b"dividing" in lib.phx_last_error()
+    assert lib.phx_sur_rollout_states(fake, fake, fake, fake, None, 1, 32, 32, 0, 1, 0, None) == capi.PHX_ERR_INVALID
 
 
 def _launch(g, dtype):
