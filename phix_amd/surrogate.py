@@ -539,8 +539,10 @@ class PararealSolver:
         self.U = torch.zeros((slices + 1, 1, n, n), dtype=torch.float32, device=dev)
         self.G = torch.zeros((slices, 1, n, n), dtype=torch.float32, device=dev)
         self.D = torch.zeros((slices, 1, n, n), dtype=torch.float32, device=dev)
-        self.Fs = torch.zeros((slices, 1, n, n), dtype=torch.float32, device=dev)   # F(U[s]) of the last sweep
         self.eslots = torch.empty(2 * slices, dtype=torch.float64, device=dev)
+        # the judge's own copy of the stored layout, so the fine results stay
+        # in `arena` until the next fine sweep (the fallback reads them there)
+        self.earena = torch.zeros_like(self.arena)
         self.lead = None
 
     def close(self):
@@ -586,11 +588,11 @@ class PararealSolver:
         phx_mass_energy on it); one readback."""
         import torch
         n, J = self.n, self.judge
-        self.arena[:, 1, 1:n + 1, 1:n + 1].copy_(states)
+        self.earena[:, 1, 1:n + 1, 1:n + 1].copy_(states)
         if getattr(self, "_ered", None) is None:
             self._ered = torch.empty(self.S * capi.lib().phx_reduce_workspace_bytes(C.byref(J.g)) // 8,
                                      dtype=torch.float64, device=self.dev)
-        capi.call("phx_mass_energy_batch", C.byref(J.g), J.dtype, self.S, capi.ptr(self.arena), C.byref(J.p),
+        capi.call("phx_mass_energy_batch", C.byref(J.g), J.dtype, self.S, capi.ptr(self.earena), C.byref(J.p),
                   capi.ptr(self._ered), capi.ptr(self.eslots), capi.stream_handle())
         return self.eslots.view(self.S, 2)[:, 1].tolist()
 
@@ -620,8 +622,8 @@ class PararealSolver:
         incs = []
         ok = True
         for it in range(iters):
-            self.Fs[:, 0].copy_(self._F_all())
-            torch.sub(self.Fs, G, out=D)
+            F = self._F_all()   # (S, n, n) view of the arena, kept until the next fine sweep
+            torch.sub(F, G[:, 0], out=D[:, 0])
             last = U[S].clone()
             if it == iters - 1:
                 self._sweep(D)   # the last sweep: G(U_new[s]) itself is not needed again
@@ -629,11 +631,12 @@ class PararealSolver:
                 for s in range(S):
                     self._G(U[s], G[s])
                     torch.add(G[s], D[s], out=U[s + 1])
-            incs.append(((U[S] - last).norm() / U[S].norm()).item())
+            inc = (U[S] - last).norm() / U[S].norm()   # read with the energies (one wait)
             es = self._energies(U[1:, 0])
+            incs.append(inc.item())
             if any(b > a * (1 + 1e-6) for a, b in zip(es, es[1:])):
                 ok = False   # fallback: the solver's states of this iteration
-                U[1:].copy_(self.Fs)
+                U[1:, 0].copy_(F)
         torch.cuda.synchronize()
         jumps = S * self.jumps * (1 + iters)
         return U[S, 0].clone(), PararealReport(time.perf_counter() - t0, iters, S, jumps,
