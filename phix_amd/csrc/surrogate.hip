@@ -1031,6 +1031,14 @@ int phx_sur_rollout_states(const float* params, void* ws, const float* c_in, flo
     if (!params || !ws || !c_in || !states) return fail(PHX_ERR_INVALID, "phx_sur_rollout_states: NULL argument");
     if (jumps < 1 || every < 1 || jumps % every)
         return fail(PHX_ERR_INVALID, "phx_sur_rollout_states: need jumps >= 1 and every >= 1 dividing jumps");
+    // the kept states are written while c_in and corr are still read
+    const size_t field = (size_t)B * H * W * sizeof(float), kept = field * (size_t)(jumps / every);
+    auto overlap = [](const void* a, size_t na, const void* b, size_t nb) {
+        const char *pa = (const char*)a, *pb = (const char*)b;
+        return pa < pb + nb && pb < pa + na;
+    };
+    if (overlap(states, kept, c_in, field) || (corr && overlap(states, kept, corr, kept)))
+        return fail(PHX_ERR_INVALID, "phx_sur_rollout_states: states overlaps c_in or corr");
     SurWS S;
     ws_layout(B, H, W, &S, (char*)ws);
     return sur_rollout(params, S, c_in, nullptr, B, H, W, jumps, flags, as_stream(s), states, every, corr);

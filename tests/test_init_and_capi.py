@@ -78,6 +78,13 @@ def test_argument_errors_are_reported():
     assert lib.phx_sur_rollout_states(fake, fake, fake, fake, None, 1, 32, 32, 4, 3, 0, None) == capi.PHX_ERR_INVALID
     assert b"dividing" in lib.phx_last_error()
     assert lib.phx_sur_rollout_states(fake, fake, fake, fake, None, 1, 32, 32, 0, 1, 0, None) == capi.PHX_ERR_INVALID
+    base = 1 << 20   # host-side range checks only: no memory is touched
+    field = 32 * 32 * 4
+    assert lib.phx_sur_rollout_states(fake, fake, C.c_void_p(base), C.c_void_p(base - field), None, 1, 32, 32, 4, 2, 0,
+                                      None) == capi.PHX_ERR_INVALID   # states[1] is c_in
+    assert b"overlaps" in lib.phx_last_error()
+    assert lib.phx_sur_rollout_states(fake, fake, C.c_void_p(base), C.c_void_p(base + field), C.c_void_p(base + 2 * field),
+                                      1, 32, 32, 4, 2, 0, None) == capi.PHX_ERR_INVALID   # states[1] is corr[0]
 
 
 def _launch(g, dtype):
