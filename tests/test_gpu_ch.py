@@ -44,6 +44,23 @@ def test_config1_euler_every_step(dev):
     assert abs(m / (128 * 128) - 0.5) < 1e-3
 
 
+def test_config1_euler_one_call_against_oracle(dev):
+    """Config 1 as the CH app advances it: all 1000 steps in one
+    phx_ch_euler_steps call (249 launches of four steps, three of one, the
+    last step with the two kernels), every stored cell of the active region
+    of c and mu against the FP64 oracle's 1000 steps: equal bit for bit."""
+    n = (128, 128)
+    g = capi.make_grid(n, bc=P)
+    p = ch_params(rho=1.0, ca=0.0, cb=1.0, kappa=0.5, M=1.0, dt=0.01)
+    c = init.to_stored(init.ch_field(n, 1, 0.5, 0.01), g)
+    mu = np.zeros_like(c)
+    tc, tm = to_dev(c, dev), to_dev(mu, dev)
+    capi.call("phx_ch_euler_steps", C.byref(g), capi.F64, capi.ptr(tc), capi.ptr(tm), C.byref(p), 1000, stream())
+    O.ch_euler_steps(g, p, c, mu, 1000)
+    assert np.array_equal(capi.active(to_np(tc), g), capi.active(c, g))
+    assert np.array_equal(capi.active(to_np(tm), g), capi.active(mu, g))
+
+
 def test_euler_multistep_batched_matches_single(dev):
     g = capi.make_grid((33, 17, 9), bc=[[N, N], [F, N], [P, P]], bc_value=[[0, 0], [0.4, 0], [0, 0]])
     p = ch_params(rho=1.0, kappa=0.5, dt=0.005)
