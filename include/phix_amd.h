@@ -156,9 +156,10 @@ int phx_convert_f32_to_f64(double* dst, const float* src, size_t n, phx_stream s
  * ghosts -- the state the reference leaves. Every face of grid->bc must be set. */
 int phx_ch_euler_steps(const phx_grid* grid, int dtype, void* c, void* mu,
                        const phx_ch_params* p, int nsteps, phx_stream stream);
-/* No reference counterpart. On 1D and 2D grids every Euler step but the last
- * runs as one launch (the exact-form stage kernel: mu on a ring around each
- * tile, then c + dt * M lap(mu)) instead of two, with the same results bit for
+/* No reference counterpart. Every Euler step but the last runs as one launch
+ * (the exact-form stage kernel: mu on a ring around each tile, then
+ * c + dt * M lap(mu); in 3D the LDS-ring kernel streaming z-chunks of
+ * 64x8 columns) instead of two, with the same results bit for
  * bit; mu serves as the second state buffer until the last step. on = 0
  * keeps two launches per step (process-wide). Returns PHX_OK. */
 int phx_ch_euler_set_fused(int on);

@@ -600,7 +600,9 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
             const long long gi = pb + ooff[j];
             const T uc = uo[(oy + 2) * UX + ox + 2];
             if (EXACT) {
-                if (st == 1) {
+                if (st == 0) {   // an explicit Euler step (multi-step Euler)
+                    A.out[gi] = uc + A.a * k;
+                } else if (st == 1) {
                     A.acc[gi] = k;
                     A.out[gi] = uc + A.a * k;
                 } else if (st < 4) {
@@ -2020,9 +2022,10 @@ static int g_euler_fused = 1;
 // oracle. Multi-step Euler always uses the reference's order.
 static int g_fp64_exact = 0;
 
-// Multi-step Euler, one launch per step (1D and 2D): the one-stage stencil
-// kernel in its exact form evaluates mu on a ring around each 32x8 tile (the
-// ghost mu follows the BC fill of mu, as k_euler_mu's ghost cells do) and
+// Multi-step Euler, one launch per step: the one-stage stencil kernel in its
+// exact form (32x8 tiles in 1D/2D, the LDS-ring kernel's 64x8 columns in 3D)
+// evaluates mu on a ring around each tile (the ghost mu follows the BC fill of
+// mu, as k_euler_mu's ghost cells do) and
 // writes c + dt * (M lap mu), the operations of k_euler_mu / k_euler_c in the
 // same order, so the results are the same bits. The state alternates
 // between c and mu (free until the last step); every step but the last runs
@@ -2050,9 +2053,11 @@ static int euler_fused_steps_t(const phx_grid* G, T* c, T* mu, const phx_ch_para
     A.third = (T)0;
     A.s2 = A.s3 = nullptr;
     A.stage = 0;
-    A.zc = 1;
-    const dim3 grid(cdiv(A.L.n0, 32), G->dim >= 2 ? cdiv(A.L.n1, 8) : 1, 1);
-    A.xcd_remap = g_xcd_remap && (grid.x * grid.y) % 8 == 0;
+    // 3D: the LDS-ring one-stage kernel, 64x8 columns over z-chunks of ~16 planes
+    const dim3 grid = G->dim == 3 ? dim3(cdiv(A.L.n0, 64), cdiv(A.L.n1, 8), cdiv(A.L.n2, 16))
+                                  : dim3(cdiv(A.L.n0, 32), G->dim >= 2 ? cdiv(A.L.n1, 8) : 1, 1);
+    A.zc = G->dim == 3 ? 16 : 1;
+    A.xcd_remap = g_xcd_remap && (grid.x * grid.y * grid.z) % 8 == 0;
     const int fused = nsteps - 1;
     const int quads = g_euler4_tile >= 0 && euler4_ok(G) ? fused / 4 : 0;
     const int singles = fused - 4 * quads;
@@ -2069,7 +2074,8 @@ static int euler_fused_steps_t(const phx_grid* G, T* c, T* mu, const phx_ch_para
         A.c = X;
         A.acc = Y;
         A.out = Y;
-        if (G->dim == 2) hipLaunchKernelGGL((k_ch_stage_2d<T, 2, 32, 8, true>), grid, dim3(256), 0, st, A);
+        if (G->dim == 3) hipLaunchKernelGGL((k_ch_stage_3d<T, 64, 8, true>), grid, dim3(256), 0, st, A);
+        else if (G->dim == 2) hipLaunchKernelGGL((k_ch_stage_2d<T, 2, 32, 8, true>), grid, dim3(256), 0, st, A);
         else hipLaunchKernelGGL((k_ch_stage_2d<T, 1, 32, 8, true>), grid, dim3(256), 0, st, A);
         std::swap(X, Y);
     }
@@ -2078,7 +2084,7 @@ static int euler_fused_steps_t(const phx_grid* G, T* c, T* mu, const phx_ch_para
 
 template <typename T>
 static int euler_steps_t(const phx_grid* G, T* c, T* mu, const phx_ch_params* p, int nsteps, hipStream_t st) {
-    if (g_euler_fused && G->dim <= 2 && nsteps >= 2) {
+    if (g_euler_fused && nsteps >= 2) {
         int rc = euler_fused_steps_t<T>(G, c, mu, p, nsteps, st);
         if (rc) return rc;
         nsteps = 1;

@@ -215,11 +215,17 @@ def test_mass_energy_against_oracle(dev, dtype, case):
     ((30, 41), [[P, P], [N, F]], [[0, 0], [0, 0.6]], 4),
     ((3, 2), N, 0.0, 3),
     ((50,), P, 0.0, 9),
+    ((32, 32, 32), P, 0.0, 6),
+    ((70, 20, 37), N, 0.0, 5),
+    ((40, 9, 33), [[F, N], [P, P], [N, F]], [[0.3, 0], [0, 0], [0, 0.6]], 4),
+    ((9, 17, 2), [[P, P], [F, F], [P, P]], [[0, 0], [0.45, 0.55], [0, 0]], 3),
+    ((2, 3, 18), [[N, N], [N, N], [F, N]], [[0, 0], [0, 0], [0.4, 0]], 7),
 ])
 def test_euler_one_launch_per_step_equals_two(dev, dtype, n, bc, vals, nsteps):
-    """1D/2D Euler: the one-launch-per-step path returns the two-kernel path's
-    bits for c and mu (ghost layers included), odd and even step counts; FP64
-    also against the oracle."""
+    """Euler in 1D, 2D and 3D: the one-launch-per-step path returns the
+    two-kernel path's bits for c and mu (ghost layers included), odd and even
+    step counts, partial tiles and several z-chunks; FP64 also against the
+    oracle."""
     g = capi.make_grid(n, bc=bc, bc_value=vals)
     p = ch_params(rho=1.0, ca=0.0, cb=1.0, kappa=0.5, M=1.0, dt=0.01)
     c0 = init.to_stored(init.ch_field(n, 9, 0.5, 0.05), g)
