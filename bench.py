@@ -167,7 +167,7 @@ def line(metric, value, unit, steps, warmup, ms, dtype, data, config, roof, cpu)
 
 def fp64_form_name():
     from phix_amd import capi
-    return "exact (reference order)" if capi.lib().phx_ch_get_fp64_exact() else "fma, stage states"
+    return "exact (reference order)" if capi.lib().phx_ch_get_fp64_exact() else "fma, stage states and b (12 passes)"
 
 
 # ------------------------------------------------------------ workloads ----
@@ -215,13 +215,14 @@ def ch_rk4_run(a, dtype_name, dim):
     kinds = launch_shares(kernel_times(run, a.steps), ms)
     cells = math.prod(n)
     if dim == 3:
-        # the kernels' own passes: one-stage 13/4 (stage-state form: FP32 and
-        # the default FP64 form) or 17/4 (FP64 exact form: the reference's
+        # the kernels' own passes: one-stage 13/4 (FP32 stage-state form),
+        # 12/4 (default FP64 form: stage 2 writes b = u2 + 2 u3 - c_n, stage 4
+        # reads u4 and b) or 17/4 (FP64 exact form: the reference's
         # k1+2k2+2k3+k4 order re-reads c_n in stage 4); two-stage (1,2): read
-        # c_n, write the stage-3 state and acc/u2; (3,4): read the stage-3
-        # state, c_n and acc/u2, write c_{n+1}
+        # c_n, write the stage-3 state and acc/b/u2; (3,4): read the stage-3
+        # state, c_n and acc/b/u2, write c_{n+1}
         exact = dtype == capi.F64 and capi.lib().phx_ch_get_fp64_exact()
-        own = {0: (17 if exact else 13) / 4, 1: 3, 2: 4}
+        own = {0: (17 if exact else (12 if dtype == capi.F64 else 13)) / 4, 1: 3, 2: 4}
         names = {0: "k_ch_stage_3dz" if tile.value >= 9 else "k_ch_stage_3d", 1: "k_ch_pair_3d (stages 1+2)",
                  2: "k_ch_pair_3d (stages 3+4)"}
         tags = {0: "ch3d_rk4", 1: "ch3d_pair12", 2: "ch3d_pair34"}
@@ -242,7 +243,10 @@ def ch_rk4_run(a, dtype_name, dim):
                                    "event_avg_us": round(v[2] * 1e3, 2),
                                    "alg_bytes": CH_ALG[k] * esz * cells, "kernel_bytes": own[k] * esz * cells}
                         for k, v in kinds.items()}
-    form = fp64_form_name() if dtype == capi.F64 else "fma, stage states" if dim == 3 else "fma, accumulator"
+    if dtype == capi.F64 and (dim == 3 or capi.lib().phx_ch_get_fp64_exact()):
+        form = fp64_form_name()
+    else:
+        form = "fma, stage states" if dim == 3 else "fma, accumulator"
     return {"value": cells * a.steps / (ms / 1e3), "ms": ms, "reps_ms": reps, "roofline": roof,
             "launch": launch, "zchunk": zc.value, "form": form, "grid": g, "params": p}
 

@@ -278,7 +278,8 @@ int phx_ch_set_xcd_remap(int on);
 /* Arithmetic form of FP64 RK4 (no reference counterpart; process-wide, host
  * setting). 0, the default: explicit FMAs with M folded into lap(mu)'s
  * coefficients and, in 3D, the step finished from the stage states
- * u2, u3, u4 (13 field passes per step instead of 17); within a few ulp of
+ * u2, u3, u4 (13 field passes per step instead of 17; FP64 12, with stage 2
+ * also writing b = u2 + 2 u3 - c_n for stage 4); within a few ulp of
  * the reference's expression per step. 1: the reference's operation order
  * without contraction and its k1+2k2+2k3+k4 accumulator
  * (src/solver/Solver.cu:61-71, src/equation/Equation.cu:164-180), equal to

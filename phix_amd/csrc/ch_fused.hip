@@ -60,7 +60,7 @@ template <typename T>
 struct CHArgs {
     const T* u;     // stage state (stencil input)
     const T* c;     // c_n
-    T* acc;         // RK4 accumulator
+    T* acc;         // RK4 accumulator (exact FP64 form); 3D FP64: b = u2 + 2 u3 - c_n
     T* out;         // next stage state, or c for stage 4
     Layout L;
     AxisBC<T> bx, by, bz;
@@ -153,6 +153,29 @@ __device__ __forceinline__ T rk4_fin(T dt6, T third, T c, T u2, T u3, T u4, T k)
     const T d = (u2 - c) + fma_t((T)2, u3 - c, u4 - c);
     return c + fma_t(dt6, k, third * d);
 }
+
+// FP64 form of the 3D kernels (12 field passes per step): the same sum
+// regrouped as c_{n+1} = u4 + (b - 2 u4) / 3 + dt/6 k4 with b = u2 + 2 u3 - c,
+// which stage 2 forms (it holds u2, c and u3 at the cell) and writes to a
+// field of its own, so stage 4 reads b instead of c, u2 and u3
+// (stage 1: c -> u2; 2: u2, c -> u3, b; 3: u3, c -> u4; 4: u4, b -> c).
+// b is rounded once at the scale of c (u2 - c is exact whenever the two lie
+// within a factor 2, as consecutive RK4 states do); b - 2 u4 is then exact
+// for the same reason and of increment size, so 1/3, itself rounded, scales
+// only an increment and adds no drift.
+// One more rounding at the scale of c per step than rk4_fin (unbiased, ~1e-16
+// in FP64), which is why FP32 keeps rk4_fin.
+template <typename T>
+__device__ __forceinline__ T rk4_b(T u2, T u3, T c) {
+    return fma_t((T)2, u3, u2 - c);
+}
+template <typename T>
+__device__ __forceinline__ T rk4_fin_b(T dt6, T third, T b, T u4, T k) {
+    return u4 + fma_t(dt6, k, third * fma_t((T)-2, u4, b));
+}
+// The 3D kernels use the b form for FP64 outside the exact (oracle-order) mode.
+template <typename T, bool EXACT>
+constexpr bool kBForm = sizeof(T) == 8 && !EXACT;
 
 // The six FIXED constants (x lo, x hi, y lo, y hi, z lo, z hi) staged in LDS
 // by each block: reading them there in divergent ghost code keeps the
@@ -533,11 +556,13 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
     // first element of the stage state for every lane (one cache line).
     // FP64: c_n (stages 2-4) and the accumulator (2-4); FP32: c_n (2-4) and
     // the stage states u2, u3 (4)
-    const bool need_c = EXACT ? (st == 2 || st == 3 || st == 4) : st >= 2;
+    // FP64 b form: c_n (stages 2-3) and b (4)
+    constexpr bool BF = kBForm<T, EXACT>;
+    const bool need_c = EXACT ? (st == 2 || st == 3 || st == 4) : (BF ? (st == 2 || st == 3) : st >= 2);
     const bool need_a = EXACT ? st >= 2 : st == 4;
-    const bool need_b = !EXACT && st == 4;
+    const bool need_b = !EXACT && !BF && st == 4;
     const T* csrc = need_c ? A.c : A.u;
-    const T* asrc = need_a ? (EXACT ? A.acc : A.s2) : A.u;
+    const T* asrc = need_a ? (EXACT || BF ? A.acc : A.s2) : A.u;
     const T* bsrc = need_b ? A.s3 : A.u;
     const long long cmask = need_c ? -1LL : 0LL, amask = need_a ? -1LL : 0LL, bmask = need_b ? -1LL : 0LL;
     T pc[NO], pa[NO], pq[NO], nc[NO], na[NO], nq[NO];
@@ -547,7 +572,7 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
         for (int j = 0; j < NO; ++j) {
             rc[j] = csrc[(pb + ooff[j]) & cmask];
             ra[j] = asrc[(pb + ooff[j]) & amask];
-            if constexpr (!EXACT) rb[j] = bsrc[(pb + ooff[j]) & bmask];
+            if constexpr (!EXACT && !BF) rb[j] = bsrc[(pb + ooff[j]) & bmask];
         }
     };
 
@@ -612,9 +637,16 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
                     A.out[gi] = pc[j] + A.dt6 * (pa[j] + k);
                 }
             } else {
-                if (st == 1) A.out[gi] = axpy<T, false>(A.a, k, uc);
-                else if (st < 4) A.out[gi] = axpy<T, false>(A.a, k, pc[j]);
-                else A.out[gi] = rk4_fin<T>(A.dt6, A.third, pc[j], pa[j], pq[j], uc, k);
+                if (st == 1) {
+                    A.out[gi] = axpy<T, false>(A.a, k, uc);
+                } else if (st < 4) {
+                    const T v = axpy<T, false>(A.a, k, pc[j]);
+                    A.out[gi] = v;
+                    if (BF && st == 2) A.acc[gi] = rk4_b<T>(uc, v, pc[j]);
+                } else {
+                    A.out[gi] = BF ? rk4_fin_b<T>(A.dt6, A.third, pa[j], uc, k)
+                                   : rk4_fin<T>(A.dt6, A.third, pc[j], pa[j], pq[j], uc, k);
+                }
             }
         }
 #pragma unroll
@@ -668,9 +700,10 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
 // thread's tile cells are then never ghosts and sit RS = NT / TX rows apart,
 // so their LDS indices and global offsets are one base plus constants
 // (immediate offsets) instead of per-slot tables: ~25 fewer VGPRs.
-// SC: 1, 2 (stages 2 and 3, which share their epilogue form) and 4 fix the
-// stage class at compile time, so the stage tests and the unused epilogue
-// loads fold away (0 would read it from A.stage).
+// SC: 1, 2 (stages 2 and 3, which share their epilogue form), 3 (stage 2
+// of the FP64 b form, which also writes b) and 4 fix the stage class at
+// compile time, so the stage tests and the unused epilogue loads fold away
+// (0 would read it from A.stage).
 template <typename T, int TX, int TY, int NT, bool EXACT, bool FULL = false, int SC = 0>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, NT))) void k_ch_stage_3dz(CHArgs<T> A) {
     constexpr int UX = TX + 4, UY = TY + 4, MX = TX + 2, MY = TY + 2;
@@ -888,12 +921,16 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, NT))) void k_ch_stage_3
     // element of the stage state for every lane: one cache line per wave).
     // FP64: c_n (stages 2-4) and the accumulator (2-4); FP32: c_n (2-4) and
     // the stage states u2, u3 (4).
-    const int st = SC ? SC : A.stage;
-    const bool need_c = EXACT ? (st == 2 || st == 3 || st == 4) : st >= 2;
+    // FP64 b form: c_n (stages 2-3) and b (4); its stage 2 (class 3) also
+    // writes b to A.acc.
+    constexpr bool BF = kBForm<T, EXACT>;
+    const int st = SC == 3 ? 2 : (SC ? SC : A.stage);
+    const bool wb = BF && (SC ? SC == 3 : A.stage == 2);
+    const bool need_c = EXACT ? (st == 2 || st == 3 || st == 4) : (BF ? (st == 2 || st == 3) : st >= 2);
     const bool need_a = EXACT ? st >= 2 : st == 4;
-    const bool need_b = !EXACT && st == 4;
+    const bool need_b = !EXACT && !BF && st == 4;
     const T* csrc = need_c ? A.c : A.u;
-    const T* asrc = need_a ? (EXACT ? A.acc : A.s2) : A.u;
+    const T* asrc = need_a ? (EXACT || BF ? A.acc : A.s2) : A.u;
     const T* bsrc = need_b ? A.s3 : A.u;
     const long long cpm = need_c ? -1LL : 0LL, apm = need_a ? -1LL : 0LL, bpm = need_b ? -1LL : 0LL;
     const unsigned com = need_c ? ~0u : 0u, aom = need_a ? ~0u : 0u, bom = need_b ? ~0u : 0u;
@@ -906,7 +943,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, NT))) void k_ch_stage_3
             const unsigned oj = FULL ? off0 : ooff[j];
             rc[j] = at(csrc + (pj & cpm), oj & com);
             ra[j] = at(asrc + (pj & apm), oj & aom);
-            if constexpr (!EXACT) rb[j] = at(bsrc + (pj & bpm), oj & bom);
+            if constexpr (!EXACT && !BF) rb[j] = at(bsrc + (pj & bpm), oj & bom);
         }
     };
 
@@ -1017,9 +1054,16 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, NT))) void k_ch_stage_3
                     out = P.ec[j] + A.dt6 * (P.ea[j] + k);
                 }
             } else {
-                if (st == 1) out = axpy<T, false>(A.a, k, uc);
-                else if (st < 4) out = axpy<T, false>(A.a, k, P.ec[j]);
-                else out = rk4_fin<T>(A.dt6, A.third, P.ec[j], P.ea[j], P.eb[j], uc, k);
+                if (st == 1) {
+                    out = axpy<T, false>(A.a, k, uc);
+                } else if (st < 4) {
+                    const T v = axpy<T, false>(A.a, k, P.ec[j]);
+                    out = v;
+                    if (wb) acc = rk4_b<T>(uc, v, P.ec[j]);
+                } else {
+                    out = BF ? rk4_fin_b<T>(A.dt6, A.third, P.ea[j], uc, k)
+                             : rk4_fin<T>(A.dt6, A.third, P.ec[j], P.ea[j], P.eb[j], uc, k);
+                }
             }
         }
         if constexpr (DEPTH == 2) load_epi(min(o + 2, z1 - 1), P.ec, P.ea, P.eb);
@@ -1087,8 +1131,9 @@ template <typename T>
 struct PairArgs {
     const T* u;    // first stage's input state: c_n for (1,2), the stage-3 state for (3,4)
     const T* c;    // c_n (read by (3,4))
-    T* acc;        // RK4 accumulator (FP64) or the stage-2 state u2 (FP32):
-                   // written by (1,2), read by (3,4)
+    T* acc;        // RK4 accumulator (exact FP64 form), b = u2 + 2 u3 - c_n
+                   // (FP64) or the stage-2 state u2 (FP32): written by (1,2),
+                   // read by (3,4)
     T* out;        // (1,2): stage-3 state; (3,4): c_{n+1}
     Layout L;
     AxisBC<T> bx, by, bz;
@@ -1175,6 +1220,7 @@ __global__ __launch_bounds__(256) void k_ch_pair_3d(PairArgs<T> A) {
     constexpr int S0 = N0 / 256;
     static_assert(N0 % 256 == 0, "tile must be a multiple of 256 cells");
     constexpr bool NEED_CR = true;   // c_n of the tile cells, read by both epilogues
+    constexpr bool BF = kBForm<T, EXACT>;   // FP64: (1,2) writes b to acc, (3,4) finishes from b
     __shared__ T UA[3][N4];
     __shared__ T MA[3][N3];
     __shared__ T UB[3][N2];
@@ -1266,7 +1312,7 @@ __global__ __launch_bounds__(256) void k_ch_pair_3d(PairArgs<T> A) {
     int ooff[S0];
     bool oval[S0];
     T areg[SECOND ? S0 : 1];
-    T breg[(SECOND && !EXACT) ? S0 : 1];   // FP32 (3,4): the stage-3 state at the tile cells
+    T breg[(SECOND && !EXACT && !BF) ? S0 : 1];   // FP32 (3,4): the stage-3 state at the tile cells
 #pragma unroll
     for (int j = 0; j < S0; ++j) {
         const int t = tid + 256 * j;
@@ -1308,7 +1354,7 @@ __global__ __launch_bounds__(256) void k_ch_pair_3d(PairArgs<T> A) {
         const long long pb = (long long)(o + g) * sxy;
 #pragma unroll
         for (int j = 0; j < (SECOND ? S0 : 1); ++j) areg[j] = A.acc[pb + ooff[j]];
-        if constexpr (SECOND && !EXACT) {
+        if constexpr (SECOND && !EXACT && !BF) {
 #pragma unroll
             for (int j = 0; j < S0; ++j) breg[j] = A.u[pb + ooff[j]];
         }
@@ -1457,15 +1503,20 @@ __global__ __launch_bounds__(256) void k_ch_pair_3d(PairArgs<T> A) {
             const T k = k_star<T, EXACT>(A, s7);
             const long long gi = pb + ooff[j];
             if constexpr (!SECOND) {
-                // stage 2: FP64 acc = k1 + 2*k2; FP32 keeps u2 there; state c + dt/2*k2
+                // stage 2: state c + dt/2*k2; acc = k1 + 2*k2 (exact form), b (FP64
+                // b form) or u2 (FP32)
+                const T v = axpy<T, EXACT>(A.a2, k, cr[t]);
                 if (EXACT) A.acc[gi] = kr[t] + (T)2 * k;
+                else if (BF) A.acc[gi] = rk4_b<T>(kr[t], v, cr[t]);
                 else A.acc[gi] = kr[t];
-                A.out[gi] = axpy<T, EXACT>(A.a2, k, cr[t]);
+                A.out[gi] = v;
             } else {
-                // stage 3 accumulates k3 (FP64) or forms u4 (FP32), stage 4 finishes the step
+                // stage 3 accumulates k3 (exact form) or forms u4, stage 4 finishes the step
                 if (EXACT) {
                     const T ac = areg[j] + (T)2 * kr[t];
                     A.out[gi] = cr[t] + A.dt6 * (ac + k);
+                } else if (BF) {
+                    A.out[gi] = rk4_fin_b<T>(A.dt6, A.third, areg[j], kr[t], k);
                 } else {
                     A.out[gi] = rk4_fin<T>(A.dt6, A.third, cr[t], areg[j], breg[j], kr[t], k);
                 }
@@ -2016,7 +2067,8 @@ static int g_xcd_remap = 1;
 static int g_euler_fused = 1;
 // FP64 RK4 arithmetic form. 0 (default): the FP32 form evaluated in double --
 // explicit FMAs, M folded into lap(mu)'s coefficients and, in 3D, the step
-// finished from the stage states (13 field passes per step instead of 17);
+// finished from the stage states and b (rk4_b, rk4_fin_b: 12 field passes per
+// step instead of 17);
 // within a few ulp of the oracle per step. 1: the reference's operation order
 // without contraction and its k1+2k2+2k3+k4 accumulator, bit-equal to the
 // oracle. Multi-step Euler always uses the reference's order.
@@ -2165,6 +2217,12 @@ static void launch_zs(dim3 grid, const CHArgs<T>& A, hipStream_t st) {
     switch (A.stage) {
         case 1: hipLaunchKernelGGL((k_ch_stage_3dz<T, TX, TY, NT, EXACT, FULL, 1>), grid, dim3(NT), 0, st, A); break;
         case 4: hipLaunchKernelGGL((k_ch_stage_3dz<T, TX, TY, NT, EXACT, FULL, 4>), grid, dim3(NT), 0, st, A); break;
+        case 2:
+            if constexpr (kBForm<T, EXACT>) {
+                hipLaunchKernelGGL((k_ch_stage_3dz<T, TX, TY, NT, EXACT, FULL, 3>), grid, dim3(NT), 0, st, A);
+                break;
+            }
+            [[fallthrough]];
         default: hipLaunchKernelGGL((k_ch_stage_3dz<T, TX, TY, NT, EXACT, FULL, 2>), grid, dim3(NT), 0, st, A);
     }
 }
@@ -2416,11 +2474,14 @@ static int rk4_steps_t(const phx_grid* G, T* c, T* work, const phx_ch_params* p,
     A.zc = zc[1];
     A.xcd_remap = 0;
     const T shift[4] = {(T)(dt * 0.5), (T)(dt * 0.5), (T)dt, (T)0};
-    // FP64 and 2D: stage states in w0, w1, w0 and the accumulator in acc;
-    // 3D FP32: u2, u3, u4 in acc, w0, w1 (stage 4 reads u2 and u3)
+    // exact form and 2D: stage states in w0, w1, w0 and the accumulator in acc;
+    // 3D FP32: u2, u3, u4 in acc, w0, w1 (stage 4 reads u2 and u3);
+    // 3D FP64: u2, u3, u4 in acc, w0, acc and b in w1 (stage 4 reads b)
     const bool states = G->dim == 3 && !EXACT;
-    const T* uin[4] = {c, states ? acc : w0, states ? w0 : w1, states ? w1 : w0};
-    T* uout[4] = {states ? acc : w0, states ? w0 : w1, states ? w1 : w0, c};
+    const bool bform = states && kBForm<T, EXACT>;
+    const T* uin[4] = {c, states ? acc : w0, states ? w0 : w1, bform ? acc : (states ? w1 : w0)};
+    T* uout[4] = {states ? acc : w0, states ? w0 : w1, bform ? acc : (states ? w1 : w0), c};
+    if (bform) A.acc = w1;
     dim3 grid[3];
     int remap[3];
     for (int k = 0; k < 3; ++k) {
