@@ -299,8 +299,8 @@ int phx_ch_rk4_autotune(const phx_grid* grid, int dtype, void* c, void* work,
  * set all three classes; phx_ch_rk4_get_launch reports class 1, whose tile
  * also chooses the schedule (a two-stage tile there runs every stage
  * two-stage). Defaults in 3D FP32: classes 0 and 1 tile 11 at 24 z-chunks,
- * class 2 tile 12 at 16; FP64: class 0 tile 9 at 12 z-chunks, classes 1 and
- * 2 tile 17 at 8 and 4 z-chunks. */
+ * class 2 tile 12 at 16; FP64: class 0 tile 11 at 8 z-chunks, classes 1 and
+ * 2 tile 17 at 4 and 8 z-chunks. */
 int phx_ch_rk4_set_stage_launch(const phx_grid* grid, int dtype, int cls, int tile, int zc);
 int phx_ch_rk4_get_stage_launch(const phx_grid* grid, int dtype, int cls, int* zc, int* tile);
 
