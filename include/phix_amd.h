@@ -158,8 +158,8 @@ int phx_ch_euler_steps(const phx_grid* grid, int dtype, void* c, void* mu,
                        const phx_ch_params* p, int nsteps, phx_stream stream);
 /* No reference counterpart. Every Euler step but the last runs as one launch
  * (the exact-form stage kernel: mu on a ring around each tile, then
- * c + dt * M lap(mu); in 3D the LDS-ring kernel streaming z-chunks of
- * 64x8 columns) instead of two, with the same results bit for
+ * c + dt * M lap(mu); in 3D the z-register kernel streaming z-chunks of
+ * 128x16 columns) instead of two, with the same results bit for
  * bit; mu serves as the second state buffer until the last step. on = 0
  * keeps two launches per step (process-wide). Returns PHX_OK. */
 int phx_ch_euler_set_fused(int on);

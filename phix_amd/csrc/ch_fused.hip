@@ -1044,7 +1044,9 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, NT))) void k_ch_stage_3
             T& out = *(T*)((char*)(FULL ? outp + j * rowe : outp) + ob);
             const T uc = u0[j];
             if (EXACT) {
-                if (st == 1) {
+                if (st == 0) {   // an explicit Euler step (multi-step Euler)
+                    out = uc + A.a * k;
+                } else if (st == 1) {
                     acc = k;
                     out = uc + A.a * k;
                 } else if (st < 4) {
@@ -2075,9 +2077,9 @@ static int g_euler_fused = 1;
 static int g_fp64_exact = 0;
 
 // Multi-step Euler, one launch per step: the one-stage stencil kernel in its
-// exact form (32x8 tiles in 1D/2D, the LDS-ring kernel's 64x8 columns in 3D)
-// evaluates mu on a ring around each tile (the ghost mu follows the BC fill of
-// mu, as k_euler_mu's ghost cells do) and
+// exact form (32x8 tiles in 1D/2D, the z-register kernel's 128x16 columns in
+// 3D) evaluates mu on a ring around each tile (the ghost mu follows the BC
+// fill of mu, as k_euler_mu's ghost cells do) and
 // writes c + dt * (M lap mu), the operations of k_euler_mu / k_euler_c in the
 // same order, so the results are the same bits. The state alternates
 // between c and mu (free until the last step); every step but the last runs
@@ -2085,6 +2087,7 @@ static int g_fp64_exact = 0;
 // layer and mu as the two-kernel path does.
 static int g_euler4_tile = 0;   // four-step Euler tile (phx_ch_euler_set_quad); -1: off
 static bool euler4_ok(const phx_grid* G);
+static bool zreg_ok(const phx_grid* G, int dtype);
 template <typename T>
 static void euler4_run(const phx_grid* G, T*& X, T*& Y, const phx_ch_params* p, int count, hipStream_t st);
 
@@ -2105,8 +2108,30 @@ static int euler_fused_steps_t(const phx_grid* G, T* c, T* mu, const phx_ch_para
     A.third = (T)0;
     A.s2 = A.s3 = nullptr;
     A.stage = 0;
-    // 3D: the LDS-ring one-stage kernel, 64x8 columns over z-chunks of ~16 planes
-    const dim3 grid = G->dim == 3 ? dim3(cdiv(A.L.n0, 64), cdiv(A.L.n1, 8), cdiv(A.L.n2, 16))
+    // 3D: the z-register one-stage kernel, 128x16 columns per 512 threads, with
+    // as many z-chunks (of at least 4 planes) as fill one round of resident
+    // workgroups on the device -- a partial second round would cost a whole
+    // round; the LDS-ring kernel's 64x8 columns over chunks of 16 planes where
+    // a plane is too large for 32-bit offsets
+    const bool zr = G->dim == 3 && zreg_ok(G, sizeof(T) == 8 ? PHX_F64 : PHX_F32);
+    const bool zfull = A.L.n0 % 128 == 0 && A.L.n1 % 16 == 0;
+    const auto kz = zfull ? &k_ch_stage_3dz<T, 128, 16, 512, true, true, 0>
+                          : &k_ch_stage_3dz<T, 128, 16, 512, true, false, 0>;
+    int ezc = 1;
+    if (zr) {
+        int dev = 0, ncu = 0, per_cu = 0;
+        PHX_HIP(hipGetDevice(&dev));
+        PHX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kz), 512, 0) !=
+            hipSuccess) {
+            (void)hipGetLastError();
+            per_cu = 1;
+        }
+        const int ncol = cdiv(A.L.n0, 128) * cdiv(A.L.n1, 16);
+        ezc = std::max(1, std::min(std::max(1, A.L.n2 / 4), ncu * std::max(per_cu, 1) / ncol));
+    }
+    const dim3 grid = zr ? dim3(cdiv(A.L.n0, 128), cdiv(A.L.n1, 16), ezc)
+                    : G->dim == 3 ? dim3(cdiv(A.L.n0, 64), cdiv(A.L.n1, 8), cdiv(A.L.n2, 16))
                                   : dim3(cdiv(A.L.n0, 32), G->dim >= 2 ? cdiv(A.L.n1, 8) : 1, 1);
     A.zc = G->dim == 3 ? 16 : 1;
     A.xcd_remap = g_xcd_remap && (grid.x * grid.y * grid.z) % 8 == 0;
@@ -2126,7 +2151,8 @@ static int euler_fused_steps_t(const phx_grid* G, T* c, T* mu, const phx_ch_para
         A.c = X;
         A.acc = Y;
         A.out = Y;
-        if (G->dim == 3) hipLaunchKernelGGL((k_ch_stage_3d<T, 64, 8, true>), grid, dim3(256), 0, st, A);
+        if (zr) hipLaunchKernelGGL(kz, grid, dim3(512), 0, st, A);
+        else if (G->dim == 3) hipLaunchKernelGGL((k_ch_stage_3d<T, 64, 8, true>), grid, dim3(256), 0, st, A);
         else if (G->dim == 2) hipLaunchKernelGGL((k_ch_stage_2d<T, 2, 32, 8, true>), grid, dim3(256), 0, st, A);
         else hipLaunchKernelGGL((k_ch_stage_2d<T, 1, 32, 8, true>), grid, dim3(256), 0, st, A);
         std::swap(X, Y);
