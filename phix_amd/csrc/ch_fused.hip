@@ -554,9 +554,8 @@ __global__ __launch_bounds__(256) void k_ch_stage_3d(CHArgs<T> A) {
     // iteration issues the same loads (the compiler then waits for exactly the
     // ones it uses); an operand the stage does not need is read from the
     // first element of the stage state for every lane (one cache line).
-    // FP64: c_n (stages 2-4) and the accumulator (2-4); FP32: c_n (2-4) and
-    // the stage states u2, u3 (4)
-    // FP64 b form: c_n (stages 2-3) and b (4)
+    // Exact FP64 form: c_n (stages 2-4) and the accumulator (2-4); FP64 b
+    // form: c_n (2-3) and b (4); FP32: c_n (2-4) and the stage states u2, u3 (4)
     constexpr bool BF = kBForm<T, EXACT>;
     const bool need_c = EXACT ? (st == 2 || st == 3 || st == 4) : (BF ? (st == 2 || st == 3) : st >= 2);
     const bool need_a = EXACT ? st >= 2 : st == 4;
@@ -919,10 +918,9 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, NT))) void k_ch_stage_3
     // epilogue operands (all loaded for every stage: the same loads each
     // iteration; an operand the stage does not need is read from the first
     // element of the stage state for every lane: one cache line per wave).
-    // FP64: c_n (stages 2-4) and the accumulator (2-4); FP32: c_n (2-4) and
-    // the stage states u2, u3 (4).
-    // FP64 b form: c_n (stages 2-3) and b (4); its stage 2 (class 3) also
-    // writes b to A.acc.
+    // Exact FP64 form: c_n (stages 2-4) and the accumulator (2-4); FP64 b
+    // form: c_n (2-3) and b (4), its stage 2 (class 3) also writing b to
+    // A.acc; FP32: c_n (2-4) and the stage states u2, u3 (4).
     constexpr bool BF = kBForm<T, EXACT>;
     const int st = SC == 3 ? 2 : (SC ? SC : A.stage);
     const bool wb = BF && (SC ? SC == 3 : A.stage == 2);
