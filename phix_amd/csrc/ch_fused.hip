@@ -863,6 +863,10 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, NT))) void k_ch_stage_3
         asm volatile("" : "+v"(off));
         return *(const T*)((const char*)base + off);
     };
+    auto at_nt = [](const T* base, unsigned off) {   // a last use: no cache allocation
+        asm volatile("" : "+v"(off));
+        return __builtin_nontemporal_load((const T*)((const char*)base + off));
+    };
     auto zplane = [&](int pz, bool& fix, T& val) -> const T* {
         const int qz = map_in(pz, A.bz);
         fix = qz < 0;
@@ -939,8 +943,12 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, NT))) void k_ch_stage_3
         for (int j = 0; j < NO; ++j) {
             const long long pj = FULL ? pb + j * rowe : pb;
             const unsigned oj = FULL ? off0 : ooff[j];
-            rc[j] = at(csrc + (pj & cpm), oj & com);
-            ra[j] = at(asrc + (pj & apm), oj & aom);
+            // b form: the last uses of c_n (stage 3, class 2) and of b (stage 4)
+            // allocate no cache lines, leaving the MALL to the next stage's state
+            if constexpr (BF && SC == 2) rc[j] = at_nt(csrc + (pj & cpm), oj & com);
+            else rc[j] = at(csrc + (pj & cpm), oj & com);
+            if constexpr (BF && SC == 4) ra[j] = at_nt(asrc + (pj & apm), oj & aom);
+            else ra[j] = at(asrc + (pj & apm), oj & aom);
             if constexpr (!EXACT && !BF) rb[j] = at(bsrc + (pj & bpm), oj & bom);
         }
     };
@@ -1059,7 +1067,9 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, NT))) void k_ch_stage_3
                 } else if (st < 4) {
                     const T v = axpy<T, false>(A.a, k, P.ec[j]);
                     out = v;
-                    if (wb) acc = rk4_b<T>(uc, v, P.ec[j]);
+                    // b is read two launches later: streamed past the caches, so
+                    // that u3 stays in the MALL for stage 3
+                    if (wb) __builtin_nontemporal_store(rk4_b<T>(uc, v, P.ec[j]), &acc);
                 } else {
                     out = BF ? rk4_fin_b<T>(A.dt6, A.third, P.ea[j], uc, k)
                              : rk4_fin<T>(A.dt6, A.third, P.ec[j], P.ea[j], P.eb[j], uc, k);
